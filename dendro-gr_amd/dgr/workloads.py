@@ -1,0 +1,95 @@
+"""Block tables for the synthetic workloads the path is measured and tested on.
+
+A block table is a numpy structured array with the layout of the C-ABI's
+``dgr_block`` (offset, padded size, face flags, coordinates of the padded
+corner points), i.e. the arguments BSSN_GR/src/rhs.cpp:49-70 derives per
+ot::Block before calling bssnrhs().
+
+Workloads (points are interior points, summed over blocks):
+  single_block_19   one 19^3 block, dx = 0.1, corner (0.25, 0.05, -0.15)
+  uniform_13        N blocks of 13^3 (7^3 interior), dx = 0.1*(1 + b mod 3)
+  puncture_octree   uneven 55^3/31^3/19^3/13^3 blocks placed by distance to two
+                    punctures at (+-3,0,0), counts 150/1000/8000/60000 x scale
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BLOCK_DTYPE = np.dtype([
+    ("offset", "<u8"), ("sz", "<u4", (3,)), ("bflag", "<u4"),
+    ("pmin", "<f8", (3,)), ("pmax", "<f8", (3,)),
+])
+
+PW = 3  # padding width (BSSN_GR/src/parameters.cpp:15-16)
+
+
+def _table(sizes, pmins, hs, bflags=None, align=1):
+    nb = len(sizes)
+    t = np.zeros(nb, dtype=BLOCK_DTYPE)
+    off = 0
+    for b in range(nb):
+        n = int(sizes[b])
+        t["offset"][b] = off
+        off += ((n ** 3 + align - 1) // align) * align
+        t["sz"][b] = (n, n, n)
+        t["bflag"][b] = 0 if bflags is None else int(bflags[b])
+        pm = np.asarray(pmins[b], dtype=np.float64)
+        t["pmin"][b] = pm
+        t["pmax"][b] = pm + (n - 1) * float(hs[b])
+    return t
+
+
+def total_padded(t: np.ndarray) -> int:
+    """Length each variable array needs (max offset + block size)."""
+    if len(t) == 0:
+        return 0
+    ends = t["offset"].astype(np.int64) + np.prod(t["sz"].astype(np.int64), axis=1)
+    return int(ends.max())
+
+
+def interior_points(t: np.ndarray) -> int:
+    return int(np.prod(t["sz"].astype(np.int64) - 2 * PW, axis=1).sum())
+
+
+def single_block_19(bflag: int = 0) -> np.ndarray:
+    return _table([19], [(0.25, 0.05, -0.15)], [0.1], [bflag])
+
+
+def uniform_13(nblocks: int = 4096, align: int = 1) -> np.ndarray:
+    b = np.arange(nblocks)
+    pmins = np.stack([0.25 + 4.0 * (b % 16), 0.05 + 4.0 * ((b // 16) % 16),
+                      -0.15 + 4.0 * (b // 256)], axis=1)
+    hs = 0.1 * (1 + b % 3)
+    return _table([13] * nblocks, pmins, hs, align=align)
+
+
+PUNCTURE_CLASSES = ((55, 150, 0.02), (31, 1000, 0.04), (19, 8000, 0.08), (13, 60000, 0.16))
+
+
+def puncture_octree(scale: float = 0.42, align: int = 1) -> np.ndarray:
+    """Uneven blocks: unit cells ordered by distance to the nearest puncture get
+    55^3 (nearest), then 31^3, 19^3, 13^3 blocks; dx halves toward the punctures so
+    every block spans ~0.96 in each direction. Blocks are emitted in cell order
+    (sizes interleaved, as an SFC-ordered octree would hand them over)."""
+    counts = [max(1, int(round(c * scale))) for _, c, _ in PUNCTURE_CLASSES]
+    total = sum(counts)
+    R = int(np.ceil((total * 3.0) ** (1.0 / 3.0) / 2.0)) + 4
+    ax = np.arange(-R, R)
+    cz, cy, cx = np.meshgrid(ax, ax, ax, indexing="ij")
+    cells = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1).astype(np.float64)
+    centers = cells + 0.5
+    d1 = np.linalg.norm(centers - np.array([3.0, 0.0, 0.0]), axis=1)
+    d2 = np.linalg.norm(centers - np.array([-3.0, 0.0, 0.0]), axis=1)
+    order = np.argsort(np.minimum(d1, d2), kind="stable")[:total]
+    cls = np.repeat(np.arange(4), counts)
+    chosen = cells[order]
+    # emit in cell (z, y, x) order so sizes interleave
+    emit = np.lexsort((chosen[:, 0], chosen[:, 1], chosen[:, 2]))
+    shift = np.array([0.0123, 0.0234, 0.0345])
+    sizes, pmins, hs = [], [], []
+    for e in emit:
+        n, _, h = PUNCTURE_CLASSES[cls[e]]
+        sizes.append(n)
+        hs.append(h)
+        pmins.append(chosen[e] + shift + 0.02 - PW * h)
+    return _table(sizes, pmins, hs, align=align)

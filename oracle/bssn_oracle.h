@@ -1,0 +1,123 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * Plain-C CPU restatement of Dendro-GR's per-block BSSN right-hand side, used
+ * as the checker for the HIP path. Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; nothing in the product links it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the Dendro-GR checkout root).
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   orc_deriv644_* / orc_ko_deriv42_*   closed forms: exactness on polynomials
+ *                                       and brute force vs textbook stencil
+ *                                       weights derived independently
+ *                                       (tests/test_oracle_stencils.py)
+ *   orc_bssn_point                      golden vectors from the reference's own
+ *                                       generated equations compiled in place
+ *                                       (oracle/_ref, tests/golden/)
+ *   orc_bssnrhs_block                   golden block outputs (reference algebra
+ *                                       + these stencils), Minkowski => 0,
+ *                                       axis-permutation covariance
+ *   orc_enforce_constraints             det(gt)=1, tr(At)=0, idempotence
+ *   orc_rk4_step                        exact on polynomial-in-t RHS (closed
+ *                                       form), fixture of one step
+ */
+#ifndef DGR_BSSN_ORACLE_H
+#define DGR_BSSN_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Variable order: BSSN_GR/include/grDef.h:35-60 (VAR enum). */
+enum {
+    ORC_ALPHA = 0, ORC_CHI = 1, ORC_K = 2,
+    ORC_GT0 = 3, ORC_GT1 = 4, ORC_GT2 = 5,          /* Gt^i (conformal connection) */
+    ORC_BETA0 = 6, ORC_BETA1 = 7, ORC_BETA2 = 8,
+    ORC_B0 = 9, ORC_B1 = 10, ORC_B2 = 11,
+    ORC_SYMGT0 = 12,                                /* gt_ij: 00 01 02 11 12 22 */
+    ORC_SYMAT0 = 18,                                /* At_ij: 00 01 02 11 12 22 */
+    ORC_NVARS = 24
+};
+
+/* Face bits of bflag: OCT_DIR_{LEFT,RIGHT,DOWN,UP,BACK,FRONT} = 0..5
+ * (usage BSSN_GR/src/derivs.cpp:2654,2676; values are Dendro's TreeNode.h). */
+enum { ORC_DIR_LEFT = 0, ORC_DIR_RIGHT, ORC_DIR_DOWN, ORC_DIR_UP, ORC_DIR_BACK, ORC_DIR_FRONT };
+
+/* Parameters the path reads (BSSN_GR/src/parameters.cpp:36-42,123-136). */
+typedef struct orc_params {
+    double lambda[4];      /* BSSN_LAMBDA (unsigned ints in the reference) */
+    double lambda_f[2];    /* BSSN_LAMBDA_F */
+    double eta_central;    /* RIT_ETA_CENTRAL */
+    double eta_outer;      /* RIT_ETA_OUTER */
+    double eta_width;      /* RIT_ETA_WIDTH */
+    double ko_sigma;       /* KO_DISS_SIGMA */
+    double chi_floor;      /* CHI_FLOOR */
+    int32_t cako_enabled;  /* BSSN_CAKO_ENABLED */
+    int32_t pad_;
+    double eps_cako_gauge; /* BSSN_EPSILON_CAKO_GAUGE */
+    double eps_cako_other; /* BSSN_EPSILON_CAKO_OTHER */
+} orc_params;
+
+/* One padded block, the arguments bssnrhs() receives (BSSN_GR/src/rhs.cpp:49-70). */
+typedef struct orc_block {
+    uint64_t offset;       /* start of the block inside each variable array */
+    uint32_t sz[3];        /* padded allocation size nx, ny, nz */
+    uint32_t bflag;        /* face bits */
+    double pmin[3];        /* coordinate of padded point (0,0,0) */
+    double pmax[3];        /* coordinate of padded point (nx-1,ny-1,nz-1) */
+} orc_block;
+
+/* Per-point algebra input, flat layout of ORC_POINT_NIN doubles:
+ *   [0,24)    fields in VAR order
+ *   [24,96)   first derivatives, index 24 + 3*v + d   (grad_d_v)
+ *   [96,162)  second derivatives of the 11 vars alpha, chi, beta0..2, gt0..5
+ *             (in that order), components 00 01 02 11 12 22: 96 + 6*m + c
+ *   [162]     eta                                                          */
+enum { ORC_POINT_NIN = 163 };
+
+/* ---- stencils: BSSN_GR/src/derivs.cpp ---------------------------------- */
+void orc_deriv644_x(double *Du, const double *u, double dx, const uint32_t *sz, uint32_t bflag);
+void orc_deriv644_y(double *Du, const double *u, double dy, const uint32_t *sz, uint32_t bflag);
+void orc_deriv644_z(double *Du, const double *u, double dz, const uint32_t *sz, uint32_t bflag);
+void orc_deriv644_xx(double *Du, const double *u, double dx, const uint32_t *sz, uint32_t bflag);
+void orc_deriv644_yy(double *Du, const double *u, double dy, const uint32_t *sz, uint32_t bflag);
+void orc_deriv644_zz(double *Du, const double *u, double dz, const uint32_t *sz, uint32_t bflag);
+void orc_ko_deriv42_x(double *Du, const double *u, double dx, const uint32_t *sz, uint32_t bflag);
+void orc_ko_deriv42_y(double *Du, const double *u, double dy, const uint32_t *sz, uint32_t bflag);
+void orc_ko_deriv42_z(double *Du, const double *u, double dz, const uint32_t *sz, uint32_t bflag);
+
+/* ---- algebra ------------------------------------------------------------ */
+void orc_default_params(orc_params *p);
+void orc_bssn_point(const double *in, const orc_params *p, double *rhs24);
+void orc_bssn_points(const double *in, int64_t npts, const orc_params *p, double *rhs);
+/* Replace the algebra orc_bssnrhs_block evaluates per point (NULL restores
+ * orc_bssn_point). Process-global: fixture generation only, single-threaded. */
+typedef void (*orc_point_fn)(const double *in, const orc_params *p, double *rhs24);
+void orc_set_point_algebra(orc_point_fn fn);
+
+/* ---- per-block RHS (rhs.cpp:80-504) and batch (rhs.cpp:10-73) ----------- */
+int orc_bssnrhs_block(double *const *rhs, const double *const *u, const orc_block *blk,
+                      const orc_params *p);
+int orc_bssnrhs_batch(double *const *rhs, const double *const *u, const orc_block *blks,
+                      int64_t nblocks, const orc_params *p, int nthreads);
+
+/* ---- constraints / RK4 -------------------------------------------------- */
+int orc_enforce_constraints_point(double *const *u, uint64_t node, double chi_floor);
+int orc_rk4_step(double *const *u, const orc_block *blks, int64_t nblocks, double dt,
+                 const orc_params *p, int enforce_per_stage, int nthreads);
+
+/* ---- synthetic inputs --------------------------------------------------- */
+uint64_t orc_splitmix64(uint64_t *state);
+void orc_init_perturbed_minkowski(double *const *u, const orc_block *blks, int64_t nblocks,
+                                  uint64_t seed, double amp);
+void orc_init_brill_lindquist(double *const *u, const orc_block *blks, int64_t nblocks,
+                              uint64_t seed, double noise, double chi_floor);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

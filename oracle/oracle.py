@@ -1,0 +1,239 @@
+"""ORACLE — test infrastructure only.
+
+ctypes loader for the CPU restatement (liboracle.so) and, where it was built,
+the reference's generated equations compiled in place (_ref/libdgr_ref.so).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NIN = 163  # per-point algebra input width (fields, grad, grad2, eta)
+NVARS = 24
+
+VAR_NAMES = [
+    "alpha", "chi", "K", "Gt0", "Gt1", "Gt2", "beta0", "beta1", "beta2", "B0", "B1", "B2",
+    "gt0", "gt1", "gt2", "gt3", "gt4", "gt5", "At0", "At1", "At2", "At3", "At4", "At5",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("lambda_", C.c_double * 4),
+        ("lambda_f", C.c_double * 2),
+        ("eta_central", C.c_double),
+        ("eta_outer", C.c_double),
+        ("eta_width", C.c_double),
+        ("ko_sigma", C.c_double),
+        ("chi_floor", C.c_double),
+        ("cako_enabled", C.c_int32),
+        ("pad_", C.c_int32),
+        ("eps_cako_gauge", C.c_double),
+        ("eps_cako_other", C.c_double),
+    ]
+
+
+class Block(C.Structure):
+    _fields_ = [
+        ("offset", C.c_uint64),
+        ("sz", C.c_uint32 * 3),
+        ("bflag", C.c_uint32),
+        ("pmin", C.c_double * 3),
+        ("pmax", C.c_double * 3),
+    ]
+
+
+BLOCK_DTYPE = np.dtype([
+    ("offset", "<u8"), ("sz", "<u4", (3,)), ("bflag", "<u4"),
+    ("pmin", "<f8", (3,)), ("pmax", "<f8", (3,)),
+])
+assert BLOCK_DTYPE.itemsize == C.sizeof(Block) == 72
+
+_lib = None
+_ref = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        L.orc_default_params.argtypes = [C.POINTER(Params)]
+        L.orc_bssn_points.argtypes = [dp, C.c_int64, C.POINTER(Params), dp]
+        L.orc_bssnrhs_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                        C.POINTER(Params), C.c_int]
+        L.orc_bssnrhs_batch.restype = C.c_int
+        L.orc_rk4_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
+                                   C.POINTER(Params), C.c_int, C.c_int]
+        L.orc_rk4_step.restype = C.c_int
+        L.orc_enforce_constraints_point.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
+        L.orc_enforce_constraints_point.restype = C.c_int
+        for nm in ("orc_deriv644_x", "orc_deriv644_y", "orc_deriv644_z", "orc_deriv644_xx",
+                   "orc_deriv644_yy", "orc_deriv644_zz", "orc_ko_deriv42_x", "orc_ko_deriv42_y",
+                   "orc_ko_deriv42_z"):
+            f = getattr(L, nm)
+            f.argtypes = [dp, dp, C.c_double, C.POINTER(C.c_uint32), C.c_uint32]
+        L.orc_init_perturbed_minkowski.argtypes = [C.c_void_p, C.c_void_p, C.c_int64,
+                                                   C.c_uint64, C.c_double]
+        L.orc_init_brill_lindquist.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
+                                               C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's generated equations compiled in place, or None where absent."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libdgr_ref.so")
+        if not os.path.exists(path):
+            return None
+        R = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        R.ref_bssn_points.argtypes = [dp, C.c_int64, dp, dp, dp]
+        _ref = R
+    return _ref
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().orc_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "lambda_":
+            for i in range(4):
+                p.lambda_[i] = v[i]
+        elif k == "lambda_f":
+            p.lambda_f[0], p.lambda_f[1] = v
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def bssn_points(inp: np.ndarray, p: Params) -> np.ndarray:
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    n = inp.shape[0]
+    out = np.zeros((n, 24))
+    lib().orc_bssn_points(_dp(inp), n, C.byref(p), _dp(out))
+    return out
+
+
+def ref_bssn_points(inp: np.ndarray, lam, lamf) -> np.ndarray:
+    R = ref_lib()
+    assert R is not None, "oracle/_ref not built (needs the reference checkout)"
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    n = inp.shape[0]
+    out = np.zeros((n, 24))
+    la = np.asarray(lam, dtype=np.float64)
+    lf = np.asarray(lamf, dtype=np.float64)
+    R.ref_bssn_points(_dp(inp), n, _dp(la), _dp(lf), _dp(out))
+    return out
+
+
+def stencil(name: str, u: np.ndarray, h: float, sz, bflag: int) -> np.ndarray:
+    """Apply one derivative routine to a (nz,ny,nx) array; untouched points stay NaN."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.full(u.shape, np.nan)
+    szc = (C.c_uint32 * 3)(*sz)
+    getattr(lib(), "orc_" + name)(_dp(out), _dp(u), h, szc, bflag)
+    return out
+
+
+def make_blocks(n_or_sizes, offsets=None, pmin=None, h=None, bflag=None) -> np.ndarray:
+    """Block table (BLOCK_DTYPE). sizes are padded cube edges."""
+    sizes = list(n_or_sizes)
+    nb = len(sizes)
+    blk = np.zeros(nb, dtype=BLOCK_DTYPE)
+    off = 0
+    for b, n in enumerate(sizes):
+        blk["offset"][b] = off if offsets is None else offsets[b]
+        off += n ** 3
+        blk["sz"][b] = (n, n, n)
+        blk["bflag"][b] = 0 if bflag is None else bflag[b]
+        pm = np.zeros(3) if pmin is None else np.asarray(pmin[b], dtype=np.float64)
+        hb = 0.1 if h is None else h[b]
+        blk["pmin"][b] = pm
+        blk["pmax"][b] = pm + (n - 1) * hb
+    return blk
+
+
+def total_points(blocks: np.ndarray) -> int:
+    return int(max(int(b["offset"]) + int(np.prod(b["sz"])) for b in blocks))
+
+
+def _ptrs(vars_: list[np.ndarray]):
+    arr = (C.c_void_p * 24)(*[v.ctypes.data for v in vars_])
+    return arr
+
+
+def state_views(state: np.ndarray) -> list[np.ndarray]:
+    """state: (24, N) contiguous; returns the 24 row views."""
+    assert state.shape[0] == 24 and state.flags.c_contiguous
+    return [state[v] for v in range(24)]
+
+
+def init_perturbed_minkowski(state: np.ndarray, blocks: np.ndarray, seed: int, amp: float = 1e-3):
+    blocks = np.ascontiguousarray(blocks)
+    lib().orc_init_perturbed_minkowski(_ptrs(state_views(state)), blocks.ctypes.data,
+                                       len(blocks), seed, amp)
+
+
+def init_brill_lindquist(state: np.ndarray, blocks: np.ndarray, seed: int, noise: float = 1e-6,
+                         chi_floor: float = 0.1):
+    blocks = np.ascontiguousarray(blocks)
+    lib().orc_init_brill_lindquist(_ptrs(state_views(state)), blocks.ctypes.data, len(blocks),
+                                   seed, noise, chi_floor)
+
+
+def bssnrhs_batch(state: np.ndarray, blocks: np.ndarray, p: Params, nthreads: int = 0,
+                  rhs: np.ndarray | None = None) -> np.ndarray:
+    if rhs is None:
+        rhs = np.zeros_like(state)
+    blocks = np.ascontiguousarray(blocks)
+    err = lib().orc_bssnrhs_batch(_ptrs(state_views(rhs)), _ptrs(state_views(state)),
+                                  blocks.ctypes.data, len(blocks), C.byref(p), nthreads)
+    if err != 0:
+        raise RuntimeError("orc_bssnrhs_batch failed")
+    return rhs
+
+
+def rk4_step(state: np.ndarray, blocks: np.ndarray, dt: float, p: Params,
+             enforce_per_stage: bool = True, nthreads: int = 0) -> None:
+    """In place: interior points advance one RK4 step, padding stays."""
+    blocks = np.ascontiguousarray(blocks)
+    err = lib().orc_rk4_step(_ptrs(state_views(state)), blocks.ctypes.data, len(blocks), dt,
+                             C.byref(p), int(enforce_per_stage), nthreads)
+    if err != 0:
+        raise RuntimeError("orc_rk4_step failed (constraint violation)")
+
+
+def enforce_point(state: np.ndarray, node: int, chi_floor: float) -> int:
+    return lib().orc_enforce_constraints_point(_ptrs(state_views(state)), node, chi_floor)
+
+
+def interior_mask(blocks: np.ndarray, npts: int) -> np.ndarray:
+    m = np.zeros(npts, dtype=bool)
+    for b in blocks:
+        nx, ny, nz = (int(s) for s in b["sz"])
+        idx = np.zeros((nz, ny, nx), dtype=bool)
+        idx[3:nz - 3, 3:ny - 3, 3:nx - 3] = True
+        off = int(b["offset"])
+        m[off:off + nx * ny * nz] = idx.ravel()
+    return m

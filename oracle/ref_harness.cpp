@@ -1,0 +1,81 @@
+// ORACLE — test infrastructure only.
+//
+// Harness that compiles the reference's own generated BSSN equations
+// (BSSN_GR/src/bssneqs_eta_const_standard_gauge.cpp, included in place from the
+// reference checkout, never copied) inside a point loop, so the fixtures pin the
+// oracle's algebra to the reference's arithmetic. It supplies exactly the names
+// the generated fragment reads, as BSSN_GR/src/rhs.cpp:84-156 does:
+// per-variable arrays, grad_/grad2_ arrays (scripts/bssnrhs_evar_derivs.h),
+// lambda[4] (unsigned, rhs.cpp:154), lambda_f[2] and a per-point eta.
+//
+// The reference's derivative stencils (BSSN_GR/src/derivs.cpp) are not built
+// here: derivs.h includes dendrolib's TreeNode.h, absent from the checkout.
+//
+// Input layout per point: the oracle's ORC_POINT_NIN = 163 doubles
+//   [0,24) fields (VAR order), [24,96) grad_d_v at 24+3v+d,
+//   [96,162) grad2 of alpha, chi, beta0..2, gt0..5 (00 01 02 11 12 22), [162] eta.
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+using std::pow;
+using std::sqrt;
+using std::exp;
+
+namespace {
+enum { NIN = 163 };
+}
+
+extern "C" int ref_bssn_points(const double *in, int64_t npts, const double *lambda_d,
+                               const double *lambda_f_in, double *out) {
+    std::vector<double> soa((size_t)NIN * (size_t)npts);
+    std::vector<double> res((size_t)24 * (size_t)npts);
+    for (int64_t q = 0; q < npts; q++)
+        for (int c = 0; c < NIN; c++) soa[(size_t)c * npts + q] = in[(size_t)q * NIN + c];
+
+    const unsigned int lambda[4] = {(unsigned)lambda_d[0], (unsigned)lambda_d[1],
+                                    (unsigned)lambda_d[2], (unsigned)lambda_d[3]};
+    const double lambda_f[2] = {lambda_f_in[0], lambda_f_in[1]};
+
+#define FIELD(name, v) const double *const name = &soa[(size_t)(v)*npts];
+    FIELD(alpha, 0) FIELD(chi, 1) FIELD(K, 2) FIELD(Gt0, 3) FIELD(Gt1, 4) FIELD(Gt2, 5)
+    FIELD(beta0, 6) FIELD(beta1, 7) FIELD(beta2, 8) FIELD(B0, 9) FIELD(B1, 10) FIELD(B2, 11)
+    FIELD(gt0, 12) FIELD(gt1, 13) FIELD(gt2, 14) FIELD(gt3, 15) FIELD(gt4, 16) FIELD(gt5, 17)
+    FIELD(At0, 18) FIELD(At1, 19) FIELD(At2, 20) FIELD(At3, 21) FIELD(At4, 22) FIELD(At5, 23)
+#define GRAD(name, v)                                                     \
+    FIELD(grad_0_##name, 24 + 3 * (v) + 0)                                \
+    FIELD(grad_1_##name, 24 + 3 * (v) + 1) FIELD(grad_2_##name, 24 + 3 * (v) + 2)
+    GRAD(alpha, 0) GRAD(chi, 1) GRAD(K, 2) GRAD(Gt0, 3) GRAD(Gt1, 4) GRAD(Gt2, 5)
+    GRAD(beta0, 6) GRAD(beta1, 7) GRAD(beta2, 8) GRAD(B0, 9) GRAD(B1, 10) GRAD(B2, 11)
+    GRAD(gt0, 12) GRAD(gt1, 13) GRAD(gt2, 14) GRAD(gt3, 15) GRAD(gt4, 16) GRAD(gt5, 17)
+    GRAD(At0, 18) GRAD(At1, 19) GRAD(At2, 20) GRAD(At3, 21) GRAD(At4, 22) GRAD(At5, 23)
+#define GRAD2(name, m)                                                    \
+    FIELD(grad2_0_0_##name, 96 + 6 * (m) + 0) FIELD(grad2_0_1_##name, 96 + 6 * (m) + 1) \
+    FIELD(grad2_0_2_##name, 96 + 6 * (m) + 2) FIELD(grad2_1_1_##name, 96 + 6 * (m) + 3) \
+    FIELD(grad2_1_2_##name, 96 + 6 * (m) + 4) FIELD(grad2_2_2_##name, 96 + 6 * (m) + 5)
+    GRAD2(alpha, 0) GRAD2(chi, 1) GRAD2(beta0, 2) GRAD2(beta1, 3) GRAD2(beta2, 4)
+    GRAD2(gt0, 5) GRAD2(gt1, 6) GRAD2(gt2, 7) GRAD2(gt3, 8) GRAD2(gt4, 9) GRAD2(gt5, 10)
+    const double *const eta_arr = &soa[(size_t)162 * npts];
+#define OUT(name, v) double *const name = &res[(size_t)(v)*npts];
+    OUT(a_rhs, 0) OUT(chi_rhs, 1) OUT(K_rhs, 2) OUT(Gt_rhs0, 3) OUT(Gt_rhs1, 4) OUT(Gt_rhs2, 5)
+    OUT(b_rhs0, 6) OUT(b_rhs1, 7) OUT(b_rhs2, 8) OUT(B_rhs0, 9) OUT(B_rhs1, 10) OUT(B_rhs2, 11)
+    OUT(gt_rhs00, 12) OUT(gt_rhs01, 13) OUT(gt_rhs02, 14) OUT(gt_rhs11, 15) OUT(gt_rhs12, 16)
+    OUT(gt_rhs22, 17) OUT(At_rhs00, 18) OUT(At_rhs01, 19) OUT(At_rhs02, 20) OUT(At_rhs11, 21)
+    OUT(At_rhs12, 22) OUT(At_rhs22, 23)
+
+    for (int64_t pp = 0; pp < npts; pp++) {
+        const double eta = eta_arr[pp];
+#include "bssneqs_eta_const_standard_gauge.cpp"
+    }
+
+    for (int64_t q = 0; q < npts; q++)
+        for (int v = 0; v < 24; v++) out[(size_t)q * 24 + v] = res[(size_t)v * npts + q];
+    return 0;
+}
+
+// Single point with the oracle's parameter struct, whose first six doubles are
+// lambda[4] and lambda_f[2]; usable as the oracle's orc_point_fn hook.
+extern "C" void ref_bssn_point_orc(const double *in, const void *params, double *out) {
+    const double *p = static_cast<const double *>(params);
+    ref_bssn_points(in, 1, p, p + 4, out);
+}

@@ -1,0 +1,119 @@
+"""Writes the golden fixtures in this directory.
+
+Expected outputs come from the reference's own generated BSSN equations
+(BSSN_GR/src/bssneqs_eta_const_standard_gauge.cpp) compiled in place by
+oracle/Makefile into oracle/_ref/libdgr_ref.so. For whole blocks the same
+reference equations are evaluated per point on derivatives from the oracle's
+stencils (the reference's derivs.cpp cannot be built here: it needs dendrolib's
+TreeNode.h), with the oracle's boundary, Kreiss-Oliger and RK4 glue.
+
+Run where the reference checkout exists:  python tests/golden/make_golden.py
+Fixtures:
+  algebra_points.npz  200 pointwise inputs (163 doubles each) and the reference
+                      equations' 24 outputs for two parameter sets
+  block19.npz         one 19^3 block (corner (0.25,0.05,-0.15), dx 0.1, seeded
+                      perturbed Minkowski, amplitude 1e-3, seed 1, KO sigma 0.4,
+                      eta 2): interior RHS for bflag 0 and 63, and the interior
+                      after one RK4 step (dt 0.025, constraints enforced per stage)
+  uniform13_8.npz     the first 8 blocks of the uniform 13^3 batch, seed 2:
+                      interior RHS (KO sigma 0.4, eta 2)
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "dendro-gr_amd"))
+
+import oracle as O  # noqa: E402
+from dgr import workloads as W  # noqa: E402
+
+PARAM_SETS = {"A": ((1, 1, 1, 1), (1.0, 0.0)), "B": ((2, 3, 5, 7), (0.7, 0.3))}
+
+
+def algebra_inputs(n=200, seed=12345):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1.0, 1.0, (n, O.NIN))
+    half = n // 2
+    # near-flat points
+    x[:half, :24] *= 0.2
+    x[:half, 24:162] *= 0.5
+    x[:half, 0] += 1.0
+    x[:half, 1] += 1.0
+    # puncture-like points: small chi and lapse, steep gradients
+    x[half:, 0] = rng.uniform(0.1, 0.6, n - half)
+    x[half:, 1] = rng.uniform(0.05, 0.3, n - half)
+    x[half:, 2:24] *= 0.3
+    x[half:, 24:162] *= 2.0
+    for c in (12, 15, 17):  # gt diagonal
+        x[:, c] += 1.0
+    x[:, 162] = rng.uniform(0.25, 2.0, n)
+    return x
+
+
+def set_ref_algebra(on: bool):
+    L = O.lib()
+    L.orc_set_point_algebra.argtypes = [C.c_void_p]
+    if on:
+        R = O.ref_lib()
+        L.orc_set_point_algebra(C.cast(R.ref_bssn_point_orc, C.c_void_p))
+    else:
+        L.orc_set_point_algebra(None)
+
+
+def interior(state, blocks):
+    out = []
+    for b in blocks:
+        n = int(b["sz"][0])
+        off = int(b["offset"])
+        blk = state[:, off:off + n ** 3].reshape(24, n, n, n)
+        out.append(blk[:, 3:n - 3, 3:n - 3, 3:n - 3])
+    return np.stack(out)
+
+
+def block_params():
+    return O.default_params(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
+
+
+def main():
+    assert O.ref_lib() is not None, "build oracle/_ref first (make -C oracle)"
+    inp = algebra_inputs()
+    outs = {k: O.ref_bssn_points(inp, lam, lf) for k, (lam, lf) in PARAM_SETS.items()}
+    np.savez_compressed(os.path.join(HERE, "algebra_points.npz"), inputs=inp,
+                        out_A=outs["A"], out_B=outs["B"])
+
+    p = block_params()
+    set_ref_algebra(True)
+    try:
+        res = {}
+        for bflag in (0, 63):
+            t = W.single_block_19(bflag)
+            st = np.zeros((24, W.total_padded(t)))
+            O.init_perturbed_minkowski(st, t, seed=1, amp=1e-3)
+            rhs = O.bssnrhs_batch(st, t, p, nthreads=1)
+            res[f"rhs_bflag{bflag}"] = interior(rhs, t)[0]
+            if bflag == 0:
+                res["input_sum"] = np.array([st.sum(), np.abs(st).sum()])
+                O.rk4_step(st, t, 0.025, p, enforce_per_stage=True, nthreads=1)
+                res["rk4_dt0.025"] = interior(st, t)[0]
+        np.savez_compressed(os.path.join(HERE, "block19.npz"), **res)
+
+        t = W.uniform_13(8)
+        st = np.zeros((24, W.total_padded(t)))
+        O.init_perturbed_minkowski(st, t, seed=2, amp=1e-3)
+        rhs = O.bssnrhs_batch(st, t, p, nthreads=1)
+        np.savez_compressed(os.path.join(HERE, "uniform13_8.npz"), rhs=interior(rhs, t),
+                            input_sum=np.array([st.sum(), np.abs(st).sum()]))
+    finally:
+        set_ref_algebra(False)
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,194 @@
+"""The oracle against the reference: golden vectors written from the
+reference's generated equations (tests/golden/make_golden.py), plus closed
+forms and invariants for the parts no fixture covers."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dgr import workloads as W
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def normwise(a, b, axis=None):
+    """max |a-b| / max |b| per variable (variables on axis 0 after moveaxis)."""
+    return np.abs(a - b).max(axis=axis) / np.maximum(np.abs(b).max(axis=axis), 1e-300)
+
+
+def block_params():
+    return O.default_params(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
+
+
+def interior(state, blocks):
+    out = []
+    for b in blocks:
+        n = int(b["sz"][0])
+        off = int(b["offset"])
+        out.append(state[:, off:off + n ** 3].reshape(24, n, n, n)[:, 3:n - 3, 3:n - 3, 3:n - 3])
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("pset", ["A", "B"])
+def test_algebra_matches_reference_vectors(pset):
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    lam, lf = {"A": ((1, 1, 1, 1), (1.0, 0.0)), "B": ((2, 3, 5, 7), (0.7, 0.3))}[pset]
+    got = O.bssn_points(g["inputs"], O.default_params(lambda_=lam, lambda_f=lf))
+    err = normwise(got, g[f"out_{pset}"], axis=0)
+    assert err.max() < 1e-13, err
+
+
+def test_algebra_fixture_still_matches_reference_build():
+    if O.ref_lib() is None:
+        pytest.skip("reference checkout not present")
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    out = O.ref_bssn_points(g["inputs"], (1, 1, 1, 1), (1.0, 0.0))
+    assert np.array_equal(out, g["out_A"])
+
+
+@pytest.mark.parametrize("bflag", [0, 63])
+def test_block19_matches_reference(bflag):
+    g = np.load(os.path.join(GOLD, "block19.npz"))
+    t = W.single_block_19(bflag)
+    st = np.zeros((24, W.total_padded(t)))
+    O.init_perturbed_minkowski(st, t, seed=1, amp=1e-3)
+    if bflag == 0:
+        np.testing.assert_allclose([st.sum(), np.abs(st).sum()], g["input_sum"], rtol=1e-14)
+    rhs = O.bssnrhs_batch(st, t, block_params(), nthreads=1)
+    got = interior(rhs, t)[0]
+    err = normwise(got.reshape(24, -1), g[f"rhs_bflag{bflag}"].reshape(24, -1), axis=1)
+    assert err.max() < 1e-12, err
+
+
+def test_block19_rk4_matches_reference():
+    g = np.load(os.path.join(GOLD, "block19.npz"))
+    t = W.single_block_19(0)
+    st = np.zeros((24, W.total_padded(t)))
+    O.init_perturbed_minkowski(st, t, seed=1, amp=1e-3)
+    u0 = interior(st, t)[0].copy()
+    O.rk4_step(st, t, 0.025, block_params(), enforce_per_stage=True, nthreads=1)
+    got = interior(st, t)[0]
+    ref = g["rk4_dt0.025"]
+    # compare the increments u1 - u0 normwise per variable
+    err = normwise((got - u0).reshape(24, -1), (ref - u0).reshape(24, -1), axis=1)
+    assert err.max() < 1e-10, err
+
+
+def test_uniform13_matches_reference():
+    g = np.load(os.path.join(GOLD, "uniform13_8.npz"))
+    t = W.uniform_13(8)
+    st = np.zeros((24, W.total_padded(t)))
+    O.init_perturbed_minkowski(st, t, seed=2, amp=1e-3)
+    np.testing.assert_allclose([st.sum(), np.abs(st).sum()], g["input_sum"], rtol=1e-14)
+    rhs = O.bssnrhs_batch(st, t, block_params(), nthreads=2)
+    got = interior(rhs, t)
+    err = normwise(np.moveaxis(got, 1, 0).reshape(24, -1), np.moveaxis(g["rhs"], 1, 0).reshape(24, -1), axis=1)
+    assert err.max() < 1e-12, err
+
+
+@pytest.mark.parametrize("bflag", [0, 63])
+def test_minkowski_rhs_is_exactly_zero(bflag):
+    t = W.single_block_19(bflag)
+    st = np.zeros((24, W.total_padded(t)))
+    O.init_perturbed_minkowski(st, t, seed=3, amp=0.0)
+    rhs = O.bssnrhs_batch(st, t, block_params(), nthreads=1)
+    assert np.count_nonzero(interior(rhs, t)) == 0
+
+
+SWAP_XY_VARS = list(range(24))
+SWAP_XY_VARS[3], SWAP_XY_VARS[4] = 4, 3      # Gt^0 <-> Gt^1
+SWAP_XY_VARS[6], SWAP_XY_VARS[7] = 7, 6      # beta
+SWAP_XY_VARS[9], SWAP_XY_VARS[10] = 10, 9    # B
+for base in (12, 18):                         # gt, At: 00<->11, 02<->12
+    SWAP_XY_VARS[base + 0], SWAP_XY_VARS[base + 3] = base + 3, base + 0
+    SWAP_XY_VARS[base + 2], SWAP_XY_VARS[base + 4] = base + 4, base + 2
+
+
+@pytest.mark.parametrize("bflag", [0, 1 | 8 | 16])
+def test_axis_swap_covariance(bflag):
+    """Relabelling x<->y (fields, tensor components, coordinates, face bits)
+    relabels the RHS the same way."""
+    n = 13
+    t = W.single_block_19(bflag)
+    t["sz"][0] = (n, n, n)
+    t["pmax"][0] = t["pmin"][0] + (n - 1) * 0.1
+    st = np.zeros((24, n ** 3))
+    O.init_perturbed_minkowski(st, t, seed=5, amp=1e-2)
+    p = block_params()
+    rhs = O.bssnrhs_batch(st, t, p, nthreads=1)
+
+    def swap(a):
+        a = a.reshape(24, n, n, n)[SWAP_XY_VARS]
+        return np.ascontiguousarray(np.swapaxes(a, 2, 3)).reshape(24, -1)
+
+    t2 = t.copy()
+    t2["pmin"][0] = t["pmin"][0][[1, 0, 2]]
+    t2["pmax"][0] = t["pmax"][0][[1, 0, 2]]
+    fb = int(bflag)
+    t2["bflag"][0] = (fb & 0b110000) | ((fb & 0b11) << 2) | ((fb >> 2) & 0b11)
+    rhs2 = O.bssnrhs_batch(swap(st), t2, p, nthreads=1)
+    a = interior(swap(rhs), t)[0].reshape(24, -1)
+    b = interior(rhs2, t2)[0].reshape(24, -1)
+    assert normwise(a, b, axis=1).max() < 1e-12
+
+
+def test_rk4_closed_form_linear_gauge_mode():
+    """Flat space with a uniform B: dbeta/dt = 3/4 B, dB/dt = -eta B, all other
+    right-hand sides vanish, so one RK4 step multiplies B by the RK4 stability
+    polynomial R(z), z = -eta dt, and adds 3/4 B0 dt (1 + z/2 + z^2/6 + z^3/24)
+    to beta (include/rkBSSN.h:50-53). The padding stays frozen at the initial
+    state, and each stage's stencils carry that jump at most 3 points inward, so
+    only points more than 12 points from the padding are checked."""
+    n = 31
+    t = W.single_block_19(0)
+    t["sz"][0] = (n, n, n)
+    t["pmax"][0] = t["pmin"][0] + (n - 1) * 0.1
+    st = np.zeros((24, W.total_padded(t)))
+    O.init_perturbed_minkowski(st, t, seed=0, amp=0.0)
+    B0 = np.array([0.3, -0.2, 0.1])
+    for i in range(3):
+        st[9 + i] = B0[i]
+    eta, dt = 2.0, 0.05
+    p = O.default_params(ko_sigma=0.4, eta_central=eta, eta_outer=eta)
+    O.rk4_step(st, t, dt, p, enforce_per_stage=True, nthreads=1)
+    z = -eta * dt
+    R = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    S = 1 + z / 2 + z ** 2 / 6 + z ** 3 / 24
+    I = st.reshape(24, n, n, n)[:, 15:16, 15:16, 15:16]
+    far = st.reshape(24, n, n, n)[:, 12:n - 12, 12:n - 12, 12:n - 12]
+    for i in range(3):
+        np.testing.assert_allclose(far[9 + i], B0[i] * R, rtol=1e-14)
+        np.testing.assert_allclose(far[6 + i], 0.75 * B0[i] * dt * S, rtol=1e-13)
+    assert I.shape[1] == 1
+    # and the frozen-padding jump did reach the points next to the padding
+    assert abs(st.reshape(24, n, n, n)[9, 3, 3, 3] - B0[0] * R) > 1e-6
+
+
+def test_enforce_constraints_projects_and_is_idempotent():
+    rng = np.random.default_rng(3)
+    st = np.zeros((24, 1))
+    g = np.eye(3) + 0.1 * rng.uniform(-1, 1, (3, 3))
+    g = 1.3 * (g + g.T) / 2
+    A = rng.uniform(-1, 1, (3, 3))
+    A = (A + A.T) / 2
+    comp = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for c, (i, j) in enumerate(comp):
+        st[12 + c, 0] = g[i, j]
+        st[18 + c, 0] = A[i, j]
+    st[0, 0], st[1, 0] = 0.05, 0.02
+    assert O.enforce_point(st, 0, 0.1) == 0
+    g2 = np.array([[st[12 + comp.index((min(i, j), max(i, j))), 0] for j in range(3)] for i in range(3)])
+    A2 = np.array([[st[18 + comp.index((min(i, j), max(i, j))), 0] for j in range(3)] for i in range(3)])
+    assert abs(np.linalg.det(g2) - 1.0) < 1e-14
+    assert abs(np.trace(np.linalg.inv(g2) @ A2)) < 1e-14
+    assert st[0, 0] == 0.1 and st[1, 0] == 0.1
+    before = st.copy()
+    assert O.enforce_point(st, 0, 0.1) == 0
+    np.testing.assert_allclose(st, before, rtol=1e-14, atol=1e-15)
+
+
+def test_enforce_constraints_rejects_negative_determinant():
+    st = np.zeros((24, 1))
+    st[12, 0], st[15, 0], st[17, 0] = 1.0, 1.0, -1.0
+    assert O.enforce_point(st, 0, 0.1) == -1
