@@ -1,0 +1,56 @@
+"""In-tree build of the native libraries (no cmake/ninja needed).
+
+  lib/libdgr_bssn.so          HIP kernels + C-ABI + C++ shims, gfx950
+  lib/libdgr_algebra_host.so  host build of the generated equations, used by
+                              the CPU tests to check the generated code itself
+
+Usage: python build.py [--regen]   (--regen reruns the equation generator)
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("DGR_OFFLOAD_ARCH", "gfx950")
+# Packed FP32 instructions are disabled for all device code.
+HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def regenerate(verbose=False):
+    _run([sys.executable, os.path.join(HERE, "codegen", "gen_bssn_rhs.py"),
+          os.path.join(HERE, "csrc", "bssn_rhs_generated.h")], verbose)
+
+
+def build(verbose: bool = False) -> None:
+    os.makedirs(LIB, exist_ok=True)
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc")]
+    out = os.path.join(LIB, "libdgr_bssn.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", *inc,
+          os.path.join(HERE, "csrc", "dgr_bssn.hip"),
+          os.path.join(HERE, "host", "bssn_shim.cpp"),
+          "-o", tmp], verbose)
+    os.replace(tmp, out)
+    host = os.path.join(LIB, "libdgr_algebra_host.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", *inc,
+          os.path.join(HERE, "csrc", "algebra_host.cpp"), "-o", host + ".tmp"], verbose)
+    os.replace(host + ".tmp", host)
+
+
+if __name__ == "__main__":
+    if "--regen" in sys.argv:
+        regenerate(verbose=True)
+    build(verbose=True)

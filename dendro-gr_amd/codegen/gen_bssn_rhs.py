@@ -1,0 +1,384 @@
+"""Generator for the pointwise BSSN right-hand side used by the HIP kernels.
+
+Restates the equations of the reference generator (CodeGen/bssn.py:78-218,
+standard gauge, constant eta damping, unstaged) with the tensor helpers of
+CodeGen/dendro.py:173-550. Unlike the reference, which expands every equation
+fully before eliminating common subexpressions, the equations are built
+through named tensor intermediates (inverse metric, Christoffel symbols,
+Ricci tensor, raised At, ...), which stay atoms; common subexpressions are then
+eliminated around them and all statements are ordered by a register-pressure
+list scheduler. The result keeps the live set small enough for FP64 on a
+64-wide wavefront.
+
+The emitted function reads inputs through an accessor object
+(`in.template fld<V>()`, `in.template d1<V, D>()`, `in.template d2<M, C>()`)
+and hands each right-hand side to `out.template put<V>(value)`, so a kernel
+decides where inputs come from and what happens to each output.
+
+Usage: python gen_bssn_rhs.py [--cse=local|global] [output.h]
+(local: common subexpressions are shared within one equation or intermediate
+only, which bounds the live set; global: shared across all of them.)
+"""
+from __future__ import annotations
+
+import sys
+import time
+
+import sympy as sp
+from sympy import Rational
+from sympy.printing.c import C99CodePrinter
+from sympy.printing.precedence import PRECEDENCE
+
+# ---------------------------------------------------------------------------
+# variables (BSSN_GR/include/grDef.h:35-60 order)
+VARS = ["alpha", "chi", "K", "Gt0", "Gt1", "Gt2", "beta0", "beta1", "beta2", "B0", "B1", "B2",
+        "gt0", "gt1", "gt2", "gt3", "gt4", "gt5", "At0", "At1", "At2", "At3", "At4", "At5"]
+VIDX = {n: i for i, n in enumerate(VARS)}
+D2VARS = ["alpha", "chi", "beta0", "beta1", "beta2", "gt0", "gt1", "gt2", "gt3", "gt4", "gt5"]
+D2IDX = {n: i for i, n in enumerate(D2VARS)}
+SYM6 = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+PAIRS = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+E_I = [0, 1, 2]
+E_IJ = [(i, j) for i in E_I for j in E_I]
+
+fld = {n: sp.Symbol(f"f_{n}", real=True) for n in VARS}
+d1s = {(n, d): sp.Symbol(f"d1_{n}_{d}", real=True) for n in VARS for d in E_I}
+d2s = {(n, c): sp.Symbol(f"d2_{n}_{c}", real=True) for n in D2VARS for c in range(6)}
+INPUTS = set(fld.values()) | set(d1s.values()) | set(d2s.values())
+eta = sp.Symbol("eta", real=True)
+l1, l2, l3, l4 = sp.symbols("lam0 lam1 lam2 lam3", real=True)
+lf0, lf1 = sp.symbols("lamf0 lamf1", real=True)
+
+DEFS: list[tuple[sp.Symbol, sp.Expr]] = []  # named intermediates, in definition order
+
+
+def let(name, expr):
+    s = sp.Symbol(name, real=True)
+    DEFS.append((s, sp.sympify(expr)))
+    return s
+
+
+def d(i, name):
+    return d1s[(name, i)]
+
+
+def dd(i, j, name):
+    return d2s[(name, SYM6[min(i, j)][max(i, j)])]
+
+
+def sym_tensor(prefix, fn):
+    """Symmetric 3x3 of named intermediates: fn(i, j) for i <= j."""
+    s = {}
+    for i, j in PAIRS:
+        s[(i, j)] = s[(j, i)] = let(f"{prefix}{i}{j}", fn(i, j))
+    return sp.Matrix(3, 3, lambda i, j: s[(i, j)])
+
+
+a, chi, K = fld["alpha"], fld["chi"], fld["K"]
+Gt_n = [f"Gt{i}" for i in E_I]
+b_n = [f"beta{i}" for i in E_I]
+B_n = [f"B{i}" for i in E_I]
+gt_n = [[f"gt{SYM6[i][j]}" for j in E_I] for i in E_I]
+At_n = [[f"At{SYM6[i][j]}" for j in E_I] for i in E_I]
+b = [fld[n] for n in b_n]
+B = [fld[n] for n in B_n]
+gt = sp.Matrix(3, 3, lambda i, j: fld[gt_n[i][j]])
+At = sp.Matrix(3, 3, lambda i, j: fld[At_n[i][j]])
+
+# inverse conformal metric: adjugate over determinant (dendro.py:375-393)
+cof = {(0, 0): gt[1, 1] * gt[2, 2] - gt[1, 2] ** 2, (0, 1): gt[0, 2] * gt[1, 2] - gt[0, 1] * gt[2, 2],
+       (0, 2): gt[0, 1] * gt[1, 2] - gt[0, 2] * gt[1, 1], (1, 1): gt[0, 0] * gt[2, 2] - gt[0, 2] ** 2,
+       (1, 2): gt[0, 1] * gt[0, 2] - gt[0, 0] * gt[1, 2], (2, 2): gt[0, 0] * gt[1, 1] - gt[0, 1] ** 2}
+idet = let("idet", 1 / (gt[0, 0] * cof[(0, 0)] + gt[0, 1] * cof[(0, 1)] + gt[0, 2] * cof[(0, 2)]))
+igt = sym_tensor("igt", lambda i, j: cof[(i, j)] * idet)
+ichi = let("ichi", 1 / chi)
+
+# first Christoffel C1_kij = 1/2 (d_j g_ki + d_i g_kj - d_k g_ij) (dendro.py:418-421)
+C1 = {}
+for k in E_I:
+    for i, j in PAIRS:
+        C1[(k, i, j)] = C1[(k, j, i)] = let(
+            f"C1_{k}{i}{j}", sp.Float(0.5) * (d(j, gt_n[k][i]) + d(i, gt_n[k][j]) - d(k, gt_n[i][j])))
+# second Christoffel C2^i_jk = igt^il C1_ljk (dendro.py:445-446)
+C2 = {}
+for i in E_I:
+    for j, k in PAIRS:
+        C2[(i, j, k)] = C2[(i, k, j)] = let(f"C2_{i}{j}{k}", sum(igt[i, l] * C1[(l, j, k)] for l in E_I))
+# contracted connection (dendro.py:512)
+CalGt = [let(f"CalGt{i}", sum(igt[k, l] * C2[(i, k, l)] for k, l in E_IJ)) for i in E_I]
+
+
+def C3(i, j, k):  # physical-metric Christoffel (dendro.py:471-480)
+    return C2[(i, j, k)] - sp.Float(0.5) * ichi * (
+        (d(k, "chi") if i == j else 0) + (d(j, "chi") if i == k else 0)
+        - gt[j, k] * sum(igt[i, m] * d(m, "chi") for m in E_I))
+
+
+# D_i D_j alpha (dendro.py:181-183)
+DDa = sym_tensor("DDa", lambda i, j: dd(i, j, "alpha") - sum(C3(l, i, j) * d(l, "alpha") for l in E_I))
+
+# Ricci tensor R = Rt + Rphi (dendro.py:495-541); 1/chi written as ichi
+lap_chi = let("lapchi", sum(igt[k, l] * (dd(k, l, "chi") - Rational(3, 2) * ichi * d(k, "chi") * d(l, "chi"))
+                            for k, l in E_IJ) - sum(CalGt[m] * d(m, "chi") for m in E_I))
+
+
+def ricci(i, j):
+    rt = (-sp.Float(0.5) * sum(igt[l, m] * dd(l, m, gt_n[i][j]) for l, m in E_IJ)
+          + sp.Float(0.5) * sum(gt[k, i] * d(j, Gt_n[k]) + gt[k, j] * d(i, Gt_n[k]) for k in E_I)
+          + sp.Float(0.5) * sum(CalGt[k] * (C1[(i, j, k)] + C1[(j, i, k)]) for k in E_I)
+          + sum(igt[l, m] * (C2[(k, l, i)] * C1[(j, k, m)] + C2[(k, l, j)] * C1[(i, k, m)]
+                             + C2[(k, i, m)] * C1[(k, l, j)]) for k in E_I for l, m in E_IJ))
+    xrphi = (Rational(1, 2) * ichi * (dd(i, j, "chi") - sum(C2[(k, j, i)] * d(k, "chi") for k in E_I))
+             - Rational(1, 4) * ichi * ichi * d(i, "chi") * d(j, "chi"))
+    return rt + xrphi + Rational(1, 2) * ichi * gt[i, j] * lap_chi
+
+
+R = sym_tensor("R", ricci)
+# raised At (dendro.py:203-208)
+AtUU = sym_tensor("AtUU", lambda i, j: sum(igt[i, k] * igt[j, l] * At[k, l] for k, l in E_IJ))
+divb = let("divb", sum(d(j, b_n[j]) for j in E_I))
+
+
+def lie_scalar(f_name, weight=0):  # dendro.py:230-231
+    return sum(b[i] * d(i, f_name) for i in E_I) + weight * fld[f_name] * divb
+
+
+def lie_sym(names, M, weight):  # dendro.py:236-239
+    return sp.Matrix(3, 3, lambda i, j: sum(
+        b[k] * d(k, names[i][j]) + M[i, k] * d(j, b_n[k]) + M[k, j] * d(i, b_n[k]) for k in E_I)
+        + weight * M[i, j] * divb)
+
+
+weight = -Rational(2, 3)  # bssn.py:39
+
+a_rhs = l1 * lie_scalar("alpha") - 2 * a * K
+b_rhs = [Rational(3, 4) * (lf0 + lf1 * a) * B[i] + l2 * sum(b[j] * d(j, b_n[i]) for j in E_I)
+         for i in E_I]
+gt_rhs = lie_sym(gt_n, gt, weight) - 2 * a * At
+chi_rhs = lie_scalar("chi", weight) + Rational(2, 3) * (chi * a * K)
+X = sp.Matrix(3, 3, lambda i, j: a * R[i, j] - DDa[i, j])
+trX = let("trX", sum(igt[i, j] * X[i, j] for i, j in E_IJ))  # trace_free, dendro.py:352-365
+AikAkj = sp.Matrix(3, 3, lambda i, j: sum(At[i, k] * sum(igt[k, l] * At[l, j] for l in E_I)
+                                          for k in E_I))
+At_rhs = lie_sym(At_n, At, weight) + chi * sp.Matrix(3, 3, lambda i, j: X[i, j] - gt[i, j] * trX / 3) \
+    + a * (K * At - 2 * AikAkj)
+# K: laplacian of alpha with the physical inverse metric chi * igt (dendro.py:288-290)
+K_rhs = (sum(b[i] * d(i, "K") for i in E_I) - chi * sum(igt[i, j] * DDa[i, j] for i, j in E_IJ)
+         + a * (K * K / 3 + sum(At[i, j] * AtUU[i, j] for i, j in E_IJ)))
+Gt_rhs = [
+    sum(b[j] * d(j, Gt_n[i]) for j in E_I)
+    - sum(CalGt[j] * d(j, b_n[i]) for j in E_I)
+    + Rational(2, 3) * CalGt[i] * divb
+    + sum(igt[j, k] * dd(j, k, b_n[i]) + igt[i, j] * dd(j, k, b_n[k]) / 3 for j, k in E_IJ)
+    - sum(2 * AtUU[i, j] * d(j, "alpha") for j in E_I)
+    + sum(2 * a * C2[(i, j, k)] * AtUU[j, k] for j, k in E_IJ)
+    - sum(a * (3 * ichi * AtUU[i, j] * d(j, "chi") + Rational(4, 3) * igt[i, j] * d(j, "K"))
+          for j in E_I)
+    for i in E_I]
+Gt_sym = [let(f"Gtrhs{i}", Gt_rhs[i]) for i in E_I]
+B_rhs = [Gt_sym[i] - eta * B[i] + l3 * sum(b[j] * d(j, B_n[i]) for j in E_I)
+         - l4 * sum(b[j] * d(j, Gt_n[i]) for j in E_I) for i in E_I]
+
+OUTPUTS = [None] * 24
+OUTPUTS[VIDX["alpha"]] = a_rhs
+OUTPUTS[VIDX["chi"]] = chi_rhs
+OUTPUTS[VIDX["K"]] = K_rhs
+for i in E_I:
+    OUTPUTS[VIDX[f"Gt{i}"]] = Gt_sym[i]
+    OUTPUTS[VIDX[f"beta{i}"]] = b_rhs[i]
+    OUTPUTS[VIDX[f"B{i}"]] = B_rhs[i]
+for i, j in PAIRS:
+    OUTPUTS[VIDX["gt0"] + SYM6[i][j]] = gt_rhs[i, j]
+    OUTPUTS[VIDX["At0"] + SYM6[i][j]] = At_rhs[i, j]
+
+
+# ---------------------------------------------------------------------------
+# code emission
+
+def input_expr(sym: sp.Symbol) -> str:
+    n = sym.name
+    if n.startswith("f_"):
+        return f"in.template fld<{VIDX[n[2:]]}>()"
+    if n.startswith("d1_"):
+        var, dd_ = n[3:].rsplit("_", 1)
+        return f"in.template d1<{VIDX[var]}, {dd_}>()"
+    if n.startswith("d2_"):
+        var, c = n[3:].rsplit("_", 1)
+        return f"in.template d2<{D2IDX[var]}, {c}>()"
+    raise KeyError(n)
+
+
+PARAMS = {"eta": "eta", "lam0": "lam[0]", "lam1": "lam[1]", "lam2": "lam[2]", "lam3": "lam[3]",
+          "lamf0": "lamf[0]", "lamf1": "lamf[1]"}
+
+
+class Printer(C99CodePrinter):
+    def __init__(self, names):
+        super().__init__({"strict": False})
+        self.names = names
+
+    def _print_Symbol(self, s):
+        if s in self.names:
+            return self.names[s]
+        if s.name in PARAMS:
+            return PARAMS[s.name]
+        raise KeyError(s.name)
+
+    def _print_Rational(self, r):
+        return f"T({r.p}.0/{r.q}.0)" if r.q != 1 else f"T({r.p}.0)"
+
+    def _print_Integer(self, i):
+        return f"T({int(i)}.0)"
+
+    def _print_Float(self, f):
+        return f"T({float(f)!r})"
+
+    def _print_Pow(self, e):
+        base, ex = e.as_base_exp()
+        bs = self.parenthesize(base, PRECEDENCE["Pow"])
+        if ex == -1:
+            return f"(T(1.0)/{bs})"
+        if ex.is_Integer and 2 <= int(ex) <= 4:
+            return "(" + "*".join([bs] * int(ex)) + ")"
+        if ex.is_Integer and -4 <= int(ex) <= -2:
+            return "(T(1.0)/(" + "*".join([bs] * (-int(ex))) + "))"
+        raise ValueError(f"unsupported power {e}")
+
+
+def schedule(defs, outs):
+    """Register-pressure-aware order of all statements (input loads,
+    temporaries, output stores). Bottom-up list scheduling: starting from the
+    stores, repeatedly place (earliest-so-far) the ready statement that adds
+    the fewest values to the live set; a statement becomes ready once every
+    statement using its value is placed. Returns program order as a list of
+    ('in', sym) / ('tmp', sym) / ('out', v)."""
+    ops = {}
+
+    def operands(e):
+        return [("tmp", x) if x in defs else ("in", x)
+                for x in e.free_symbols if x in defs or x in INPUTS]
+
+    for s_, e in defs.items():
+        ops[("tmp", s_)] = operands(e)
+    for v, e in enumerate(outs):
+        ops[("out", v)] = operands(e)
+    for x in INPUTS:
+        ops[("in", x)] = []
+    users = {n: 0 for n in ops}
+    for n, o in ops.items():
+        for x in set(o):
+            users[x] += 1
+    ready = {n for n in ops if users[n] == 0 and n[0] != "in"}
+
+    def key(n):
+        return (n[0], n[1] if isinstance(n[1], int) else n[1].name)
+
+    live = set()
+    rev = []
+    while ready:
+        best, best_score = None, None
+        for n in ready:
+            new = sum(1 for x in set(ops[n]) if x not in live)
+            score = (new - (1 if n in live else 0), -len(ops[n]), key(n))
+            if best_score is None or score < best_score:
+                best, best_score = n, score
+        ready.discard(best)
+        rev.append(best)
+        live.discard(best)
+        for x in set(ops[best]):
+            live.add(x)
+            users[x] -= 1
+            if users[x] == 0:
+                ready.add(x)
+    return rev[::-1]
+
+
+def generate(path):
+    t0 = time.time()
+    outs = [sp.sympify(e) for e in OUTPUTS]
+    def_syms = [s for s, _ in DEFS]
+    exprs = [e for _, e in DEFS] + outs
+    if CSE_MODE == "global":
+        repl, reduced = sp.cse(exprs, symbols=sp.numbered_symbols("c"), optimizations="basic")
+    else:  # per expression: sharing across expressions only through named intermediates
+        ctr = sp.numbered_symbols("c")
+        repl, reduced = [], []
+        for e in exprs:
+            r, rd = sp.cse([e], symbols=ctr, optimizations="basic")
+            repl += r
+            reduced.append(rd[0])
+    defs = dict(repl)
+    for s, e in zip(def_syms, reduced[:len(DEFS)]):
+        defs[s] = e
+    outs_r = reduced[len(DEFS):]
+    ops_after = sum(sp.count_ops(e) for e in defs.values()) + sum(sp.count_ops(e) for e in outs_r)
+    order = schedule(defs, outs_r)
+
+    names, lines = {}, []
+    pr = Printer(names)
+    ntmp = 0
+    last_use = {}
+    for k, (kind, x) in enumerate(order):
+        e = defs[x] if kind == "tmp" else outs_r[x] if kind == "out" else None
+        if e is not None:
+            for y in e.free_symbols:
+                last_use[y] = k
+    live, max_live = set(), 0
+    for k, (kind, x) in enumerate(order):
+        if kind == "in":
+            nm = "i_" + x.name
+            names[x] = nm
+            lines.append(f"    const T {nm} = {input_expr(x)};")
+            live.add(x)
+        elif kind == "tmp":
+            nm = f"t{ntmp}"
+            ntmp += 1
+            lines.append(f"    const T {nm} = {pr.doprint(defs[x])};")
+            names[x] = nm
+            live.add(x)
+        else:
+            lines.append(f"    out.template put<{x}>({pr.doprint(outs_r[x])});  // {VARS[x]}")
+        max_live = max(max_live, len(live))
+        live = {y for y in live if last_use.get(y, -1) > k}
+    hdr = f"""// Generated by dendro-gr_amd/codegen/gen_bssn_rhs.py -- do not edit.
+//
+// Pointwise BSSN right-hand side (standard gauge, constant eta damping): the
+// equations of CodeGen/bssn.py:78-218 with the tensor helpers of
+// CodeGen/dendro.py:173-550, built through named tensor intermediates with
+// common subexpressions eliminated around them.
+// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values
+// live at once in the emitted order.
+//
+// In: accessor with fld<V>() (24 fields, VAR order of grDef.h), d1<V, D>()
+//     (first derivatives), d2<M, C>() (second derivatives of alpha, chi,
+//     beta0..2, gt0..5; components 00 01 02 11 12 22).
+// Out: receives put<V>(rhs) once per variable.
+#pragma once
+
+#ifndef DGR_HD
+#define DGR_HD
+#endif
+
+namespace dgr {{
+
+template <typename T, typename In, typename Out>
+DGR_HD inline void bssn_rhs_point(const In &in, const T eta, const T *lam, const T *lamf, Out &out) {{
+{chr(10).join(lines)}
+}}
+
+}}  // namespace dgr
+"""
+    with open(path, "w") as fh:
+        fh.write(hdr)
+    print(f"generated {path}: {ntmp} temporaries, ops {ops_after}, max live {max_live}, "
+          f"{time.time() - t0:.1f}s")
+
+
+CSE_MODE = "local"
+
+if __name__ == "__main__":
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    for a in sys.argv[1:]:
+        if a.startswith("--cse="):
+            CSE_MODE = a.split("=", 1)[1]
+    generate(args[0] if args else "bssn_rhs_generated.h")

@@ -1,0 +1,47 @@
+// Host build of the generated pointwise BSSN equations, for checking the
+// generated code itself on the CPU against the reference's golden vectors.
+// Not part of the device path. Input layout per point: 163 doubles
+// ([0,24) fields, [24,96) first derivatives 3*v+d, [96,162) second
+// derivatives 6*m+c of alpha, chi, beta0..2, gt0..5, [162] eta).
+#include <cstdint>
+
+#include "bssn_rhs_generated.h"
+
+namespace {
+template <typename T>
+struct In163 {
+    const double *p;
+    template <int V>
+    T fld() const { return (T)p[V]; }
+    template <int V, int D>
+    T d1() const { return (T)p[24 + 3 * V + D]; }
+    template <int M, int C>
+    T d2() const { return (T)p[96 + 6 * M + C]; }
+};
+template <typename T>
+struct Out24 {
+    double *o;
+    template <int V>
+    void put(T x) { o[V] = (double)x; }
+};
+template <typename T>
+void run(const double *in, int64_t n, const double *lam, const double *lamf, double *out) {
+    T l[4] = {(T)lam[0], (T)lam[1], (T)lam[2], (T)lam[3]};
+    T lf[2] = {(T)lamf[0], (T)lamf[1]};
+    for (int64_t q = 0; q < n; q++) {
+        In163<T> a{in + 163 * q};
+        Out24<T> b{out + 24 * q};
+        dgr::bssn_rhs_point<T>(a, (T)in[163 * q + 162], l, lf, b);
+    }
+}
+}  // namespace
+
+extern "C" void dgr_host_bssn_points(const double *in, int64_t n, const double *lam, const double *lamf,
+                                     double *out) {
+    run<double>(in, n, lam, lamf, out);
+}
+
+extern "C" void dgr_host_bssn_points_f32(const double *in, int64_t n, const double *lam, const double *lamf,
+                                         double *out) {
+    run<float>(in, n, lam, lamf, out);
+}

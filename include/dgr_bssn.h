@@ -1,0 +1,140 @@
+/*
+ * dgr_bssn.h — C-ABI of the MI355X BSSN right-hand side (Dendro-GR hot path).
+ *
+ * Everything here is plain C: device pointers are caller-owned `double*`
+ * (hipMalloc'd or any device allocation), streams are passed as `void*`
+ * (a hipStream_t; NULL = the null stream). All functions return 0 on success
+ * or a negative DGR_E* code; dgr_last_error() describes the last failure of
+ * the calling thread. Nothing here calls exit().
+ *
+ * Layout: 24 BSSN variables in the order of BSSN_GR/include/grDef.h:35-60
+ * (alpha, chi, K, Gt0-2, beta0-2, B0-2, gt00..gt22, At00..At22), each a
+ * separate device array of `szpdof` doubles holding the padded ("unzipped")
+ * blocks; block b occupies [offset_b, offset_b + nx*ny*nz) with point
+ * (i,j,k) at offset_b + i + nx*(j + ny*k) — the reference's unzip layout
+ * (BSSN_GR/src/rhs.cpp:84-132, 208).
+ */
+#ifndef DGR_BSSN_H
+#define DGR_BSSN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGR_NUM_VARS 24
+#define DGR_PADDING_WIDTH 3
+
+enum {
+    DGR_OK = 0,
+    DGR_E_INVALID = -1,  /* bad argument (sizes, offsets, null pointers) */
+    DGR_E_HIP = -2,      /* HIP runtime error */
+    DGR_E_NOMEM = -3,    /* device allocation failed */
+    DGR_E_CONSTRAINT = -4 /* det(gt) <= 0 met while enforcing constraints */
+};
+
+/* Face bits of bflag: OCT_DIR_LEFT/RIGHT (-x/+x) = bits 0/1, DOWN/UP (-y/+y) =
+ * bits 2/3, BACK/FRONT (-z/+z) = bits 4/5 (Dendro's OCT_DIR_* numbering, used as
+ * 1u << OCT_DIR_* at BSSN_GR/src/derivs.cpp:2654,2676 and rhs.cpp:542-629). */
+#define DGR_FACE_LEFT (1u << 0)
+#define DGR_FACE_RIGHT (1u << 1)
+#define DGR_FACE_DOWN (1u << 2)
+#define DGR_FACE_UP (1u << 3)
+#define DGR_FACE_BACK (1u << 4)
+#define DGR_FACE_FRONT (1u << 5)
+
+/* One padded block: exactly what bssnRHS() derives from ot::Block before it
+ * calls bssnrhs() (BSSN_GR/src/rhs.cpp:49-70): offset, allocation size,
+ * face flags and the coordinates of the first and last padded points.
+ * Grid spacing is (pmax - pmin)/(n - 1) per axis (rhs.cpp:144-146). */
+typedef struct dgr_block {
+    uint64_t offset;
+    uint32_t sz[3];
+    uint32_t bflag;
+    double pmin[3];
+    double pmax[3];
+} dgr_block;
+
+/* Parameters bssnrhs() reads from the bssn:: globals (BSSN_GR/src/rhs.cpp:
+ * 144-168, 209-215, 338-355; defaults BSSN_GR/src/parameters.cpp:36-42,
+ * 123-136). dgr_default_params() fills the reference defaults. */
+typedef struct dgr_bssn_params {
+    double lambda[4];       /* BSSN_LAMBDA */
+    double lambda_f[2];     /* BSSN_LAMBDA_F */
+    double eta_central;     /* RIT_ETA_CENTRAL */
+    double eta_outer;       /* RIT_ETA_OUTER */
+    double eta_width;       /* RIT_ETA_WIDTH */
+    double ko_sigma;        /* KO_DISS_SIGMA */
+    double chi_floor;       /* CHI_FLOOR */
+    int32_t cako_enabled;   /* BSSN_CAKO_ENABLED */
+    int32_t reserved;
+    double eps_cako_gauge;  /* BSSN_EPSILON_CAKO_GAUGE */
+    double eps_cako_other;  /* BSSN_EPSILON_CAKO_OTHER */
+} dgr_bssn_params;
+
+typedef struct dgr_plan dgr_plan;
+
+/* Reference defaults (BSSN_GR/src/parameters.cpp:36-42,123-136). */
+void dgr_default_params(dgr_bssn_params *p);
+
+/* Last error message of the calling thread ("" if none). */
+const char *dgr_last_error(void);
+
+/* Validates the block table (sizes >= 13 per axis, blocks inside [0, szpdof)),
+ * splits every block into tiles of at most 7^3 interior points, orders the
+ * work for the device and allocates the derivative workspace on `device`.
+ * Replaces the reference's global derivative workspace and its per-block
+ * loop setup (BSSN_GR/src/rhs.cpp:10-73, grUtils.cpp:2374-2396). The table is
+ * copied; the plan does not keep `blocks`. */
+int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint64_t szpdof,
+                    dgr_plan **out);
+int dgr_plan_destroy(dgr_plan *plan);
+
+/* Interior points (sum over blocks of (nx-6)(ny-6)(nz-6)) the plan covers. */
+int64_t dgr_plan_interior_points(const dgr_plan *plan);
+
+/* Kernel timing (no counterpart in the reference, whose profiler_t timers wrap
+ * the phases of bssnrhs, include/profile_params.h:15-53): when enabled, every
+ * launch is bracketed by HIP events on the caller's stream.
+ * dgr_plan_kernel_times waits for them and returns, per kernel kind
+ * (0 derivatives, 1 equations, 2 boundary faces), the summed milliseconds
+ * and the number of launches since the previous call. */
+int dgr_plan_set_timing(dgr_plan *plan, int enable);
+int dgr_plan_kernel_times(dgr_plan *plan, double *ms, int64_t *launches);
+
+/* Right-hand side of all blocks: for each block the work of bssnrhs()
+ * (BSSN_GR/src/rhs.cpp:80-504): 6th-order derivatives, the BSSN equations,
+ * the radiative boundary fix-up on flagged faces, Kreiss-Oliger dissipation.
+ * d_rhs[v] receives interior points only; its padding is not written.
+ * `t` is accepted for signature parity (the standard-gauge equations do not
+ * read it). Asynchronous on `stream`. Replaces bssnRHS() (rhs.h:28-30) and
+ * device::bssnrhs3 (cuda/include/bssn_kernels.cuh:1526-1538). */
+int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u, double t,
+                 const dgr_bssn_params *params, void *stream);
+
+/* One classic RK4 step (BSSN_GR/include/rkBSSN.h:50-53, src/rkBSSN.cpp:764-860)
+ * of every block, in place on d_u. Padding points keep their values (blocks are
+ * advanced without re-filling padding between stages). With
+ * enforce_per_stage != 0 the algebraic constraints are enforced on every stage
+ * state as rkBSSN.cpp:810-812 does; they are always enforced on the result
+ * (rkBSSN.cpp:858, bssnCtx.cpp:1174-1182). Uses plan-owned stage buffers
+ * (two states of szpdof doubles per variable, allocated on first use). */
+int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
+                 int enforce_per_stage, void *stream);
+
+/* enforce_bssn_constraints (BSSN_GR/include/bssn_constraints.h:15-172) on the
+ * interior points of every block: det(gt) = 1, tr(At) = 0, chi and alpha
+ * floored at chi_floor. */
+int dgr_enforce_constraints(dgr_plan *plan, double *const *d_u, double chi_floor, void *stream);
+
+/* Waits for `stream`, then reports and clears constraint violations met by
+ * dgr_rk4_step / dgr_enforce_constraints since the last call: DGR_E_CONSTRAINT
+ * for det(gt) < 0 or a projection off by more than 1e-6 (where the reference
+ * calls exit(0), bssn_constraints.h:42-47, 76-88, 121-132), else DGR_OK. */
+int dgr_plan_status(dgr_plan *plan, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
