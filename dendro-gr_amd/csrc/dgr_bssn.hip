@@ -232,6 +232,7 @@ struct RKArgs {
 // enforce_bssn_constraints on one point's values: bssn_constraints.h:15-172.
 // Order in v14: alpha, chi, gt00..gt22, At00..At22. Returns error bits.
 __device__ int enforce_point(double v14[14], double chi_floor) {
+#pragma clang fp contract(off)
     const double one_third = 1.0 / 3.0;
     double g00 = v14[2], g01 = v14[3], g02 = v14[4], g11 = v14[5], g12 = v14[6], g22 = v14[7];
     double A00 = v14[8], A01 = v14[9], A02 = v14[10], A11 = v14[11], A12 = v14[12], A22 = v14[13];
@@ -276,6 +277,7 @@ struct Epilogue {
 
     template <int V>
     __device__ __forceinline__ void store(double val) {
+#pragma clang fp contract(off)
         if constexpr (MODE == MODE_RHS) {
             rhs->p[V][idx] = val;
         } else {
@@ -320,10 +322,30 @@ struct PointOut : Epilogue<MODE> {
 
     template <int V>
     __device__ __forceinline__ void put(double val) {
+#pragma clang fp contract(off)
         const double ko = W[(int64_t)(SLOT_KO + V) * P + q];
         this->template store<V>(val + (is_gauge(V) ? sig_gauge : sig_other) * ko);
     }
 };
+
+// x = pmin + i*h and the RIT eta of rhs.cpp:202-215, rounded as the reference does.
+__device__ __forceinline__ double coord(double pmin, int i, double h) {
+#pragma clang fp contract(off)
+    return pmin + i * h;
+}
+
+__device__ __forceinline__ double rit_eta(double x, double y, double z, const RhsParams &prm) {
+#pragma clang fp contract(off)
+    const double r_coord = sqrt(x * x + y * y + z * z);
+    const double w = r_coord / prm.eta_width;
+    const double arg = -w * w * w * w;
+    return (prm.eta_central - prm.eta_outer) * exp(arg) + prm.eta_outer;
+}
+
+__device__ __forceinline__ double inv_radius(double x, double y, double z) {
+#pragma clang fp contract(off)
+    return 1.0 / sqrt(x * x + y * y + z * z);
+}
 
 struct PointGeom {
     int tile, i, j, k;
@@ -375,13 +397,10 @@ __global__ __launch_bounds__(NTHREADS) void k_rhs(VarPtrs u, VarPtrs rhs, RKArgs
     if (g.face) return;
 
     // coordinates and eta (rhs.cpp:202-215)
-    const double x = B.pmin[0] + g.i * B.h[0];
-    const double y = B.pmin[1] + g.j * B.h[1];
-    const double z = B.pmin[2] + g.k * B.h[2];
-    const double r_coord = sqrt(x * x + y * y + z * z);
-    const double w = r_coord / prm.eta_width;
-    const double arg = -w * w * w * w;
-    const double eta = (prm.eta_central - prm.eta_outer) * exp(arg) + prm.eta_outer;
+    const double x = coord(B.pmin[0], g.i, B.h[0]);
+    const double y = coord(B.pmin[1], g.j, B.h[1]);
+    const double z = coord(B.pmin[2], g.k, B.h[2]);
+    const double eta = rit_eta(x, y, z, prm);
 
     PointIn in{&u, g.idx, W, P, q};
     PointOut<MODE> out;
@@ -403,6 +422,7 @@ template <int MODE, int V>
 __device__ __forceinline__ void face_var(Epilogue<MODE> &ep, const VarPtrs &u, const double *W, int64_t P,
                                          int64_t q, double x, double y, double z, double inv_r, double sg,
                                          double so) {
+#pragma clang fp contract(off)
     const double f = u.p[V][ep.idx];
     const double dxf = W[(int64_t)(3 * V + 0) * P + q];
     const double dyf = W[(int64_t)(3 * V + 1) * P + q];
@@ -426,10 +446,10 @@ __global__ __launch_bounds__(NTHREADS) void k_face(VarPtrs u, VarPtrs rhs, RKArg
         const PointGeom g = locate(T, B, p);
         if (!g.face) continue;
         const int64_t q = (int64_t)T.q0 + p;
-        const double x = B.pmin[0] + g.i * B.h[0];
-        const double y = B.pmin[1] + g.j * B.h[1];
-        const double z = B.pmin[2] + g.k * B.h[2];
-        const double inv_r = 1.0 / sqrt(x * x + y * y + z * z);
+        const double x = coord(B.pmin[0], g.i, B.h[0]);
+        const double y = coord(B.pmin[1], g.j, B.h[1]);
+        const double z = coord(B.pmin[2], g.k, B.h[2]);
+        const double inv_r = inv_radius(x, y, z);
         Epilogue<MODE> ep;
         ep.rhs = &rhs;
         ep.rk = &rk;

@@ -3,7 +3,9 @@
 // the axis, `a` the sample's index along the axis inside its padded block of
 // extent `n`, `lo`/`hi` whether the block's low/high face on that axis is a
 // domain boundary. Each expression keeps the term order of the reference
-// stencil it restates so the sums round alike.
+// stencil it restates, and floating-point contraction is off, so the sums
+// round exactly as the reference's do (the right-hand sides of near-puncture
+// data are small differences of these sums, so FMA rounding would show).
 #pragma once
 
 namespace dgr {
@@ -12,6 +14,7 @@ template <typename T>
 struct Axis1 {  // deriv644 first derivative constants (derivs.cpp:2621-2623)
     T by12, by60;
     __device__ explicit Axis1(T h) {
+#pragma clang fp contract(off)
         const T ih = T(1.0) / h;
         by12 = ih / T(12.0);
         by60 = ih / T(60.0);
@@ -22,6 +25,7 @@ template <typename T>
 struct Axis2 {  // deriv644 second derivative constants (derivs.cpp:2950-2952)
     T by12, by180;
     __device__ explicit Axis2(T h) {
+#pragma clang fp contract(off)
         const T ih2 = T(1.0) / (h * h);
         by180 = ih2 / T(180.0);
         by12 = ih2 / T(12.0);
@@ -32,6 +36,7 @@ template <typename T>
 struct AxisKO {  // ko_deriv42 constants (derivs.cpp:966-972)
     T pre, smr3, smr2, smr1;
     __device__ explicit AxisKO(T h) {
+#pragma clang fp contract(off)
         pre = T(-1.0) / T(64.0) / h;
         smr3 = T(59.0) / T(48.0) * T(64) * h;
         smr2 = T(43.0) / T(48.0) * T(64) * h;
@@ -43,6 +48,7 @@ struct AxisKO {  // ko_deriv42 constants (derivs.cpp:966-972)
 template <typename T>
 __device__ __forceinline__ T d1(const T *s, int st, int a, int n, bool lo, bool hi,
                                 const Axis1<T> &c) {
+#pragma clang fp contract(off)
 #define U(o) s[(o) * st]
     const int ie = n - 3;
     if (lo && a < 6) {
@@ -63,6 +69,7 @@ __device__ __forceinline__ T d1(const T *s, int st, int a, int n, bool lo, bool 
 template <typename T>
 __device__ __forceinline__ T d2(const T *s, int st, int a, int n, bool lo, bool hi,
                                 const Axis2<T> &c) {
+#pragma clang fp contract(off)
 #define U(o) s[(o) * st]
     const int ie = n - 3;
     if (lo && a < 6) {
@@ -83,6 +90,7 @@ __device__ __forceinline__ T d2(const T *s, int st, int a, int n, bool lo, bool 
 template <typename T>
 __device__ __forceinline__ T ko(const T *s, int st, int a, int n, bool lo, bool hi,
                                 const AxisKO<T> &c) {
+#pragma clang fp contract(off)
 #define U(o) s[(o) * st]
     const int ie = n - 3;
     if (lo && a < 6) {
