@@ -147,12 +147,13 @@ def main():
     rhs_only = npts * nrhs / (time.perf_counter() - r0) / 1e6
 
     stages = 4 * args.steps
-    stage_ms = (kt["derivs"][0] + kt["rhs"][0] + kt["face"][0]) / stages
+    stage_ms = sum(v[0] for v in kt.values()) / stages
     alg_bytes = stage_bytes_per_point(13) * npts
     achieved = alg_bytes / (stage_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "RK stage = k_derivs + k_rhs over all chunks",
+                "kernel": ("k_fused (one launch per RK stage)" if kt["fused"][1] else
+                           "RK stage = k_derivs + k_rhs over all chunks"),
                 "stage_ms": round(stage_ms, 4),
                 "per_kernel_ms_per_stage": {k: round(v[0] / stages, 4) for k, v in kt.items()},
                 "launches_per_stage": {k: v[1] / stages for k, v in kt.items()},

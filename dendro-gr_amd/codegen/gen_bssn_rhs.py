@@ -15,7 +15,7 @@ The emitted function reads inputs through an accessor object
 and hands each right-hand side to `out.template put<V>(value)`, so a kernel
 decides where inputs come from and what happens to each output.
 
-Usage: python gen_bssn_rhs.py [--cse=local|global] [output.h]
+Usage: python gen_bssn_rhs.py [--cse=local|global] [--phased] [output.h]
 (local: common subexpressions are shared within one equation or intermediate
 only, which bounds the live set; global: shared across all of them.)
 """
@@ -293,8 +293,8 @@ def schedule(defs, outs):
     return rev[::-1]
 
 
-def generate(path):
-    t0 = time.time()
+def eliminate():
+    """Common-subexpression elimination; returns (definitions, outputs, op count)."""
     outs = [sp.sympify(e) for e in OUTPUTS]
     def_syms = [s for s, _ in DEFS]
     exprs = [e for _, e in DEFS] + outs
@@ -311,10 +311,28 @@ def generate(path):
     for s, e in zip(def_syms, reduced[:len(DEFS)]):
         defs[s] = e
     outs_r = reduced[len(DEFS):]
-    ops_after = sum(sp.count_ops(e) for e in defs.values()) + sum(sp.count_ops(e) for e in outs_r)
-    order = schedule(defs, outs_r)
+    ops = sum(sp.count_ops(e) for e in defs.values()) + sum(sp.count_ops(e) for e in outs_r)
+    return defs, outs_r, ops
 
+
+HEADER_TOP = """// Generated by dendro-gr_amd/codegen/gen_bssn_rhs.py -- do not edit.
+//
+// Pointwise BSSN right-hand side (standard gauge, constant eta damping): the
+// equations of CodeGen/bssn.py:78-218 with the tensor helpers of
+// CodeGen/dendro.py:173-550, built through named tensor intermediates with
+// common subexpressions eliminated around them.
+"""
+
+
+def emit(order, defs, outs_r, input_fn, phase_line=None, inline_fields=False, pin=False):
+    """C++ statements for a program order of ('in'|'tmp'|'out', x) nodes, with
+    optional ('phase', p) markers; returns (lines, temporaries, max live).
+    With inline_fields, every use of a field reads it again (the kernel keeps
+    fields in memory rather than in registers across phases)."""
     names, lines = {}, []
+    if inline_fields:
+        for x in fld.values():
+            names[x] = input_fn(x)
     pr = Printer(names)
     ntmp = 0
     last_use = {}
@@ -325,28 +343,39 @@ def generate(path):
                 last_use[y] = k
     live, max_live = set(), 0
     for k, (kind, x) in enumerate(order):
+        if kind == "phase":
+            lines.append(phase_line(x))
+            continue
         if kind == "in":
+            if inline_fields and x.name.startswith("f_"):
+                continue
             nm = "i_" + x.name
             names[x] = nm
-            lines.append(f"    const T {nm} = {input_expr(x)};")
+            lines.append(f"    const T {nm} = {input_fn(x)};")
             live.add(x)
         elif kind == "tmp":
             nm = f"t{ntmp}"
             ntmp += 1
-            lines.append(f"    const T {nm} = {pr.doprint(defs[x])};")
+            if pin:
+                lines.append(f"    T {nm} = {pr.doprint(defs[x])};")
+                lines.append(f"    DGR_PIN({nm});")
+            else:
+                lines.append(f"    const T {nm} = {pr.doprint(defs[x])};")
             names[x] = nm
             live.add(x)
         else:
             lines.append(f"    out.template put<{x}>({pr.doprint(outs_r[x])});  // {VARS[x]}")
         max_live = max(max_live, len(live))
         live = {y for y in live if last_use.get(y, -1) > k}
-    hdr = f"""// Generated by dendro-gr_amd/codegen/gen_bssn_rhs.py -- do not edit.
-//
-// Pointwise BSSN right-hand side (standard gauge, constant eta damping): the
-// equations of CodeGen/bssn.py:78-218 with the tensor helpers of
-// CodeGen/dendro.py:173-550, built through named tensor intermediates with
-// common subexpressions eliminated around them.
-// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values
+    return lines, ntmp, max_live
+
+
+def generate(path):
+    t0 = time.time()
+    defs, outs_r, ops_after = eliminate()
+    order = schedule(defs, outs_r)
+    lines, ntmp, max_live = emit(order, defs, outs_r, input_expr)
+    hdr = HEADER_TOP + f"""// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values
 // live at once in the emitted order.
 //
 // In: accessor with fld<V>() (24 fields, VAR order of grDef.h), d1<V, D>()
@@ -358,11 +387,17 @@ def generate(path):
 #ifndef DGR_HD
 #define DGR_HD
 #endif
+#ifndef DGR_INLINE
+#define DGR_INLINE inline
+#endif
+#ifndef DGR_PIN
+#define DGR_PIN(x) ((void)0)
+#endif
 
 namespace dgr {{
 
 template <typename T, typename In, typename Out>
-DGR_HD inline void bssn_rhs_point(const In &in, const T eta, const T *lam, const T *lamf, Out &out) {{
+DGR_HD DGR_INLINE void bssn_rhs_point(const In &in, const T eta, const T *lam, const T *lamf, Out &out) {{
 {chr(10).join(lines)}
 }}
 
@@ -374,6 +409,175 @@ DGR_HD inline void bssn_rhs_point(const In &in, const T eta, const T *lam, const
           f"{time.time() - t0:.1f}s")
 
 
+# Order in which the variables stream through the fused kernel's LDS.
+# Found by local search over orders for the smallest live set of the schedule.
+PHASE_ORDER = ["Gt2", "gt1", "Gt0", "chi", "Gt1", "gt3", "gt5", "gt4", "gt0", "gt2", "B2", "beta2",
+               "B0", "At5", "beta0", "beta1", "B1", "K", "alpha", "At0", "At4", "At1", "At3", "At2"]
+
+
+def _var_of(sym):
+    n = sym.name
+    if n.startswith("d1_") or n.startswith("d2_"):
+        return n[3:].rsplit("_", 1)[0]
+    return None
+
+
+def schedule_phased(defs, outs, order_vars):
+    """Statement order for a kernel that streams one variable at a time: the
+    derivatives of variable v exist only during phase P(v), so a statement can
+    run no earlier than the phase of its latest derivative input, and every
+    derivative input must be placed in its own phase. Outputs run after the last
+    phase (when every Kreiss-Oliger term is ready). Bottom-up over phases: place
+    the statements that must sit in the current phase, plus earlier-capable ones
+    that do not grow the live set, choosing each time the one adding the fewest
+    live values. Fields are re-read at use and not counted as live."""
+    P = {v: i for i, v in enumerate(order_vars)}
+    NP = len(order_vars)
+    nodes = {}
+
+    def operands(e):
+        return [("tmp", x) if x in defs else ("in", x)
+                for x in e.free_symbols if x in defs or x in INPUTS]
+
+    for s_, e in defs.items():
+        nodes[("tmp", s_)] = operands(e)
+    for v, e in enumerate(outs):
+        nodes[("out", v)] = operands(e)
+    used = {x for o in nodes.values() for x in o}
+    for x in INPUTS:
+        if ("in", x) in used:
+            nodes[("in", x)] = []
+    memo = {}
+
+    def ep(n):
+        if n not in memo:
+            if n[0] == "in":
+                v = _var_of(n[1])
+                memo[n] = P[v] if v else 0
+            else:
+                r = max([ep(x) for x in nodes[n]] + [0])
+                memo[n] = NP if n[0] == "out" else r
+        return memo[n]
+
+    for n in nodes:
+        ep(n)
+    users = {n: 0 for n in nodes}
+    for n, o in nodes.items():
+        for x in set(o):
+            users[x] += 1
+
+    def counted(x):
+        return not (x[0] == "in" and x[1].name.startswith("f_"))
+
+    live, rev = set(), []
+    ready = {n for n in nodes if users[n] == 0}
+    for p in range(NP, -1, -1):
+        while True:
+            best, bs = None, None
+            for n in ready:
+                if ep(n) > p:
+                    continue
+                if n[0] == "in" and _var_of(n[1]) and ep(n) != p:
+                    continue  # derivatives exist only during their own phase
+                new = sum(1 for x in set(nodes[n]) if x not in live and counted(x))
+                gain = new - (1 if n in live and counted(n) else 0)
+                if ep(n) < p and gain > 0:
+                    continue
+                sc = (gain, 0 if ep(n) == p else 1, -len(nodes[n]),
+                      n[0], n[1] if isinstance(n[1], int) else n[1].name)
+                if bs is None or sc < bs:
+                    best, bs = n, sc
+            if best is None:
+                break
+            ready.discard(best)
+            rev.append(best)
+            live.discard(best)
+            for x in set(nodes[best]):
+                live.add(x)
+                users[x] -= 1
+                if users[x] == 0:
+                    ready.add(x)
+        rev.append(("phase", p))
+        assert not [n for n in ready if ep(n) == p and not (n[0] == "in" and not _var_of(n[1]))]
+    assert not ready
+    prog = rev[::-1]
+    return prog
+
+
+def generate_phased(path):
+    t0 = time.time()
+    defs, outs_r, ops_after = eliminate()
+    prog = schedule_phased(defs, outs_r, PHASE_ORDER)
+    vidx = [VIDX[v] for v in PHASE_ORDER]
+
+    def phase_line(p):
+        if p == len(PHASE_ORDER):
+            return "    ctx.finish();"
+        nxt = vidx[p + 1] if p + 1 < len(vidx) else -1
+        return f"    ctx.template phase<{vidx[p]}, {nxt}>();  // {PHASE_ORDER[p]}"
+
+    def phased_input(x):
+        n = x.name
+        if n.startswith("f_"):
+            return f"ctx.template fld<{VIDX[n[2:]]}>()"
+        return input_expr(x).replace("in.template", "ctx.template")
+
+    lines, ntmp, _ = emit(prog, defs, outs_r, phased_input, phase_line, inline_fields=True, pin=True)
+    # live set without fields (they are re-read when used)
+    last_use, live, max_live = {}, set(), 0
+    for k, (kind, x) in enumerate(prog):
+        e = defs[x] if kind == "tmp" else outs_r[x] if kind == "out" else None
+        if e is not None:
+            for y in e.free_symbols:
+                last_use[y] = k
+    for k, (kind, x) in enumerate(prog):
+        if kind in ("in", "tmp") and not x.name.startswith("f_"):
+            live.add(x)
+        max_live = max(max_live, len(live))
+        live = {y for y in live if last_use.get(y, -1) > k}
+    hdr = HEADER_TOP + f"""//
+// Phased form for a kernel that streams the variables through LDS one at a
+// time, in the order {', '.join(PHASE_ORDER)}.
+// ctx.phase<V, NEXT>() makes variable V's derivatives at this thread's point
+// available (and may start fetching NEXT); ctx.d1<V, D>() / ctx.d2<M, C>() may
+// only be called in V's phase; ctx.fld<V>() may be called anywhere;
+// ctx.finish() ends the last phase. Every thread of a workgroup must execute
+// the whole function (the phases synchronize the workgroup). DGR_PIN(t) marks
+// where a temporary must be computed: a device build defines it to keep the
+// compiler from sinking a statement into a later phase (which would keep the
+// statement's inputs alive across phases).
+// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values
+// (fields excluded) live at once in the emitted order.
+#pragma once
+
+#ifndef DGR_HD
+#define DGR_HD
+#endif
+#ifndef DGR_INLINE
+#define DGR_INLINE inline
+#endif
+#ifndef DGR_PIN
+#define DGR_PIN(x) ((void)0)
+#endif
+
+namespace dgr {{
+
+// first variable of the phase order (the kernel fetches it before calling)
+constexpr int kBssnFirstPhaseVar = {vidx[0]};
+
+template <typename T, typename Ctx, typename Out>
+DGR_HD DGR_INLINE void bssn_rhs_phased(Ctx &ctx, const T eta, const T *lam, const T *lamf, Out &out) {{
+{chr(10).join(lines)}
+}}
+
+}}  // namespace dgr
+"""
+    with open(path, "w") as fh:
+        fh.write(hdr)
+    print(f"generated {path}: {ntmp} temporaries, ops {ops_after}, max live {max_live} (phased), "
+          f"{time.time() - t0:.1f}s")
+
+
 CSE_MODE = "local"
 
 if __name__ == "__main__":
@@ -381,4 +585,7 @@ if __name__ == "__main__":
     for a in sys.argv[1:]:
         if a.startswith("--cse="):
             CSE_MODE = a.split("=", 1)[1]
-    generate(args[0] if args else "bssn_rhs_generated.h")
+    if "--phased" in sys.argv:
+        generate_phased(args[0] if args else "bssn_rhs_phased.h")
+    else:
+        generate(args[0] if args else "bssn_rhs_generated.h")

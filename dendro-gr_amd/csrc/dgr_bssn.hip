@@ -30,6 +30,9 @@
 #include "dgr_bssn.h"
 
 #define DGR_HD __host__ __device__
+#define DGR_INLINE __forceinline__
+// keeps a value's computation where it is written (see bssn_rhs_phased.h)
+#define DGR_PIN(x) asm volatile("" : "+v"(x))
 #define DGR_GEN_SQRT
 template <typename T>
 __host__ __device__ inline T dgr_sqrt(T x) {
@@ -37,6 +40,7 @@ __host__ __device__ inline T dgr_sqrt(T x) {
 }
 #include "bssn_rhs_generated.h"
 #include "stencils.h"
+#include "bssn_rhs_phased.h"
 
 namespace {
 
@@ -55,6 +59,7 @@ struct VarPtrs {
 
 struct DBlock {  // device copy of a block with its grid spacing
     uint64_t offset;
+    int64_t q0;  // first interior point of the block in interior-indexed buffers
     int n[3];
     uint32_t bflag;
     double pmin[3];
@@ -461,6 +466,288 @@ __global__ __launch_bounds__(NTHREADS) void k_face(VarPtrs u, VarPtrs rhs, RKArg
     }
 }
 
+// -------------------------------------------------------------------------
+// k_fused: derivatives, equations, KO, BC and the stage update in one kernel.
+// One workgroup per tile; the 24 variables stream through LDS in the phase
+// order of bssn_rhs_phased.h: while the equations of phase v run, the next
+// variable's region is already in flight into registers. Each thread owns one
+// point (threads beyond the tile's points shadow the last point and store
+// nothing), computes its derivatives of the resident variable, and leaves that
+// variable's Kreiss-Oliger term (on flagged faces: boundary value plus KO term)
+// in LDS for the final stores.
+// -------------------------------------------------------------------------
+// One wave per SIMD: the equations need ~430 registers per lane (256 VGPRs
+// plus AGPRs used as spill space), which only a single wave per SIMD has.
+constexpr int FT = 256;                                   // threads per workgroup
+constexpr int FZMAX = 5;                                  // max tile extent along z
+constexpr int TPTS = FT;                                  // max points per tile (7 x 7 x 5 = 245)
+constexpr int RPTS = RMAX * RMAX * (FZMAX + 2 * PW);      // staged region points
+constexpr int DXPTS = TMAX * RMAX * (FZMAX + 2 * PW);     // D_x region
+constexpr int DYPTS = TMAX * TMAX * (FZMAX + 2 * PW);     // D_y region
+constexpr int PREF = (RPTS + FT - 1) / FT;                // prefetch registers per thread
+
+// Where a stage reads its state and writes its results. Stage buffers hold
+// interior points only, indexed by DBlock::q0 + (i-3) + mx*((j-3) + my*(k-3));
+// padding always comes from the caller's state `u`.
+struct StageIO {
+    VarPtrs u;     // caller's padded state (padding; interior at stage 0 and in RHS mode)
+    VarPtrs sin;   // interior-only stage input (stages 1..3)
+    VarPtrs sout;  // interior-only next-stage state (stages 0..2)
+    VarPtrs acc;   // interior-only accumulator
+    VarPtrs rhs;   // RHS mode: padded output
+    double cacc, cnext;
+    int stage;     // -1 in RHS mode
+    int enforce_stage;
+    int *err;
+};
+
+struct FusedCtx {
+    // tile geometry
+    int nx, ny, nz, rx, ry, nreg, ex, ey;
+    int gx0, gy0, gz0;
+    uint64_t boff;
+    int64_t bq0;
+    int mx, my;
+    bool lox, hix, loy, hiy, loz, hiz;
+    double h[3];
+    // this thread's point
+    int p, li, lj, lk, ax, ay, az;
+    bool active, face;
+    uint64_t idx;
+    int64_t q;
+    double x, y, z, inv_r, sig_gauge, sig_other;
+    // sources and LDS
+    const StageIO *io;
+    double *s_tile, *s_dx, *s_dy, *s_ko;
+    double pre[PREF];
+    double d1v[3], d2v[6];
+
+    __device__ __forceinline__ void fetch(int v) {
+        const bool from_stage = io->stage >= 1;
+        const double *__restrict__ up = io->u.p[v];
+        const double *__restrict__ sp_ = from_stage ? io->sin.p[v] : nullptr;
+#pragma unroll
+        for (int c = 0; c < PREF; c++) {
+            const int e = threadIdx.x + c * FT;
+            if (e < nreg) {
+                const int ii = e % rx, jj = (e / rx) % ry, kk = e / (rx * ry);
+                const int gi = gx0 + ii, gj = gy0 + jj, gk = gz0 + kk;
+                const bool inner = gi >= PW && gi < nx - PW && gj >= PW && gj < ny - PW && gk >= PW && gk < nz - PW;
+                if (from_stage && inner)
+                    pre[c] = sp_[bq0 + (gi - PW) + (int64_t)mx * ((gj - PW) + (int64_t)my * (gk - PW))];
+                else
+                    pre[c] = up[boff + (uint64_t)gi + (uint64_t)nx * ((uint64_t)gj + (uint64_t)ny * gk)];
+            }
+        }
+    }
+
+    __device__ __forceinline__ void stash() {
+#pragma unroll
+        for (int c = 0; c < PREF; c++) {
+            const int e = threadIdx.x + c * FT;
+            if (e < nreg) s_tile[e] = pre[c];
+        }
+    }
+
+    template <int V, int NEXT>
+    __device__ __forceinline__ void phase() {
+        __syncthreads();  // everyone is done with the previous variable's LDS data
+        stash();
+        if constexpr (NEXT >= 0) fetch(NEXT);
+        __syncthreads();
+        constexpr int m2 = d2slot(V);
+        const dgr::Axis1<double> c1x(h[0]), c1y(h[1]), c1z(h[2]);
+        const int ez = nreg / (rx * ry) - 2 * PW;
+        if constexpr (m2 >= 0) {
+            const int ndx = ex * ry * (ez + 2 * PW);
+            for (int e = threadIdx.x; e < ndx; e += FT) {
+                const int i0 = e % ex, jj = (e / ex) % ry, kk = e / (ex * ry);
+                s_dx[e] = dgr::d1(&s_tile[(PW + i0) + rx * (jj + ry * kk)], 1, gx0 + PW + i0, nx, lox, hix, c1x);
+            }
+            const int ndy = ex * ey * (ez + 2 * PW);
+            for (int e = threadIdx.x; e < ndy; e += FT) {
+                const int i0 = e % ex, j0 = (e / ex) % ey, kk = e / (ex * ey);
+                s_dy[e] = dgr::d1(&s_tile[(PW + i0) + rx * ((PW + j0) + ry * kk)], rx, gy0 + PW + j0, ny, loy, hiy, c1y);
+            }
+            __syncthreads();
+        }
+        const double *c = &s_tile[(PW + li) + rx * ((PW + lj) + ry * (PW + lk))];
+        if constexpr (m2 >= 0) {
+            const double *cx = &s_dx[li + ex * ((PW + lj) + ry * (PW + lk))];
+            const double *cy = &s_dy[li + ex * (lj + ey * (PW + lk))];
+            d1v[0] = cx[0];
+            d1v[1] = cy[0];
+            const dgr::Axis2<double> c2x(h[0]), c2y(h[1]), c2z(h[2]);
+            d2v[0] = dgr::d2(c, 1, ax, nx, lox, hix, c2x);
+            d2v[1] = dgr::d1(cx, ex, ay, ny, loy, hiy, c1y);        // D_y(D_x u)
+            d2v[2] = dgr::d1(cx, ex * ry, az, nz, loz, hiz, c1z);   // D_z(D_x u)
+            d2v[3] = dgr::d2(c, rx, ay, ny, loy, hiy, c2y);
+            d2v[4] = dgr::d1(cy, ex * ey, az, nz, loz, hiz, c1z);   // D_z(D_y u)
+            d2v[5] = dgr::d2(c, rx * ry, az, nz, loz, hiz, c2z);
+        } else {
+            d1v[0] = dgr::d1(c, 1, ax, nx, lox, hix, c1x);
+            d1v[1] = dgr::d1(c, rx, ay, ny, loy, hiy, c1y);
+        }
+        d1v[2] = dgr::d1(c, rx * ry, az, nz, loz, hiz, c1z);
+        // computed here, not at their last use (that would keep the LDS taps live)
+        DGR_PIN(d1v[0]);
+        DGR_PIN(d1v[1]);
+        DGR_PIN(d1v[2]);
+        if constexpr (m2 >= 0) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) DGR_PIN(d2v[k]);
+        }
+        ko_term<V>(c);
+    }
+
+    // KO term sigma*(ko_x+ko_y+ko_z) (rhs.cpp:434-489); on a flagged face the
+    // boundary value replaces the equations (bssn_bcs, rhs.cpp:511-631) and the
+    // KO term is added to it, as the reference adds KO after the boundary pass.
+    template <int V>
+    __device__ __forceinline__ void ko_term(const double *c) {
+#pragma clang fp contract(off)
+        const dgr::AxisKO<double> kx(h[0]), ky(h[1]), kz(h[2]);
+        const double kox = dgr::ko(c, 1, ax, nx, lox, hix, kx);
+        const double koy = dgr::ko(c, rx, ay, ny, loy, hiy, ky);
+        const double koz = dgr::ko(c, rx * ry, az, nz, loz, hiz, kz);
+        const double kt = (is_gauge(V) ? sig_gauge : sig_other) * (kox + koy + koz);
+        double v = kt;
+        if (face) {
+            const double bc = -inv_r * (x * d1v[0] + y * d1v[1] + z * d1v[2] + BC_FALLOFF[V] * (c[0] - BC_ASYM[V]));
+            v = bc + kt;
+        }
+        if (active) s_ko[V * TPTS + p] = v;
+    }
+
+    template <int V, int D>
+    __device__ __forceinline__ double d1() const { return d1v[D]; }
+    template <int M, int C>
+    __device__ __forceinline__ double d2() const { return d2v[C]; }
+    template <int V>
+    __device__ __forceinline__ double fld() const {
+        return io->stage >= 1 ? io->sin.p[V][q] : io->u.p[V][idx];
+    }
+    __device__ __forceinline__ void finish() {}
+};
+
+struct FusedOut {
+    const FusedCtx *ctx;
+    const StageIO *io;
+    double v14[14];
+
+    template <int V>
+    __device__ __forceinline__ void put(double val) {
+#pragma clang fp contract(off)
+        const double kb = ctx->s_ko[V * TPTS + ctx->p];
+        val = ctx->face ? kb : val + kb;
+        if (!ctx->active) return;
+        const uint64_t idx = ctx->idx;
+        const int64_t q = ctx->q;
+        const int s = io->stage;
+        if (s < 0) {
+            io->rhs.p[V][idx] = val;
+        } else if (s < 3) {
+            const double u0 = io->u.p[V][idx];
+            const double nxt = u0 + io->cnext * val;
+            io->acc.p[V][q] = (s == 0 ? u0 : io->acc.p[V][q]) + io->cacc * val;
+            if (slot14(V) >= 0 && io->enforce_stage) v14[slot14(V)] = nxt;
+            else io->sout.p[V][q] = nxt;
+        } else {
+            const double fin = io->acc.p[V][q] + io->cacc * val;
+            if (slot14(V) >= 0) v14[slot14(V)] = fin;
+            else io->u.p[V][idx] = fin;
+        }
+    }
+
+    __device__ __forceinline__ void finish(double chi_floor) {
+        const int s = io->stage;
+        if (!ctx->active || s < 0 || (s < 3 && !io->enforce_stage)) return;
+        const int e = enforce_point(v14, chi_floor);
+        if (e) atomicOr(io->err, e);
+        if (s == 3) {
+            const uint64_t idx = ctx->idx;
+            io->u.p[0][idx] = v14[0];
+            io->u.p[1][idx] = v14[1];
+#pragma unroll
+            for (int c = 0; c < 12; c++) io->u.p[12 + c][idx] = v14[2 + c];
+        } else {
+            const int64_t q = ctx->q;
+            io->sout.p[0][q] = v14[0];
+            io->sout.p[1][q] = v14[1];
+#pragma unroll
+            for (int c = 0; c < 12; c++) io->sout.p[12 + c][q] = v14[2 + c];
+        }
+    }
+};
+
+__global__ __launch_bounds__(FT) void k_fused(StageIO io, RhsParams prm, const DBlock *__restrict__ blocks,
+                                             const Tile *__restrict__ tiles) {
+    __shared__ double s_tile[RPTS];
+    __shared__ double s_dx[DXPTS];
+    __shared__ double s_dy[DYPTS];
+    __shared__ double s_ko[NV * TPTS];
+
+    const Tile T = tiles[blockIdx.x];
+    const DBlock B = blocks[T.block];
+    FusedCtx ctx;
+    ctx.io = &io;
+    ctx.s_tile = s_tile;
+    ctx.s_dx = s_dx;
+    ctx.s_dy = s_dy;
+    ctx.s_ko = s_ko;
+    ctx.nx = B.n[0];
+    ctx.ny = B.n[1];
+    ctx.nz = B.n[2];
+    ctx.ex = T.e[0];
+    ctx.ey = T.e[1];
+    ctx.rx = T.e[0] + 2 * PW;
+    ctx.ry = T.e[1] + 2 * PW;
+    ctx.nreg = ctx.rx * ctx.ry * (T.e[2] + 2 * PW);
+    ctx.gx0 = T.o[0] - PW;
+    ctx.gy0 = T.o[1] - PW;
+    ctx.gz0 = T.o[2] - PW;
+    ctx.boff = B.offset;
+    ctx.bq0 = B.q0;
+    ctx.mx = B.n[0] - 2 * PW;
+    ctx.my = B.n[1] - 2 * PW;
+    const uint32_t bf = B.bflag;
+    ctx.lox = bf & 1u; ctx.hix = bf & 2u; ctx.loy = bf & 4u; ctx.hiy = bf & 8u; ctx.loz = bf & 16u; ctx.hiz = bf & 32u;
+    ctx.h[0] = B.h[0];
+    ctx.h[1] = B.h[1];
+    ctx.h[2] = B.h[2];
+    const int npts = T.e[0] * T.e[1] * T.e[2];
+    ctx.active = (int)threadIdx.x < npts;
+    ctx.p = ctx.active ? (int)threadIdx.x : npts - 1;
+    ctx.li = ctx.p % T.e[0];
+    ctx.lj = (ctx.p / T.e[0]) % T.e[1];
+    ctx.lk = ctx.p / (T.e[0] * T.e[1]);
+    ctx.ax = T.o[0] + ctx.li;
+    ctx.ay = T.o[1] + ctx.lj;
+    ctx.az = T.o[2] + ctx.lk;
+    ctx.idx = B.offset + (uint64_t)ctx.ax + (uint64_t)ctx.nx * ((uint64_t)ctx.ay + (uint64_t)ctx.ny * ctx.az);
+    ctx.q = B.q0 + (ctx.ax - PW) + (int64_t)ctx.mx * ((ctx.ay - PW) + (int64_t)ctx.my * (ctx.az - PW));
+    ctx.face = (ctx.lox && ctx.ax == PW) || (ctx.hix && ctx.ax == ctx.nx - PW - 1) || (ctx.loy && ctx.ay == PW) ||
+               (ctx.hiy && ctx.ay == ctx.ny - PW - 1) || (ctx.loz && ctx.az == PW) || (ctx.hiz && ctx.az == ctx.nz - PW - 1);
+    ctx.x = coord(B.pmin[0], ctx.ax, B.h[0]);
+    ctx.y = coord(B.pmin[1], ctx.ay, B.h[1]);
+    ctx.z = coord(B.pmin[2], ctx.az, B.h[2]);
+    ctx.inv_r = inv_radius(ctx.x, ctx.y, ctx.z);
+    if (prm.cako) {
+        const double sg = sqrt(ctx.fld<1>());
+        ctx.sig_gauge = sg * prm.eps_gauge;
+        ctx.sig_other = sg * prm.eps_other;
+    } else {
+        ctx.sig_gauge = ctx.sig_other = prm.ko_sigma;
+    }
+    const double eta = rit_eta(ctx.x, ctx.y, ctx.z, prm);
+    ctx.fetch(dgr::kBssnFirstPhaseVar);
+    FusedOut out;
+    out.ctx = &ctx;
+    out.io = &io;
+    dgr::bssn_rhs_phased<double>(ctx, eta, prm.lam, prm.lamf, out);
+    out.finish(prm.chi_floor);
+}
+
 // enforce_bssn_constraints on interior points (standalone entry point)
 __global__ __launch_bounds__(NTHREADS) void k_enforce(VarPtrs u, const DBlock *__restrict__ blocks,
                                                      const Tile *__restrict__ tiles, const int *__restrict__ wg_tile,
@@ -513,12 +800,16 @@ struct dgr_plan {
     int *d_face_tiles = nullptr;
     double *d_W = nullptr;
     int64_t max_chunk_points = 0;
-    double *d_stage = nullptr;  // 2 * 24 * szpdof: stage state, accumulator
+    double *d_stage = nullptr;  // split path RK4: 2 * 24 * szpdof (stage state, accumulator)
+    bool fused = false;         // k_fused (DGR_KERNEL=fused) or the split k_derivs/k_rhs path (default)
+    Tile *d_ftiles = nullptr;   // k_fused tiles (<= 7 x 7 x 5 points), XCD-grouped order
+    int nftiles = 0;
+    double *d_istage = nullptr; // k_fused RK4: 3 * 24 * interior (two stage states, accumulator)
     int *d_err = nullptr;
     // optional per-kernel timing: event pairs recorded around each launch
     bool timing = false;
     std::vector<hipEvent_t> event_pool;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[3];
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[4];
 
     hipEvent_t take_event() {
         if (event_pool.empty()) {
@@ -550,6 +841,17 @@ void dgr_default_params(dgr_bssn_params *p) {
 }
 
 const char *dgr_last_error(void) { return g_err.c_str(); }
+
+// m interior points -> ceil(m/emax) near-equal parts of at most emax points
+static void split_axis_max(int m, int emax, std::vector<std::pair<int, int>> &parts) {
+    const int np = (m + emax - 1) / emax;
+    int o = PW;
+    for (int p = 0; p < np; p++) {
+        const int e = m / np + (p < m % np ? 1 : 0);
+        parts.push_back({o, e});
+        o += e;
+    }
+}
 
 static void split_axis(int m, std::vector<std::pair<int, int>> &parts) {
     // m interior points -> ceil(m/7) near-equal parts (each >= 4 when m >= 7)
@@ -644,23 +946,61 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
                     chunk_pts += (int64_t)t.e[0] * t.e[1] * t.e[2];
                 }
         cur.tile_end = (int)tiles.size();
+        hb[b].q0 = plan->interior;
         plan->interior += pts;
     }
     close_chunk();
+
+    // k_fused tiles: at most 7 x 7 x 5 points (one point per thread of a 256-thread
+    // workgroup). Blocks are taken eight at a time and their tiles interleaved, so
+    // that (with workgroups dealt round-robin over the 8 XCDs) the tiles of one
+    // block share an XCD and its L2 for their overlapping halos.
+    std::vector<Tile> ftiles;
+    {
+        std::vector<std::vector<Tile>> per(nblocks);
+        for (uint32_t b = 0; b < nblocks; b++) {
+            const DBlock &d = hb[b];
+            std::vector<std::pair<int, int>> px, py, pz;
+            split_axis_max(d.n[0] - 6, TMAX, px);
+            split_axis_max(d.n[1] - 6, TMAX, py);
+            split_axis_max(d.n[2] - 6, FZMAX, pz);
+            for (auto &z : pz)
+                for (auto &y : py)
+                    for (auto &x : px) {
+                        Tile t;
+                        t.block = (int)b;
+                        t.q0 = 0;
+                        t.o[0] = x.first; t.o[1] = y.first; t.o[2] = z.first;
+                        t.e[0] = x.second; t.e[1] = y.second; t.e[2] = z.second;
+                        per[b].push_back(t);
+                    }
+        }
+        for (uint32_t g = 0; g < nblocks; g += 8) {
+            const uint32_t ge = std::min<uint32_t>(nblocks, g + 8);
+            size_t mx = 0;
+            for (uint32_t b = g; b < ge; b++) mx = std::max(mx, per[b].size());
+            for (size_t t = 0; t < mx; t++)
+                for (uint32_t b = g; b < ge; b++)
+                    if (t < per[b].size()) ftiles.push_back(per[b][t]);
+        }
+    }
+    plan->nftiles = (int)ftiles.size();
+    if (const char *k = std::getenv("DGR_KERNEL")) plan->fused = std::string(k) == "fused";
 
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess && nblocks) e = hipMalloc(&plan->d_blocks, sizeof(DBlock) * hb.size());
     if (e == hipSuccess && !tiles.empty()) e = hipMalloc(&plan->d_tiles, sizeof(Tile) * tiles.size());
     if (e == hipSuccess && !wg_tile.empty()) e = hipMalloc(&plan->d_wg_tile, sizeof(int) * wg_tile.size());
     if (e == hipSuccess && !face_tiles.empty()) e = hipMalloc(&plan->d_face_tiles, sizeof(int) * face_tiles.size());
-    if (e == hipSuccess && plan->max_chunk_points)
-        e = hipMalloc(&plan->d_W, sizeof(double) * NSLOT * (size_t)plan->max_chunk_points);
+    if (e == hipSuccess && !ftiles.empty()) e = hipMalloc(&plan->d_ftiles, sizeof(Tile) * ftiles.size());
     if (e == hipSuccess) e = hipMalloc(&plan->d_err, sizeof(int));
     if (e == hipSuccess && nblocks) e = hipMemcpy(plan->d_blocks, hb.data(), sizeof(DBlock) * hb.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !tiles.empty()) e = hipMemcpy(plan->d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !wg_tile.empty()) e = hipMemcpy(plan->d_wg_tile, wg_tile.data(), sizeof(int) * wg_tile.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !face_tiles.empty())
         e = hipMemcpy(plan->d_face_tiles, face_tiles.data(), sizeof(int) * face_tiles.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !ftiles.empty())
+        e = hipMemcpy(plan->d_ftiles, ftiles.data(), sizeof(Tile) * ftiles.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(plan->d_err, 0, sizeof(int));
     if (e != hipSuccess) {
         dgr_plan_destroy(plan);
@@ -685,6 +1025,8 @@ int dgr_plan_destroy(dgr_plan *plan) {
     hipFree(plan->d_face_tiles);
     hipFree(plan->d_W);
     hipFree(plan->d_stage);
+    hipFree(plan->d_ftiles);
+    hipFree(plan->d_istage);
     hipFree(plan->d_err);
     delete plan;
     return DGR_OK;
@@ -701,7 +1043,7 @@ int dgr_plan_set_timing(dgr_plan *plan, int enable) {
 int dgr_plan_kernel_times(dgr_plan *plan, double *ms, int64_t *launches) {
     if (!plan || !ms || !launches) return fail(DGR_E_INVALID, "null argument");
     HIP_OK(hipSetDevice(plan->device));
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
         ms[k] = 0.0;
         launches[k] = (int64_t)plan->pending[k].size();
         for (auto &pr : plan->pending[k]) {
@@ -763,6 +1105,10 @@ void timed(dgr_plan *plan, int kind, hipStream_t st, F launch) {
 template <int MODE>
 int run_chunks(dgr_plan *plan, const VarPtrs &state, const VarPtrs &rhs, const RKArgs &rk,
                const RhsParams &prm, hipStream_t st) {
+    if (!plan->d_W && plan->max_chunk_points) {
+        hipError_t e = hipMalloc(&plan->d_W, sizeof(double) * NSLOT * (size_t)plan->max_chunk_points);
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "derivative workspace");
+    }
     for (const Chunk &c : plan->chunks) {
         const int ntile = c.tile_end - c.tile_begin;
         timed(plan, 0, st, [&] {
@@ -785,6 +1131,15 @@ int run_chunks(dgr_plan *plan, const VarPtrs &state, const VarPtrs &rhs, const R
     return DGR_OK;
 }
 
+int run_fused(dgr_plan *plan, const StageIO &io, const RhsParams &prm, hipStream_t st) {
+    if (plan->nftiles == 0) return DGR_OK;
+    timed(plan, 3, st, [&] {
+        hipLaunchKernelGGL(k_fused, dim3(plan->nftiles), dim3(FT), 0, st, io, prm, plan->d_blocks, plan->d_ftiles);
+    });
+    HIP_OK(hipGetLastError());
+    return DGR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -798,6 +1153,14 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
     int rc;
     if ((rc = load_ptrs(u, d_u, "d_u")) || (rc = load_ptrs(r, d_rhs, "d_rhs"))) return rc;
     HIP_OK(hipSetDevice(plan->device));
+    if (plan->fused) {
+        StageIO io{};
+        io.u = u;
+        io.rhs = r;
+        io.stage = -1;
+        io.err = plan->d_err;
+        return run_fused(plan, io, to_params(params), (hipStream_t)stream);
+    }
     RKArgs rk{};
     return run_chunks<MODE_RHS>(plan, u, r, rk, to_params(params), (hipStream_t)stream);
 }
@@ -811,6 +1174,41 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
     if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
     HIP_OK(hipSetDevice(plan->device));
     hipStream_t st = (hipStream_t)stream;
+    // RK4 table: include/rkBSSN.h:50-53
+    static const double RKC[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    static const double RKU[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
+    if (plan->fused) {
+        const size_t ni = (size_t)plan->interior;
+        if (!plan->d_istage && ni) {
+            hipError_t e = hipMalloc(&plan->d_istage, sizeof(double) * 3 * NV * ni);
+            if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "stage buffers");
+        }
+        VarPtrs sa, sb, acc;
+        for (int v = 0; v < NV; v++) {
+            sa.p[v] = plan->d_istage + (size_t)v * ni;
+            sb.p[v] = plan->d_istage + (size_t)(NV + v) * ni;
+            acc.p[v] = plan->d_istage + (size_t)(2 * NV + v) * ni;
+        }
+        const RhsParams prm = to_params(params);
+        // stage s reads u (s = 0) or the previous stage buffer, writes the other one;
+        // the last stage writes the result into u (rkBSSN.cpp:764-860)
+        const VarPtrs *ins[4] = {&sa, &sa, &sb, &sa};
+        const VarPtrs *outs[4] = {&sa, &sb, &sa, &sb};
+        for (int s = 0; s < 4; s++) {
+            StageIO io{};
+            io.u = u;
+            io.sin = *ins[s];
+            io.sout = *outs[s];
+            io.acc = acc;
+            io.cacc = RKC[s] * dt;
+            io.cnext = s < 3 ? RKU[s + 1] * dt : 0.0;
+            io.stage = s;
+            io.enforce_stage = enforce_per_stage ? 1 : 0;
+            io.err = plan->d_err;
+            if ((rc = run_fused(plan, io, prm, st))) return rc;
+        }
+        return DGR_OK;
+    }
     const size_t n = plan->szpdof;
     if (!plan->d_stage && n) {
         hipError_t e = hipMalloc(&plan->d_stage, sizeof(double) * 2 * NV * n);
@@ -822,17 +1220,14 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
         acc.p[v] = plan->d_stage + (size_t)(NV + v) * n;
         HIP_OK(hipMemcpyAsync(stage.p[v], u.p[v], sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     }
-    // RK4 table: include/rkBSSN.h:50-53
-    static const double C[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-    static const double U[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
     const RhsParams prm = to_params(params);
     for (int s = 0; s < 4; s++) {
         RKArgs rk;
         rk.u0 = u;
         rk.acc = acc;
         rk.snext = stage;
-        rk.cacc = C[s] * dt;
-        rk.cnext = s < 3 ? U[s + 1] * dt : 0.0;
+        rk.cacc = RKC[s] * dt;
+        rk.cnext = s < 3 ? RKU[s + 1] * dt : 0.0;
         rk.stage = s;
         rk.enforce_stage = enforce_per_stage ? 1 : 0;
         rk.err = plan->d_err;

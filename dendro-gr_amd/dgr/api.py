@@ -145,10 +145,10 @@ class Plan:
 
     def kernel_times(self):
         """{kind: (total ms, launches)} since the previous call."""
-        ms = (C.c_double * 3)()
-        n = (C.c_int64 * 3)()
+        ms = (C.c_double * 4)()
+        n = (C.c_int64 * 4)()
         _check(load().dgr_plan_kernel_times(self._h, ms, n), "dgr_plan_kernel_times")
-        return {k: (ms[i], n[i]) for i, k in enumerate(("derivs", "rhs", "face"))}
+        return {k: (ms[i], n[i]) for i, k in enumerate(("derivs", "rhs", "face", "fused"))}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -82,8 +82,11 @@ void dgr_default_params(dgr_bssn_params *p);
 const char *dgr_last_error(void);
 
 /* Validates the block table (sizes >= 13 per axis, blocks inside [0, szpdof)),
- * splits every block into tiles of at most 7^3 interior points, orders the
- * work for the device and allocates the derivative workspace on `device`.
+ * splits every block into tiles and orders the work for the device. The
+ * default is the two-kernel path (tiles of at most 7^3 points; a derivative
+ * workspace of 162 doubles per point in chunks of DGR_CHUNK_POINTS points);
+ * the environment variable DGR_KERNEL=fused selects the single-kernel path
+ * (tiles of at most 7 x 7 x 5 points, no workspace).
  * Replaces the reference's global derivative workspace and its per-block
  * loop setup (BSSN_GR/src/rhs.cpp:10-73, grUtils.cpp:2374-2396). The table is
  * copied; the plan does not keep `blocks`. */
@@ -98,8 +101,9 @@ int64_t dgr_plan_interior_points(const dgr_plan *plan);
  * the phases of bssnrhs, include/profile_params.h:15-53): when enabled, every
  * launch is bracketed by HIP events on the caller's stream.
  * dgr_plan_kernel_times waits for them and returns, per kernel kind
- * (0 derivatives, 1 equations, 2 boundary faces), the summed milliseconds
- * and the number of launches since the previous call. */
+ * (0 derivatives, 1 equations, 2 boundary faces — the split path; 3 the fused
+ * kernel), the summed milliseconds and the number of launches since the
+ * previous call; `ms` and `launches` have 4 entries. */
 int dgr_plan_set_timing(dgr_plan *plan, int enable);
 int dgr_plan_kernel_times(dgr_plan *plan, double *ms, int64_t *launches);
 
@@ -118,8 +122,9 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
  * advanced without re-filling padding between stages). With
  * enforce_per_stage != 0 the algebraic constraints are enforced on every stage
  * state as rkBSSN.cpp:810-812 does; they are always enforced on the result
- * (rkBSSN.cpp:858, bssnCtx.cpp:1174-1182). Uses plan-owned stage buffers
- * (two states of szpdof doubles per variable, allocated on first use). */
+ * (rkBSSN.cpp:858, bssnCtx.cpp:1174-1182). Uses plan-owned stage buffers,
+ * allocated on first use: three interior-only states (two stage states and
+ * the accumulator, 3 x 24 x interior-points doubles). */
 int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
                  int enforce_per_stage, void *stream);
 

@@ -217,3 +217,93 @@ def test_negative_metric_determinant_is_reported():
     assert plan.status() == dgr.api.C.c_int(-4).value
     assert plan.status() == 0  # cleared
     plan.close()
+
+
+@pytest.mark.parametrize("bflag", [0, 63, 21])
+def test_fused_path_matches_oracle(monkeypatch, bflag):
+    """The single-kernel path (no derivative workspace) gives the same RHS."""
+    monkeypatch.setenv("DGR_KERNEL", "fused")
+    log(f"fused block19 bflag={bflag}")
+    t = W.single_block_19(bflag)
+    st = host_state(t, seed=1)
+    p = block_params()
+    got = gpu_rhs(t, st, p)
+    ref = O.bssnrhs_batch(st, t, oracle_params(p), nthreads=1)
+    assert normwise(interior_values(got, t), interior_values(ref, t)).max() <= TOL
+    pad = padding_mask(t, st.shape[1])
+    assert np.isnan(got[:, pad]).all(), "padding of the output was written"
+
+
+def test_fused_path_uneven_blocks_match_oracle(monkeypatch):
+    monkeypatch.setenv("DGR_KERNEL", "fused")
+    log("fused uneven puncture blocks")
+    full = W.puncture_octree(0.42)
+    pick = []
+    for n in (55, 31, 19, 13):
+        idx = np.nonzero(full["sz"][:, 0] == n)[0]
+        pick += list(idx[:2])
+    t = full[np.array(sorted(pick))].copy()
+    off = 0
+    for b in range(len(t)):
+        t["offset"][b] = off
+        off += int(np.prod(t["sz"][b].astype(np.int64)))
+    t["bflag"][0] = 63
+    t["bflag"][3] = 42
+    st = host_state(t, init="puncture", seed=5)
+    p = dgr.default_params(ko_sigma=0.4, cako_enabled=1)
+    got = gpu_rhs(t, st, p)
+    ref = O.bssnrhs_batch(st, t, oracle_params(p), nthreads=8)
+    assert normwise(interior_values(got, t), interior_values(ref, t)).max() <= TOL
+
+
+@pytest.mark.parametrize("enforce", [True, False])
+def test_fused_path_rk4_matches_oracle(monkeypatch, enforce):
+    import torch
+    monkeypatch.setenv("DGR_KERNEL", "fused")
+    log(f"fused rk4 enforce={enforce}")
+    t = W.uniform_13(6)
+    t["bflag"][2] = 63
+    st = host_state(t, seed=9)
+    p = block_params()
+    plan = dgr.Plan(t, st.shape[1])
+    u = to_dev(st)
+    plan.rk4_step(u, 0.025, p, enforce_per_stage=enforce)
+    torch.cuda.synchronize()
+    assert plan.status() == 0
+    ref = st.copy()
+    O.rk4_step(ref, t, 0.025, oracle_params(p), enforce_per_stage=enforce, nthreads=6)
+    got = u.cpu().numpy()
+    pad = padding_mask(t, st.shape[1])
+    assert np.array_equal(got[:, pad], st[:, pad])
+    assert normwise(interior_values(got - st, t), interior_values(ref - st, t)).max() <= 1e-11
+    plan.close()
+
+
+@pytest.mark.parametrize("bflag", [0, 63])
+def test_split_path_matches_oracle(monkeypatch, bflag):
+    """The two-kernel path (derivative workspace) stays selectable and correct."""
+    monkeypatch.setenv("DGR_KERNEL", "split")
+    t = W.single_block_19(bflag)
+    st = host_state(t, seed=1)
+    p = block_params()
+    got = gpu_rhs(t, st, p)
+    ref = O.bssnrhs_batch(st, t, oracle_params(p), nthreads=1)
+    assert normwise(interior_values(got, t), interior_values(ref, t)).max() <= TOL
+
+
+def test_split_path_rk4_matches_oracle(monkeypatch):
+    import torch
+    monkeypatch.setenv("DGR_KERNEL", "split")
+    t = W.uniform_13(6)
+    t["bflag"][2] = 63
+    st = host_state(t, seed=9)
+    p = block_params()
+    plan = dgr.Plan(t, st.shape[1])
+    u = to_dev(st)
+    plan.rk4_step(u, 0.025, p, enforce_per_stage=True)
+    torch.cuda.synchronize()
+    ref = st.copy()
+    O.rk4_step(ref, t, 0.025, oracle_params(p), enforce_per_stage=True, nthreads=6)
+    got = u.cpu().numpy()
+    assert normwise(interior_values(got - st, t), interior_values(ref - st, t)).max() <= 1e-11
+    plan.close()
