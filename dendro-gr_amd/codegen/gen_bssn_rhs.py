@@ -514,7 +514,7 @@ def generate_phased(path):
         if p == len(PHASE_ORDER):
             return "    ctx.finish();"
         nxt = vidx[p + 1] if p + 1 < len(vidx) else -1
-        return f"    ctx.template phase<{vidx[p]}, {nxt}>();  // {PHASE_ORDER[p]}"
+        return f"    ctx.template phase<{vidx[p]}, {nxt}, {p}>();  // {PHASE_ORDER[p]}"
 
     def phased_input(x):
         n = x.name
@@ -538,10 +538,10 @@ def generate_phased(path):
     hdr = HEADER_TOP + f"""//
 // Phased form for a kernel that streams the variables through LDS one at a
 // time, in the order {', '.join(PHASE_ORDER)}.
-// ctx.phase<V, NEXT>() makes variable V's derivatives at this thread's point
-// available (and may start fetching NEXT); ctx.d1<V, D>() / ctx.d2<M, C>() may
-// only be called in V's phase; ctx.fld<V>() may be called anywhere;
-// ctx.finish() ends the last phase. Every thread of a workgroup must execute
+// ctx.phase<V, NEXT, K>() (K: position of V in the order) makes variable V's
+// derivatives at this thread's point available (and may start fetching NEXT);
+// ctx.d1<V, D>() / ctx.d2<M, C>() may only be called in V's phase;
+// ctx.fld<V>() may be called anywhere. Every thread of a workgroup must execute
 // the whole function (the phases synchronize the workgroup). DGR_PIN(t) marks
 // where a temporary must be computed: a device build defines it to keep the
 // compiler from sinking a statement into a later phase (which would keep the

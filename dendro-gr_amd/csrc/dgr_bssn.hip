@@ -468,23 +468,33 @@ __global__ __launch_bounds__(NTHREADS) void k_face(VarPtrs u, VarPtrs rhs, RKArg
 
 // -------------------------------------------------------------------------
 // k_fused: derivatives, equations, KO, BC and the stage update in one kernel.
-// One workgroup per tile; the 24 variables stream through LDS in the phase
-// order of bssn_rhs_phased.h: while the equations of phase v run, the next
-// variable's region is already in flight into registers. Each thread owns one
-// point (threads beyond the tile's points shadow the last point and store
-// nothing), computes its derivatives of the resident variable, and leaves that
-// variable's Kreiss-Oliger term (on flagged faces: boundary value plus KO term)
-// in LDS for the final stores.
+// One workgroup per tile of at most 7 x 7 x 5 points, one point per thread
+// (threads beyond the tile's points shadow its last point and store nothing).
+// The 24 variables stream through LDS in the phase order of
+// bssn_rhs_phased.h: while phase K works on variable V (staged in LDS buffer
+// K&1), the next variable's region is already in flight into registers.
+// Every thread's own field values sit in LDS for the equations, its KO term
+// (on flagged faces: boundary value plus KO term) is left in LDS, and the
+// right-hand side is combined with both in one batched epilogue.
+//
+// Instruction economy matters more than usual here: the equations need about
+// 430 registers per lane, so a SIMD runs a single wave and every instruction
+// costs a full issue slot. LDS strides are compile-time (region 13 x 13 x
+// (ez+6)), stencil constants come precomputed per block, and face closures
+// sit behind a branch that is uniform over the tile.
 // -------------------------------------------------------------------------
-// One wave per SIMD: the equations need ~430 registers per lane (256 VGPRs
-// plus AGPRs used as spill space), which only a single wave per SIMD has.
-constexpr int FT = 256;                                   // threads per workgroup
-constexpr int FZMAX = 5;                                  // max tile extent along z
-constexpr int TPTS = FT;                                  // max points per tile (7 x 7 x 5 = 245)
-constexpr int RPTS = RMAX * RMAX * (FZMAX + 2 * PW);      // staged region points
-constexpr int DXPTS = TMAX * RMAX * (FZMAX + 2 * PW);     // D_x region
-constexpr int DYPTS = TMAX * TMAX * (FZMAX + 2 * PW);     // D_y region
-constexpr int PREF = (RPTS + FT - 1) / FT;                // prefetch registers per thread
+constexpr int FT = 256;                        // threads per workgroup
+constexpr int FZMAX = 5;                       // max tile extent along z
+constexpr int FRX = RMAX, FRXY = RMAX * RMAX;  // LDS strides of the staged region (13, 169)
+constexpr int FRZMAX = FZMAX + 2 * PW;         // max staged extent along z (11)
+constexpr int PREF = (FRXY * FRZMAX + FT - 1) / FT;  // region elements per thread (8)
+constexpr int FBUF = PREF * FT;                // one LDS buffer of the staged region
+constexpr int DXS_Y = TMAX, DXS_Z = TMAX * RMAX;     // D_x u: tile x, region y, region z
+constexpr int NDX = TMAX * RMAX * FRZMAX;
+constexpr int DYS_Y = TMAX, DYS_Z = TMAX * TMAX;     // D_y u: tile x, tile y, region z
+constexpr int NDY = TMAX * TMAX * FRZMAX;
+constexpr int DXIT = (NDX + FT - 1) / FT;            // D_x evaluations per thread (4)
+constexpr int DYIT = (NDY + FT - 1) / FT;            // D_y evaluations per thread (3)
 
 // Where a stage reads its state and writes its results. Stage buffers hold
 // interior points only, indexed by DBlock::q0 + (i-3) + mx*((j-3) + my*(k-3));
@@ -501,94 +511,120 @@ struct StageIO {
     int *err;
 };
 
+// FACE: the tile's block has flagged faces (closures and boundary values are
+// needed); interior tiles run a body without any of that code.
+template <bool FACE>
 struct FusedCtx {
-    // tile geometry
-    int nx, ny, nz, rx, ry, nreg, ex, ey;
-    int gx0, gy0, gz0;
+    const StageIO &io;
+    const dgr::AxisConsts *K;  // x, y, z constants of this block
+    double *s_tile, *s_dx, *s_dy, *s_ko, *s_fld;
+    // tile (uniform)
+    int ox, oy, oz, iex, iey, iez;  // tile origin, ie = n - 3 per axis
+    bool lox, hix, loy, hiy, loz, hiz;
+    int ndx, ndy;                   // D_x / D_y points of the staged region
     uint64_t boff;
     int64_t bq0;
-    int mx, my;
-    bool lox, hix, loy, hiy, loz, hiz;
-    double h[3];
-    // this thread's point
-    int p, li, lj, lk, ax, ay, az;
-    bool active, face;
-    uint64_t idx;
-    int64_t q;
-    double x, y, z, inv_r, sig_gauge, sig_other;
-    // sources and LDS
-    const StageIO *io;
-    double *s_tile, *s_dx, *s_dy, *s_ko;
+    double pmin[3], h[3];
+    // this thread
+    int tid, ax, ay, az, cofs, dxo, dyo;
+    bool face;
+    double sig_gauge, sig_other;
+    // region elements this thread fetches: offset in the padded block, and in
+    // the interior-only stage buffer (-1: a padding point)
+    uint32_t goff[PREF];
+    int32_t qoff[PREF];
     double pre[PREF];
+    // LDS source offsets of this thread's D_x / D_y evaluations
+    int dxs[DXIT], dys[DYIT];
     double d1v[3], d2v[6];
 
+    __device__ FusedCtx(const StageIO &io_) : io(io_) {}
+
     __device__ __forceinline__ void fetch(int v) {
-        const bool from_stage = io->stage >= 1;
-        const double *__restrict__ up = io->u.p[v];
-        const double *__restrict__ sp_ = from_stage ? io->sin.p[v] : nullptr;
+        const double *__restrict__ ub = io.u.p[v] + boff;
+        if (io.stage >= 1) {
+            const double *__restrict__ sb = io.sin.p[v] + bq0;
 #pragma unroll
-        for (int c = 0; c < PREF; c++) {
-            const int e = threadIdx.x + c * FT;
-            if (e < nreg) {
-                const int ii = e % rx, jj = (e / rx) % ry, kk = e / (rx * ry);
-                const int gi = gx0 + ii, gj = gy0 + jj, gk = gz0 + kk;
-                const bool inner = gi >= PW && gi < nx - PW && gj >= PW && gj < ny - PW && gk >= PW && gk < nz - PW;
-                if (from_stage && inner)
-                    pre[c] = sp_[bq0 + (gi - PW) + (int64_t)mx * ((gj - PW) + (int64_t)my * (gk - PW))];
-                else
-                    pre[c] = up[boff + (uint64_t)gi + (uint64_t)nx * ((uint64_t)gj + (uint64_t)ny * gk)];
-            }
+            for (int c = 0; c < PREF; c++) pre[c] = qoff[c] >= 0 ? sb[qoff[c]] : ub[goff[c]];
+        } else {
+#pragma unroll
+            for (int c = 0; c < PREF; c++) pre[c] = ub[goff[c]];
         }
     }
 
-    __device__ __forceinline__ void stash() {
-#pragma unroll
-        for (int c = 0; c < PREF; c++) {
-            const int e = threadIdx.x + c * FT;
-            if (e < nreg) s_tile[e] = pre[c];
-        }
-    }
-
-    template <int V, int NEXT>
+    template <int V, int NEXT, int KPH>
     __device__ __forceinline__ void phase() {
-        __syncthreads();  // everyone is done with the previous variable's LDS data
-        stash();
+        double *t = s_tile + (KPH & 1) * FBUF;
+#pragma unroll
+        for (int c = 0; c < PREF; c++) t[tid + c * FT] = pre[c];
         if constexpr (NEXT >= 0) fetch(NEXT);
         __syncthreads();
+        const dgr::AxisConsts &kx = K[0], &ky = K[1], &kz = K[2];
         constexpr int m2 = d2slot(V);
-        const dgr::Axis1<double> c1x(h[0]), c1y(h[1]), c1z(h[2]);
-        const int ez = nreg / (rx * ry) - 2 * PW;
         if constexpr (m2 >= 0) {
-            const int ndx = ex * ry * (ez + 2 * PW);
-            for (int e = threadIdx.x; e < ndx; e += FT) {
-                const int i0 = e % ex, jj = (e / ex) % ry, kk = e / (ex * ry);
-                s_dx[e] = dgr::d1(&s_tile[(PW + i0) + rx * (jj + ry * kk)], 1, gx0 + PW + i0, nx, lox, hix, c1x);
+            // D_x u on tile-x x region-y x region-z and D_y u on tile-x x tile-y x region-z
+            // (the reference's mixed derivatives are D_y(D_x u), D_z(D_x u), D_z(D_y u))
+#pragma unroll
+            for (int it = 0; it < DXIT; it++) {
+                const int e = tid + it * FT;
+                if (e < ndx) {
+                    const double *s = t + dxs[it];
+                    double d = dgr::d1_in<1>(s, kx.d1_60);
+                    if (FACE && (lox || hix)) d = dgr::d1_edge<1>(s, d, ox + e % TMAX, iex, lox, hix, kx.d1_12);
+                    s_dx[e] = d;
+                }
             }
-            const int ndy = ex * ey * (ez + 2 * PW);
-            for (int e = threadIdx.x; e < ndy; e += FT) {
-                const int i0 = e % ex, j0 = (e / ex) % ey, kk = e / (ex * ey);
-                s_dy[e] = dgr::d1(&s_tile[(PW + i0) + rx * ((PW + j0) + ry * kk)], rx, gy0 + PW + j0, ny, loy, hiy, c1y);
+#pragma unroll
+            for (int it = 0; it < DYIT; it++) {
+                const int e = tid + it * FT;
+                if (e < ndy) {
+                    const double *s = t + dys[it];
+                    double d = dgr::d1_in<FRX>(s, ky.d1_60);
+                    if (FACE && (loy || hiy))
+                        d = dgr::d1_edge<FRX>(s, d, oy + (e / TMAX) % TMAX, iey, loy, hiy, ky.d1_12);
+                    s_dy[e] = d;
+                }
             }
             __syncthreads();
         }
-        const double *c = &s_tile[(PW + li) + rx * ((PW + lj) + ry * (PW + lk))];
+        const double *c = t + cofs;
         if constexpr (m2 >= 0) {
-            const double *cx = &s_dx[li + ex * ((PW + lj) + ry * (PW + lk))];
-            const double *cy = &s_dy[li + ex * (lj + ey * (PW + lk))];
+            const double *cx = s_dx + dxo;
+            const double *cy = s_dy + dyo;
             d1v[0] = cx[0];
             d1v[1] = cy[0];
-            const dgr::Axis2<double> c2x(h[0]), c2y(h[1]), c2z(h[2]);
-            d2v[0] = dgr::d2(c, 1, ax, nx, lox, hix, c2x);
-            d2v[1] = dgr::d1(cx, ex, ay, ny, loy, hiy, c1y);        // D_y(D_x u)
-            d2v[2] = dgr::d1(cx, ex * ry, az, nz, loz, hiz, c1z);   // D_z(D_x u)
-            d2v[3] = dgr::d2(c, rx, ay, ny, loy, hiy, c2y);
-            d2v[4] = dgr::d1(cy, ex * ey, az, nz, loz, hiz, c1z);   // D_z(D_y u)
-            d2v[5] = dgr::d2(c, rx * ry, az, nz, loz, hiz, c2z);
+            double r;
+            r = dgr::d2_in<1>(c, kx.d2_180);
+            if (FACE && (lox || hix)) r = dgr::d2_edge<1>(c, r, ax, iex, lox, hix, kx.d2_12);
+            d2v[0] = r;
+            r = dgr::d1_in<DXS_Y>(cx, ky.d1_60);  // D_y(D_x u)
+            if (FACE && (loy || hiy)) r = dgr::d1_edge<DXS_Y>(cx, r, ay, iey, loy, hiy, ky.d1_12);
+            d2v[1] = r;
+            r = dgr::d1_in<DXS_Z>(cx, kz.d1_60);  // D_z(D_x u)
+            if (FACE && (loz || hiz)) r = dgr::d1_edge<DXS_Z>(cx, r, az, iez, loz, hiz, kz.d1_12);
+            d2v[2] = r;
+            r = dgr::d2_in<FRX>(c, ky.d2_180);
+            if (FACE && (loy || hiy)) r = dgr::d2_edge<FRX>(c, r, ay, iey, loy, hiy, ky.d2_12);
+            d2v[3] = r;
+            r = dgr::d1_in<DYS_Z>(cy, kz.d1_60);  // D_z(D_y u)
+            if (FACE && (loz || hiz)) r = dgr::d1_edge<DYS_Z>(cy, r, az, iez, loz, hiz, kz.d1_12);
+            d2v[4] = r;
+            r = dgr::d2_in<FRXY>(c, kz.d2_180);
+            if (FACE && (loz || hiz)) r = dgr::d2_edge<FRXY>(c, r, az, iez, loz, hiz, kz.d2_12);
+            d2v[5] = r;
         } else {
-            d1v[0] = dgr::d1(c, 1, ax, nx, lox, hix, c1x);
-            d1v[1] = dgr::d1(c, rx, ay, ny, loy, hiy, c1y);
+            double r = dgr::d1_in<1>(c, kx.d1_60);
+            if (FACE && (lox || hix)) r = dgr::d1_edge<1>(c, r, ax, iex, lox, hix, kx.d1_12);
+            d1v[0] = r;
+            r = dgr::d1_in<FRX>(c, ky.d1_60);
+            if (FACE && (loy || hiy)) r = dgr::d1_edge<FRX>(c, r, ay, iey, loy, hiy, ky.d1_12);
+            d1v[1] = r;
         }
-        d1v[2] = dgr::d1(c, rx * ry, az, nz, loz, hiz, c1z);
+        {
+            double r = dgr::d1_in<FRXY>(c, kz.d1_60);
+            if (FACE && (loz || hiz)) r = dgr::d1_edge<FRXY>(c, r, az, iez, loz, hiz, kz.d1_12);
+            d1v[2] = r;
+        }
         // computed here, not at their last use (that would keep the LDS taps live)
         DGR_PIN(d1v[0]);
         DGR_PIN(d1v[1]);
@@ -597,26 +633,33 @@ struct FusedCtx {
 #pragma unroll
             for (int k = 0; k < 6; k++) DGR_PIN(d2v[k]);
         }
-        ko_term<V>(c);
+        ko_term<V>(c, kx, ky, kz);
     }
 
     // KO term sigma*(ko_x+ko_y+ko_z) (rhs.cpp:434-489); on a flagged face the
     // boundary value replaces the equations (bssn_bcs, rhs.cpp:511-631) and the
     // KO term is added to it, as the reference adds KO after the boundary pass.
     template <int V>
-    __device__ __forceinline__ void ko_term(const double *c) {
+    __device__ __forceinline__ void ko_term(const double *c, const dgr::AxisConsts &kx, const dgr::AxisConsts &ky,
+                                            const dgr::AxisConsts &kz) {
 #pragma clang fp contract(off)
-        const dgr::AxisKO<double> kx(h[0]), ky(h[1]), kz(h[2]);
-        const double kox = dgr::ko(c, 1, ax, nx, lox, hix, kx);
-        const double koy = dgr::ko(c, rx, ay, ny, loy, hiy, ky);
-        const double koz = dgr::ko(c, rx * ry, az, nz, loz, hiz, kz);
+        double kox = dgr::ko_in<1>(c, kx.ko_pre);
+        if (FACE && (lox || hix)) kox = dgr::ko_edge<1>(c, kox, ax, iex, lox, hix, kx);
+        double koy = dgr::ko_in<FRX>(c, ky.ko_pre);
+        if (FACE && (loy || hiy)) koy = dgr::ko_edge<FRX>(c, koy, ay, iey, loy, hiy, ky);
+        double koz = dgr::ko_in<FRXY>(c, kz.ko_pre);
+        if (FACE && (loz || hiz)) koz = dgr::ko_edge<FRXY>(c, koz, az, iez, loz, hiz, kz);
         const double kt = (is_gauge(V) ? sig_gauge : sig_other) * (kox + koy + koz);
         double v = kt;
-        if (face) {
+        if (FACE && face) {
+            const double x = coord(pmin[0], ax, h[0]);
+            const double y = coord(pmin[1], ay, h[1]);
+            const double z = coord(pmin[2], az, h[2]);
+            const double inv_r = inv_radius(x, y, z);
             const double bc = -inv_r * (x * d1v[0] + y * d1v[1] + z * d1v[2] + BC_FALLOFF[V] * (c[0] - BC_ASYM[V]));
             v = bc + kt;
         }
-        if (active) s_ko[V * TPTS + p] = v;
+        s_ko[V * FT + tid] = v;
     }
 
     template <int V, int D>
@@ -624,128 +667,191 @@ struct FusedCtx {
     template <int M, int C>
     __device__ __forceinline__ double d2() const { return d2v[C]; }
     template <int V>
-    __device__ __forceinline__ double fld() const {
-        return io->stage >= 1 ? io->sin.p[V][q] : io->u.p[V][idx];
-    }
+    __device__ __forceinline__ double fld() const { return s_fld[V * FT + tid]; }
     __device__ __forceinline__ void finish() {}
 };
 
+// The equations' outputs: on a flagged face the boundary value (already
+// holding its KO term), elsewhere equations + KO term, kept in LDS until the
+// epilogue.
+template <bool FACE>
 struct FusedOut {
-    const FusedCtx *ctx;
-    const StageIO *io;
-    double v14[14];
-
+    const FusedCtx<FACE> &ctx;
+    __device__ explicit FusedOut(const FusedCtx<FACE> &c) : ctx(c) {}
     template <int V>
     __device__ __forceinline__ void put(double val) {
 #pragma clang fp contract(off)
-        const double kb = ctx->s_ko[V * TPTS + ctx->p];
-        val = ctx->face ? kb : val + kb;
-        if (!ctx->active) return;
-        const uint64_t idx = ctx->idx;
-        const int64_t q = ctx->q;
-        const int s = io->stage;
-        if (s < 0) {
-            io->rhs.p[V][idx] = val;
-        } else if (s < 3) {
-            const double u0 = io->u.p[V][idx];
-            const double nxt = u0 + io->cnext * val;
-            io->acc.p[V][q] = (s == 0 ? u0 : io->acc.p[V][q]) + io->cacc * val;
-            if (slot14(V) >= 0 && io->enforce_stage) v14[slot14(V)] = nxt;
-            else io->sout.p[V][q] = nxt;
-        } else {
-            const double fin = io->acc.p[V][q] + io->cacc * val;
-            if (slot14(V) >= 0) v14[slot14(V)] = fin;
-            else io->u.p[V][idx] = fin;
-        }
-    }
-
-    __device__ __forceinline__ void finish(double chi_floor) {
-        const int s = io->stage;
-        if (!ctx->active || s < 0 || (s < 3 && !io->enforce_stage)) return;
-        const int e = enforce_point(v14, chi_floor);
-        if (e) atomicOr(io->err, e);
-        if (s == 3) {
-            const uint64_t idx = ctx->idx;
-            io->u.p[0][idx] = v14[0];
-            io->u.p[1][idx] = v14[1];
-#pragma unroll
-            for (int c = 0; c < 12; c++) io->u.p[12 + c][idx] = v14[2 + c];
-        } else {
-            const int64_t q = ctx->q;
-            io->sout.p[0][q] = v14[0];
-            io->sout.p[1][q] = v14[1];
-#pragma unroll
-            for (int c = 0; c < 12; c++) io->sout.p[12 + c][q] = v14[2 + c];
-        }
+        double *s = ctx.s_ko + V * FT + ctx.tid;
+        const double kb = *s;
+        *s = (FACE && ctx.face) ? kb : val + kb;
     }
 };
 
-__global__ __launch_bounds__(FT) void k_fused(StageIO io, RhsParams prm, const DBlock *__restrict__ blocks,
-                                             const Tile *__restrict__ tiles) {
-    __shared__ double s_tile[RPTS];
-    __shared__ double s_dx[DXPTS];
-    __shared__ double s_dy[DYPTS];
-    __shared__ double s_ko[NV * TPTS];
+// RHS store or RK4 stage update of one point (rkBSSN.cpp:764-860), with the
+// loads of all 24 variables issued before any store.
+__device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *s_ko, const double *s_fld, int tid,
+                                               uint64_t idx, int64_t q, double chi_floor) {
+#pragma clang fp contract(off)
+    const int s = io.stage;
+    if (s < 0) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) io.rhs.p[v][idx] = s_ko[v * FT + tid];
+        return;
+    }
+    double u0[NV], a0[NV], v14[14];
+    if (s == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) u0[v] = a0[v] = s_fld[v * FT + tid];
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            if (s < 3) u0[v] = io.u.p[v][idx];
+            a0[v] = io.acc.p[v][q];
+        }
+    }
+    const bool enf = s == 3 || io.enforce_stage;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const double k = s_ko[v * FT + tid];
+        const double acc = a0[v] + io.cacc * k;
+        double out;
+        if (s < 3) {
+            io.acc.p[v][q] = acc;
+            out = u0[v] + io.cnext * k;
+        } else {
+            out = acc;
+        }
+        if (slot14(v) >= 0 && enf) v14[slot14(v)] = out;
+        else if (s < 3) io.sout.p[v][q] = out;
+        else io.u.p[v][idx] = out;
+    }
+    if (!enf) return;
+    const int e = enforce_point(v14, chi_floor);
+    if (e) atomicOr(io.err, e);
+#pragma unroll
+    for (int c = 0; c < 14; c++) {
+        const int v = c < 2 ? c : c + 10;
+        if (s < 3) io.sout.p[v][q] = v14[c];
+        else io.u.p[v][idx] = v14[c];
+    }
+}
 
-    const Tile T = tiles[blockIdx.x];
-    const DBlock B = blocks[T.block];
-    FusedCtx ctx;
-    ctx.io = &io;
+template <bool FACE>
+__device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &T, const DBlock &B,
+                                           const dgr::AxisConsts *K, double *s_tile, double *s_dx, double *s_dy,
+                                           double *s_ko, double *s_fld) {
+    FusedCtx<FACE> ctx(io);
+    ctx.K = K;
     ctx.s_tile = s_tile;
     ctx.s_dx = s_dx;
     ctx.s_dy = s_dy;
     ctx.s_ko = s_ko;
-    ctx.nx = B.n[0];
-    ctx.ny = B.n[1];
-    ctx.nz = B.n[2];
-    ctx.ex = T.e[0];
-    ctx.ey = T.e[1];
-    ctx.rx = T.e[0] + 2 * PW;
-    ctx.ry = T.e[1] + 2 * PW;
-    ctx.nreg = ctx.rx * ctx.ry * (T.e[2] + 2 * PW);
-    ctx.gx0 = T.o[0] - PW;
-    ctx.gy0 = T.o[1] - PW;
-    ctx.gz0 = T.o[2] - PW;
-    ctx.boff = B.offset;
-    ctx.bq0 = B.q0;
-    ctx.mx = B.n[0] - 2 * PW;
-    ctx.my = B.n[1] - 2 * PW;
+    ctx.s_fld = s_fld;
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int ex = T.e[0], ey = T.e[1], ez = T.e[2];
+    ctx.ox = T.o[0];
+    ctx.oy = T.o[1];
+    ctx.oz = T.o[2];
+    ctx.iex = nx - PW;
+    ctx.iey = ny - PW;
+    ctx.iez = nz - PW;
     const uint32_t bf = B.bflag;
     ctx.lox = bf & 1u; ctx.hix = bf & 2u; ctx.loy = bf & 4u; ctx.hiy = bf & 8u; ctx.loz = bf & 16u; ctx.hiz = bf & 32u;
-    ctx.h[0] = B.h[0];
-    ctx.h[1] = B.h[1];
-    ctx.h[2] = B.h[2];
-    const int npts = T.e[0] * T.e[1] * T.e[2];
-    ctx.active = (int)threadIdx.x < npts;
-    ctx.p = ctx.active ? (int)threadIdx.x : npts - 1;
-    ctx.li = ctx.p % T.e[0];
-    ctx.lj = (ctx.p / T.e[0]) % T.e[1];
-    ctx.lk = ctx.p / (T.e[0] * T.e[1]);
-    ctx.ax = T.o[0] + ctx.li;
-    ctx.ay = T.o[1] + ctx.lj;
-    ctx.az = T.o[2] + ctx.lk;
-    ctx.idx = B.offset + (uint64_t)ctx.ax + (uint64_t)ctx.nx * ((uint64_t)ctx.ay + (uint64_t)ctx.ny * ctx.az);
-    ctx.q = B.q0 + (ctx.ax - PW) + (int64_t)ctx.mx * ((ctx.ay - PW) + (int64_t)ctx.my * (ctx.az - PW));
-    ctx.face = (ctx.lox && ctx.ax == PW) || (ctx.hix && ctx.ax == ctx.nx - PW - 1) || (ctx.loy && ctx.ay == PW) ||
-               (ctx.hiy && ctx.ay == ctx.ny - PW - 1) || (ctx.loz && ctx.az == PW) || (ctx.hiz && ctx.az == ctx.nz - PW - 1);
-    ctx.x = coord(B.pmin[0], ctx.ax, B.h[0]);
-    ctx.y = coord(B.pmin[1], ctx.ay, B.h[1]);
-    ctx.z = coord(B.pmin[2], ctx.az, B.h[2]);
-    ctx.inv_r = inv_radius(ctx.x, ctx.y, ctx.z);
-    if (prm.cako) {
-        const double sg = sqrt(ctx.fld<1>());
-        ctx.sig_gauge = sg * prm.eps_gauge;
-        ctx.sig_other = sg * prm.eps_other;
-    } else {
-        ctx.sig_gauge = ctx.sig_other = prm.ko_sigma;
+    ctx.ndx = TMAX * RMAX * (ez + 2 * PW);
+    ctx.ndy = TMAX * TMAX * (ez + 2 * PW);
+    ctx.boff = B.offset;
+    ctx.bq0 = B.q0;
+    for (int a = 0; a < 3; a++) {
+        ctx.pmin[a] = B.pmin[a];
+        ctx.h[a] = B.h[a];
     }
-    const double eta = rit_eta(ctx.x, ctx.y, ctx.z, prm);
+    const int mx = nx - 2 * PW, my = ny - 2 * PW;
+    const int tid = threadIdx.x;
+    ctx.tid = tid;
+    const int npts = ex * ey * ez;
+    const bool active = tid < npts;
+    const int p = active ? tid : npts - 1;
+    const int li = p % ex, lj = (p / ex) % ey, lk = p / (ex * ey);
+    ctx.ax = ctx.ox + li;
+    ctx.ay = ctx.oy + lj;
+    ctx.az = ctx.oz + lk;
+    ctx.cofs = (PW + li) + FRX * (PW + lj) + FRXY * (PW + lk);
+    ctx.dxo = li + DXS_Y * (PW + lj) + DXS_Z * (PW + lk);
+    ctx.dyo = li + DYS_Y * lj + DYS_Z * (PW + lk);
+    const uint64_t idx = B.offset + (uint64_t)ctx.ax + (uint64_t)nx * ((uint64_t)ctx.ay + (uint64_t)ny * ctx.az);
+    const int64_t q = B.q0 + (ctx.ax - PW) + (int64_t)mx * ((ctx.ay - PW) + (int64_t)my * (ctx.az - PW));
+    ctx.face = (ctx.lox && ctx.ax == PW) || (ctx.hix && ctx.ax == nx - PW - 1) || (ctx.loy && ctx.ay == PW) ||
+               (ctx.hiy && ctx.ay == ny - PW - 1) || (ctx.loz && ctx.az == PW) || (ctx.hiz && ctx.az == nz - PW - 1);
+    // region elements: e = tid + c*FT -> (ii, jj, kk) in the 13 x 13 x (ez+6) box;
+    // elements outside the region (x or y beyond a short tile, z beyond ez+6)
+    // load the block's first point and are never read
+#pragma unroll
+    for (int c = 0; c < PREF; c++) {
+        const int e = tid + c * FT;
+        const int kk = e / FRXY, r = e - kk * FRXY;
+        const int jj = r / FRX, ii = r - jj * FRX;
+        const bool in_region = ii < ex + 2 * PW && jj < ey + 2 * PW && kk < ez + 2 * PW;
+        const int gi = ctx.ox - PW + ii, gj = ctx.oy - PW + jj, gk = ctx.oz - PW + kk;
+        const bool inner = in_region && gi >= PW && gi < nx - PW && gj >= PW && gj < ny - PW && gk >= PW && gk < nz - PW;
+        ctx.goff[c] = in_region ? (uint32_t)(gi + nx * (gj + ny * gk)) : 0u;
+        ctx.qoff[c] = inner ? (gi - PW) + mx * ((gj - PW) + my * (gk - PW)) : -1;
+    }
+    // LDS sources of the D_x points e -> (i0, jj, kk) and D_y points e -> (i0, j0, kk)
+#pragma unroll
+    for (int it = 0; it < DXIT; it++) {
+        const int e = tid + it * FT;
+        const int i0 = e % TMAX, r = e / TMAX;
+        ctx.dxs[it] = (PW + i0) + FRX * (r % RMAX) + FRXY * (r / RMAX);
+    }
+#pragma unroll
+    for (int it = 0; it < DYIT; it++) {
+        const int e = tid + it * FT;
+        const int i0 = e % TMAX, r = e / TMAX;
+        ctx.dys[it] = (PW + i0) + FRX * (PW + r % TMAX) + FRXY * (r / TMAX);
+    }
+    // this point's field values
+    {
+        double f[NV];
+        if (io.stage >= 1) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) f[v] = io.sin.p[v][q];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; v++) f[v] = io.u.p[v][idx];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) s_fld[v * FT + tid] = f[v];
+        if (prm.cako) {
+            const double sg = sqrt(f[1]);
+            ctx.sig_gauge = sg * prm.eps_gauge;
+            ctx.sig_other = sg * prm.eps_other;
+        } else {
+            ctx.sig_gauge = ctx.sig_other = prm.ko_sigma;
+        }
+    }
+    const double eta = rit_eta(coord(B.pmin[0], ctx.ax, B.h[0]), coord(B.pmin[1], ctx.ay, B.h[1]),
+                               coord(B.pmin[2], ctx.az, B.h[2]), prm);
     ctx.fetch(dgr::kBssnFirstPhaseVar);
-    FusedOut out;
-    out.ctx = &ctx;
-    out.io = &io;
+    FusedOut<FACE> out(ctx);
     dgr::bssn_rhs_phased<double>(ctx, eta, prm.lam, prm.lamf, out);
-    out.finish(prm.chi_floor);
+    if (active) fused_epilogue(io, s_ko, s_fld, tid, idx, q, prm.chi_floor);
+}
+
+// One launch per tile kind: FACE = false over the tiles of blocks without
+// flagged faces, FACE = true over the others.
+template <bool FACE>
+__global__ __launch_bounds__(FT) void k_fused(const StageIO io, const RhsParams prm, const DBlock *__restrict__ blocks,
+                                             const dgr::AxisConsts *__restrict__ bconst,
+                                             const Tile *__restrict__ tiles) {
+    __shared__ double s_tile[2 * FBUF];
+    __shared__ double s_dx[NDX];
+    __shared__ double s_dy[NDY];
+    __shared__ double s_ko[NV * FT];
+    __shared__ double s_fld[NV * FT];
+
+    const Tile T = tiles[blockIdx.x];
+    const DBlock B = blocks[T.block];
+    fused_tile<FACE>(io, prm, T, B, bconst + 3 * (size_t)T.block, s_tile, s_dx, s_dy, s_ko, s_fld);
 }
 
 // enforce_bssn_constraints on interior points (standalone entry point)
@@ -801,9 +907,12 @@ struct dgr_plan {
     double *d_W = nullptr;
     int64_t max_chunk_points = 0;
     double *d_stage = nullptr;  // split path RK4: 2 * 24 * szpdof (stage state, accumulator)
-    bool fused = false;         // k_fused (DGR_KERNEL=fused) or the split k_derivs/k_rhs path (default)
-    Tile *d_ftiles = nullptr;   // k_fused tiles (<= 7 x 7 x 5 points), XCD-grouped order
+    bool fused = true;          // k_fused (default) or the split k_derivs/k_rhs path (DGR_KERNEL=split)
+    Tile *d_ftiles = nullptr;   // k_fused tiles (<= 7 x 7 x 5 points), XCD-grouped order:
+                                // blocks without flagged faces first, then the others
+    dgr::AxisConsts *d_bconst = nullptr;  // k_fused: x, y, z stencil constants per block
     int nftiles = 0;
+    int nftiles_inner = 0;      // the first nftiles_inner tiles belong to blocks with bflag == 0
     double *d_istage = nullptr; // k_fused RK4: 3 * 24 * interior (two stage states, accumulator)
     int *d_err = nullptr;
     // optional per-kernel timing: event pairs recorded around each launch
@@ -975,17 +1084,26 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
                         per[b].push_back(t);
                     }
         }
-        for (uint32_t g = 0; g < nblocks; g += 8) {
-            const uint32_t ge = std::min<uint32_t>(nblocks, g + 8);
-            size_t mx = 0;
-            for (uint32_t b = g; b < ge; b++) mx = std::max(mx, per[b].size());
-            for (size_t t = 0; t < mx; t++)
-                for (uint32_t b = g; b < ge; b++)
-                    if (t < per[b].size()) ftiles.push_back(per[b][t]);
+        for (int kind = 0; kind < 2; kind++) {  // blocks without flagged faces, then the others
+            std::vector<uint32_t> bl;
+            for (uint32_t b = 0; b < nblocks; b++)
+                if ((hb[b].bflag != 0) == (kind == 1)) bl.push_back(b);
+            for (size_t g = 0; g < bl.size(); g += 8) {
+                const size_t ge = std::min(bl.size(), g + 8);
+                size_t mx = 0;
+                for (size_t i = g; i < ge; i++) mx = std::max(mx, per[bl[i]].size());
+                for (size_t t = 0; t < mx; t++)
+                    for (size_t i = g; i < ge; i++)
+                        if (t < per[bl[i]].size()) ftiles.push_back(per[bl[i]][t]);
+            }
+            if (kind == 0) plan->nftiles_inner = (int)ftiles.size();
         }
     }
     plan->nftiles = (int)ftiles.size();
-    if (const char *k = std::getenv("DGR_KERNEL")) plan->fused = std::string(k) == "fused";
+    std::vector<dgr::AxisConsts> bconst(3 * (size_t)nblocks);
+    for (uint32_t b = 0; b < nblocks; b++)
+        for (int a = 0; a < 3; a++) bconst[3 * b + a] = dgr::axis_consts(hb[b].h[a]);
+    if (const char *k = std::getenv("DGR_KERNEL")) plan->fused = std::string(k) != "split";
 
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess && nblocks) e = hipMalloc(&plan->d_blocks, sizeof(DBlock) * hb.size());
@@ -993,6 +1111,9 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
     if (e == hipSuccess && !wg_tile.empty()) e = hipMalloc(&plan->d_wg_tile, sizeof(int) * wg_tile.size());
     if (e == hipSuccess && !face_tiles.empty()) e = hipMalloc(&plan->d_face_tiles, sizeof(int) * face_tiles.size());
     if (e == hipSuccess && !ftiles.empty()) e = hipMalloc(&plan->d_ftiles, sizeof(Tile) * ftiles.size());
+    if (e == hipSuccess && nblocks) e = hipMalloc(&plan->d_bconst, sizeof(dgr::AxisConsts) * bconst.size());
+    if (e == hipSuccess && nblocks)
+        e = hipMemcpy(plan->d_bconst, bconst.data(), sizeof(dgr::AxisConsts) * bconst.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&plan->d_err, sizeof(int));
     if (e == hipSuccess && nblocks) e = hipMemcpy(plan->d_blocks, hb.data(), sizeof(DBlock) * hb.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !tiles.empty()) e = hipMemcpy(plan->d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice);
@@ -1026,6 +1147,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     hipFree(plan->d_W);
     hipFree(plan->d_stage);
     hipFree(plan->d_ftiles);
+    hipFree(plan->d_bconst);
     hipFree(plan->d_istage);
     hipFree(plan->d_err);
     delete plan;
@@ -1132,10 +1254,17 @@ int run_chunks(dgr_plan *plan, const VarPtrs &state, const VarPtrs &rhs, const R
 }
 
 int run_fused(dgr_plan *plan, const StageIO &io, const RhsParams &prm, hipStream_t st) {
-    if (plan->nftiles == 0) return DGR_OK;
-    timed(plan, 3, st, [&] {
-        hipLaunchKernelGGL(k_fused, dim3(plan->nftiles), dim3(FT), 0, st, io, prm, plan->d_blocks, plan->d_ftiles);
-    });
+    const int ni = plan->nftiles_inner, nf = plan->nftiles - plan->nftiles_inner;
+    if (ni)
+        timed(plan, 3, st, [&] {
+            hipLaunchKernelGGL(k_fused<false>, dim3(ni), dim3(FT), 0, st, io, prm, plan->d_blocks, plan->d_bconst,
+                               plan->d_ftiles);
+        });
+    if (nf)
+        timed(plan, 3, st, [&] {
+            hipLaunchKernelGGL(k_fused<true>, dim3(nf), dim3(FT), 0, st, io, prm, plan->d_blocks, plan->d_bconst,
+                               plan->d_ftiles + ni);
+        });
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }

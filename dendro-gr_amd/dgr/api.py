@@ -41,7 +41,9 @@ class BssnParams(C.Structure):
 
 
 def lib_path() -> str:
-    return os.path.join(PKG, "lib", "libdgr_bssn.so")
+    """lib/libdgr_bssn.so of this package, or the build named by DGR_LIBRARY
+    (for comparing builds made with different compiler options)."""
+    return os.environ.get("DGR_LIBRARY") or os.path.join(PKG, "lib", "libdgr_bssn.so")
 
 
 _lib = None

@@ -83,10 +83,10 @@ const char *dgr_last_error(void);
 
 /* Validates the block table (sizes >= 13 per axis, blocks inside [0, szpdof)),
  * splits every block into tiles and orders the work for the device. The
- * default is the two-kernel path (tiles of at most 7^3 points; a derivative
- * workspace of 162 doubles per point in chunks of DGR_CHUNK_POINTS points);
- * the environment variable DGR_KERNEL=fused selects the single-kernel path
- * (tiles of at most 7 x 7 x 5 points, no workspace).
+ * default is the single-kernel path (tiles of at most 7 x 7 x 5 points, no
+ * workspace); the environment variable DGR_KERNEL=split selects the
+ * two-kernel path (tiles of at most 7^3 points; a derivative workspace of 162
+ * doubles per point in chunks of DGR_CHUNK_POINTS points).
  * Replaces the reference's global derivative workspace and its per-block
  * loop setup (BSSN_GR/src/rhs.cpp:10-73, grUtils.cpp:2374-2396). The table is
  * copied; the plan does not keep `blocks`. */
