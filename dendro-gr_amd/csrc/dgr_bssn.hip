@@ -483,18 +483,33 @@ __global__ __launch_bounds__(NTHREADS) void k_face(VarPtrs u, VarPtrs rhs, RKArg
 // (ez+6)), stencil constants come precomputed per block, and face closures
 // sit behind a branch that is uniform over the tile.
 // -------------------------------------------------------------------------
-constexpr int FT = 256;                        // threads per workgroup
-constexpr int FZMAX = 5;                       // max tile extent along z
 constexpr int FRX = RMAX, FRXY = RMAX * RMAX;  // LDS strides of the staged region (13, 169)
-constexpr int FRZMAX = FZMAX + 2 * PW;         // max staged extent along z (11)
-constexpr int PREF = (FRXY * FRZMAX + FT - 1) / FT;  // region elements per thread (8)
-constexpr int FBUF = PREF * FT;                // one LDS buffer of the staged region
 constexpr int DXS_Y = TMAX, DXS_Z = TMAX * RMAX;     // D_x u: tile x, region y, region z
-constexpr int NDX = TMAX * RMAX * FRZMAX;
 constexpr int DYS_Y = TMAX, DYS_Z = TMAX * TMAX;     // D_y u: tile x, tile y, region z
-constexpr int NDY = TMAX * TMAX * FRZMAX;
-constexpr int DXIT = (NDX + FT - 1) / FT;            // D_x evaluations per thread (4)
-constexpr int DYIT = (NDY + FT - 1) / FT;            // D_y evaluations per thread (3)
+
+// Tile shape of a k_fused launch: FT threads (one point each) and tiles of at
+// most 7 x 7 x FZ points. The regular shape is 256 threads / z <= 5 (one
+// workgroup per CU: it takes 143 KB of LDS); the thin shape, 128 threads /
+// z <= 2, takes the 2-point z remainder of a block (a 13^3 block's interior
+// is 7x7x5 + 7x7x2) in half the threads and ~80 KB, so two share a CU.
+template <int FT_, int FZ_>
+struct FCfg {
+    static constexpr int FT = FT_;
+    static constexpr int FZ = FZ_;
+    static constexpr int NREG = FRXY * (FZ + 2 * PW);       // staged region (13 x 13 x (FZ+6))
+    static constexpr int PREF = (NREG + FT - 1) / FT;       // region elements per thread
+    static constexpr int NDX = TMAX * RMAX * (FZ + 2 * PW);  // D_x points
+    static constexpr int NDY = TMAX * TMAX * (FZ + 2 * PW);  // D_y points
+    static constexpr int DXIT = (NDX + FT - 1) / FT;
+    static constexpr int DYIT = (NDY + FT - 1) / FT;
+    static constexpr size_t LDS = sizeof(double) * (2 * NREG + NDX + NDY + 2 * NV * FT);
+};
+using CfgWide = FCfg<256, 5>;
+using CfgThin = FCfg<128, 2>;
+static_assert(7 * 7 * CfgWide::FZ <= CfgWide::FT && 7 * 7 * CfgThin::FZ <= CfgThin::FT, "one point per thread");
+static_assert(2 * CfgThin::LDS <= 160 * 1024, "two thin workgroups per CU");
+// tile groups of the fused path, one launch each
+enum { FG_WIDE = 0, FG_THIN = 1, FG_FACE = 2, NFGROUP = 3 };
 
 // Where a stage reads its state and writes its results. Stage buffers hold
 // interior points only, indexed by DBlock::q0 + (i-3) + mx*((j-3) + my*(k-3));
@@ -513,8 +528,9 @@ struct StageIO {
 
 // FACE: the tile's block has flagged faces (closures and boundary values are
 // needed); interior tiles run a body without any of that code.
-template <bool FACE>
+template <bool FACE, class C>
 struct FusedCtx {
+    static constexpr int FT = C::FT, NREG = C::NREG, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT;
     const StageIO &io;
     const dgr::AxisConsts *K;  // x, y, z constants of this block
     double *s_tile, *s_dx, *s_dy, *s_ko, *s_fld;
@@ -554,9 +570,10 @@ struct FusedCtx {
 
     template <int V, int NEXT, int KPH>
     __device__ __forceinline__ void phase() {
-        double *t = s_tile + (KPH & 1) * FBUF;
+        double *t = s_tile + (KPH & 1) * NREG;
 #pragma unroll
-        for (int c = 0; c < PREF; c++) t[tid + c * FT] = pre[c];
+        for (int c = 0; c < PREF; c++)
+            if (c + 1 < PREF || tid + c * FT < NREG) t[tid + c * FT] = pre[c];
         if constexpr (NEXT >= 0) fetch(NEXT);
         __syncthreads();
         const dgr::AxisConsts &kx = K[0], &ky = K[1], &kz = K[2];
@@ -664,8 +681,8 @@ struct FusedCtx {
 
     template <int V, int D>
     __device__ __forceinline__ double d1() const { return d1v[D]; }
-    template <int M, int C>
-    __device__ __forceinline__ double d2() const { return d2v[C]; }
+    template <int M, int K2>
+    __device__ __forceinline__ double d2() const { return d2v[K2]; }
     template <int V>
     __device__ __forceinline__ double fld() const { return s_fld[V * FT + tid]; }
     __device__ __forceinline__ void finish() {}
@@ -674,10 +691,11 @@ struct FusedCtx {
 // The equations' outputs: on a flagged face the boundary value (already
 // holding its KO term), elsewhere equations + KO term, kept in LDS until the
 // epilogue.
-template <bool FACE>
+template <bool FACE, class C>
 struct FusedOut {
-    const FusedCtx<FACE> &ctx;
-    __device__ explicit FusedOut(const FusedCtx<FACE> &c) : ctx(c) {}
+    static constexpr int FT = C::FT;
+    const FusedCtx<FACE, C> &ctx;
+    __device__ explicit FusedOut(const FusedCtx<FACE, C> &c) : ctx(c) {}
     template <int V>
     __device__ __forceinline__ void put(double val) {
 #pragma clang fp contract(off)
@@ -689,9 +707,11 @@ struct FusedOut {
 
 // RHS store or RK4 stage update of one point (rkBSSN.cpp:764-860), with the
 // loads of all 24 variables issued before any store.
+template <class C>
 __device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *s_ko, const double *s_fld, int tid,
                                                uint64_t idx, int64_t q, double chi_floor) {
 #pragma clang fp contract(off)
+    constexpr int FT = C::FT;
     const int s = io.stage;
     if (s < 0) {
 #pragma unroll
@@ -736,11 +756,12 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *
     }
 }
 
-template <bool FACE>
+template <bool FACE, class C>
 __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &T, const DBlock &B,
                                            const dgr::AxisConsts *K, double *s_tile, double *s_dx, double *s_dy,
                                            double *s_ko, double *s_fld) {
-    FusedCtx<FACE> ctx(io);
+    constexpr int FT = C::FT, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT;
+    FusedCtx<FACE, C> ctx(io);
     ctx.K = K;
     ctx.s_tile = s_tile;
     ctx.s_dx = s_dx;
@@ -757,7 +778,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     ctx.iez = nz - PW;
     const uint32_t bf = B.bflag;
     ctx.lox = bf & 1u; ctx.hix = bf & 2u; ctx.loy = bf & 4u; ctx.hiy = bf & 8u; ctx.loz = bf & 16u; ctx.hiz = bf & 32u;
-    ctx.ndx = TMAX * RMAX * (ez + 2 * PW);
+    ctx.ndx = TMAX * RMAX * (ez + 2 * PW);  // <= C::NDX
     ctx.ndy = TMAX * TMAX * (ez + 2 * PW);
     ctx.boff = B.offset;
     ctx.bq0 = B.q0;
@@ -832,26 +853,27 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     const double eta = rit_eta(coord(B.pmin[0], ctx.ax, B.h[0]), coord(B.pmin[1], ctx.ay, B.h[1]),
                                coord(B.pmin[2], ctx.az, B.h[2]), prm);
     ctx.fetch(dgr::kBssnFirstPhaseVar);
-    FusedOut<FACE> out(ctx);
+    FusedOut<FACE, C> out(ctx);
     dgr::bssn_rhs_phased<double>(ctx, eta, prm.lam, prm.lamf, out);
-    if (active) fused_epilogue(io, s_ko, s_fld, tid, idx, q, prm.chi_floor);
+    if (active) fused_epilogue<C>(io, s_ko, s_fld, tid, idx, q, prm.chi_floor);
 }
 
 // One launch per tile kind: FACE = false over the tiles of blocks without
-// flagged faces, FACE = true over the others.
-template <bool FACE>
-__global__ __launch_bounds__(FT) void k_fused(const StageIO io, const RhsParams prm, const DBlock *__restrict__ blocks,
-                                             const dgr::AxisConsts *__restrict__ bconst,
-                                             const Tile *__restrict__ tiles) {
-    __shared__ double s_tile[2 * FBUF];
-    __shared__ double s_dx[NDX];
-    __shared__ double s_dy[NDY];
-    __shared__ double s_ko[NV * FT];
-    __shared__ double s_fld[NV * FT];
+// flagged faces, FACE = true over the others; C the tile shape.
+template <bool FACE, class C>
+__global__ __launch_bounds__(C::FT) void k_fused(const StageIO io, const RhsParams prm,
+                                                const DBlock *__restrict__ blocks,
+                                                const dgr::AxisConsts *__restrict__ bconst,
+                                                const Tile *__restrict__ tiles) {
+    __shared__ double s_tile[2 * C::NREG];
+    __shared__ double s_dx[C::NDX];
+    __shared__ double s_dy[C::NDY];
+    __shared__ double s_ko[NV * C::FT];
+    __shared__ double s_fld[NV * C::FT];
 
     const Tile T = tiles[blockIdx.x];
     const DBlock B = blocks[T.block];
-    fused_tile<FACE>(io, prm, T, B, bconst + 3 * (size_t)T.block, s_tile, s_dx, s_dy, s_ko, s_fld);
+    fused_tile<FACE, C>(io, prm, T, B, bconst + 3 * (size_t)T.block, s_tile, s_dx, s_dy, s_ko, s_fld);
 }
 
 // enforce_bssn_constraints on interior points (standalone entry point)
@@ -908,11 +930,10 @@ struct dgr_plan {
     int64_t max_chunk_points = 0;
     double *d_stage = nullptr;  // split path RK4: 2 * 24 * szpdof (stage state, accumulator)
     bool fused = true;          // k_fused (default) or the split k_derivs/k_rhs path (DGR_KERNEL=split)
-    Tile *d_ftiles = nullptr;   // k_fused tiles (<= 7 x 7 x 5 points), XCD-grouped order:
-                                // blocks without flagged faces first, then the others
+    Tile *d_ftiles = nullptr;   // k_fused tiles, XCD-grouped, group k in [fgroup[k], fgroup[k+1])
     dgr::AxisConsts *d_bconst = nullptr;  // k_fused: x, y, z stencil constants per block
     int nftiles = 0;
-    int nftiles_inner = 0;      // the first nftiles_inner tiles belong to blocks with bflag == 0
+    int fgroup[NFGROUP + 1] = {};
     double *d_istage = nullptr; // k_fused RK4: 3 * 24 * interior (two stage states, accumulator)
     int *d_err = nullptr;
     // optional per-kernel timing: event pairs recorded around each launch
@@ -1060,19 +1081,35 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
     }
     close_chunk();
 
-    // k_fused tiles: at most 7 x 7 x 5 points (one point per thread of a 256-thread
-    // workgroup). Blocks are taken eight at a time and their tiles interleaved, so
-    // that (with workgroups dealt round-robin over the 8 XCDs) the tiles of one
-    // block share an XCD and its L2 for their overlapping halos.
+    // k_fused tiles: x and y cut into near-equal parts of at most 7 points.
+    // Along z, blocks without flagged faces are cut into parts of 5 with the
+    // remainder last; a remainder of 1 or 2 points makes a thin tile
+    // (128-thread launch). Blocks with flagged faces are cut into near-equal
+    // parts of at most 5 (each >= 4 points, so the one-sided closures, which
+    // reach 5 points along their axis, stay inside the staged region). Tiles
+    // are grouped by kind (interior-wide, interior-thin, face-wide); within a
+    // group, blocks are taken eight at a time and their tiles interleaved, so
+    // that (with workgroups dealt round-robin over the 8 XCDs) the tiles of
+    // one block share an XCD and its L2 for their overlapping halos.
     std::vector<Tile> ftiles;
     {
-        std::vector<std::vector<Tile>> per(nblocks);
+        std::vector<std::vector<Tile>> per[NFGROUP];
+        for (auto &v : per) v.resize(nblocks);
         for (uint32_t b = 0; b < nblocks; b++) {
             const DBlock &d = hb[b];
             std::vector<std::pair<int, int>> px, py, pz;
             split_axis_max(d.n[0] - 6, TMAX, px);
             split_axis_max(d.n[1] - 6, TMAX, py);
-            split_axis_max(d.n[2] - 6, FZMAX, pz);
+            if (d.bflag) {
+                split_axis_max(d.n[2] - 6, CfgWide::FZ, pz);
+            } else {
+                for (int o = PW, m = d.n[2] - 6; m > 0;) {
+                    const int e = std::min(m, CfgWide::FZ);
+                    pz.push_back({o, e});
+                    o += e;
+                    m -= e;
+                }
+            }
             for (auto &z : pz)
                 for (auto &y : py)
                     for (auto &x : px) {
@@ -1081,23 +1118,25 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
                         t.q0 = 0;
                         t.o[0] = x.first; t.o[1] = y.first; t.o[2] = z.first;
                         t.e[0] = x.second; t.e[1] = y.second; t.e[2] = z.second;
-                        per[b].push_back(t);
+                        const int kind = d.bflag ? FG_FACE : t.e[2] <= CfgThin::FZ ? FG_THIN : FG_WIDE;
+                        per[kind][b].push_back(t);
                     }
         }
-        for (int kind = 0; kind < 2; kind++) {  // blocks without flagged faces, then the others
+        for (int kind = 0; kind < NFGROUP; kind++) {
+            plan->fgroup[kind] = (int)ftiles.size();
             std::vector<uint32_t> bl;
             for (uint32_t b = 0; b < nblocks; b++)
-                if ((hb[b].bflag != 0) == (kind == 1)) bl.push_back(b);
+                if (!per[kind][b].empty()) bl.push_back(b);
             for (size_t g = 0; g < bl.size(); g += 8) {
                 const size_t ge = std::min(bl.size(), g + 8);
                 size_t mx = 0;
-                for (size_t i = g; i < ge; i++) mx = std::max(mx, per[bl[i]].size());
+                for (size_t i = g; i < ge; i++) mx = std::max(mx, per[kind][bl[i]].size());
                 for (size_t t = 0; t < mx; t++)
                     for (size_t i = g; i < ge; i++)
-                        if (t < per[bl[i]].size()) ftiles.push_back(per[bl[i]][t]);
+                        if (t < per[kind][bl[i]].size()) ftiles.push_back(per[kind][bl[i]][t]);
             }
-            if (kind == 0) plan->nftiles_inner = (int)ftiles.size();
         }
+        plan->fgroup[NFGROUP] = (int)ftiles.size();
     }
     plan->nftiles = (int)ftiles.size();
     std::vector<dgr::AxisConsts> bconst(3 * (size_t)nblocks);
@@ -1253,18 +1292,20 @@ int run_chunks(dgr_plan *plan, const VarPtrs &state, const VarPtrs &rhs, const R
     return DGR_OK;
 }
 
+template <bool FACE, class C>
+void launch_fused(dgr_plan *plan, int group, const StageIO &io, const RhsParams &prm, hipStream_t st) {
+    const int n = plan->fgroup[group + 1] - plan->fgroup[group];
+    if (!n) return;
+    timed(plan, 3, st, [&] {
+        hipLaunchKernelGGL((k_fused<FACE, C>), dim3(n), dim3(C::FT), 0, st, io, prm, plan->d_blocks, plan->d_bconst,
+                           plan->d_ftiles + plan->fgroup[group]);
+    });
+}
+
 int run_fused(dgr_plan *plan, const StageIO &io, const RhsParams &prm, hipStream_t st) {
-    const int ni = plan->nftiles_inner, nf = plan->nftiles - plan->nftiles_inner;
-    if (ni)
-        timed(plan, 3, st, [&] {
-            hipLaunchKernelGGL(k_fused<false>, dim3(ni), dim3(FT), 0, st, io, prm, plan->d_blocks, plan->d_bconst,
-                               plan->d_ftiles);
-        });
-    if (nf)
-        timed(plan, 3, st, [&] {
-            hipLaunchKernelGGL(k_fused<true>, dim3(nf), dim3(FT), 0, st, io, prm, plan->d_blocks, plan->d_bconst,
-                               plan->d_ftiles + ni);
-        });
+    launch_fused<false, CfgWide>(plan, FG_WIDE, io, prm, st);
+    launch_fused<false, CfgThin>(plan, FG_THIN, io, prm, st);
+    launch_fused<true, CfgWide>(plan, FG_FACE, io, prm, st);
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }
