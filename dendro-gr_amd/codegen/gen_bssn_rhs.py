@@ -564,6 +564,8 @@ namespace dgr {{
 
 // first variable of the phase order (the kernel fetches it before calling)
 constexpr int kBssnFirstPhaseVar = {vidx[0]};
+// the whole phase order (variable of phase K)
+constexpr int kBssnPhaseVar[{len(vidx)}] = {{{', '.join(str(v) for v in vidx)}}};
 
 template <typename T, typename Ctx, typename Out>
 DGR_HD DGR_INLINE void bssn_rhs_phased(Ctx &ctx, const T eta, const T *lam, const T *lamf, Out &out) {{

@@ -492,20 +492,25 @@ constexpr int DYS_Y = TMAX, DYS_Z = TMAX * TMAX;     // D_y u: tile x, tile y, r
 // workgroup per CU: it takes 143 KB of LDS); the thin shape, 128 threads /
 // z <= 2, takes the 2-point z remainder of a block (a 13^3 block's interior
 // is 7x7x5 + 7x7x2) in half the threads and ~80 KB, so two share a CU.
-template <int FT_, int FZ_>
+// PD variables are fetched ahead of the one being differentiated: with a
+// single wave per SIMD, the loads in flight are what keeps HBM busy.
+template <int FT_, int FZ_, int PD_>
 struct FCfg {
     static constexpr int FT = FT_;
     static constexpr int FZ = FZ_;
+    static constexpr int PD = PD_;                          // variables in flight ahead of the current one
     static constexpr int NREG = FRXY * (FZ + 2 * PW);       // staged region (13 x 13 x (FZ+6))
     static constexpr int PREF = (NREG + FT - 1) / FT;       // region elements per thread
     static constexpr int NDX = TMAX * RMAX * (FZ + 2 * PW);  // D_x points
     static constexpr int NDY = TMAX * TMAX * (FZ + 2 * PW);  // D_y points
     static constexpr int DXIT = (NDX + FT - 1) / FT;
     static constexpr int DYIT = (NDY + FT - 1) / FT;
-    static constexpr size_t LDS = sizeof(double) * (2 * NREG + NDX + NDY + 2 * NV * FT);
+    // LDS: two region buffers, D_x / D_y (padded to whole passes of the
+    // workgroup, so their evaluation needs no bounds test), KO terms
+    static constexpr size_t LDS = sizeof(double) * (2 * NREG + (DXIT + DYIT) * FT + NV * FT);
 };
-using CfgWide = FCfg<256, 5>;
-using CfgThin = FCfg<128, 2>;
+using CfgWide = FCfg<256, 5, 3>;
+using CfgThin = FCfg<128, 2, 2>;
 static_assert(7 * 7 * CfgWide::FZ <= CfgWide::FT && 7 * 7 * CfgThin::FZ <= CfgThin::FT, "one point per thread");
 static_assert(2 * CfgThin::LDS <= 160 * 1024, "two thin workgroups per CU");
 // tile groups of the fused path, one launch each
@@ -530,14 +535,13 @@ struct StageIO {
 // needed); interior tiles run a body without any of that code.
 template <bool FACE, class C>
 struct FusedCtx {
-    static constexpr int FT = C::FT, NREG = C::NREG, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT;
+    static constexpr int FT = C::FT, NREG = C::NREG, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT, PD = C::PD;
     const StageIO &io;
     const dgr::AxisConsts *K;  // x, y, z constants of this block
-    double *s_tile, *s_dx, *s_dy, *s_ko, *s_fld;
+    double *s_tile, *s_dx, *s_dy, *s_ko;
     // tile (uniform)
     int ox, oy, oz, iex, iey, iez;  // tile origin, ie = n - 3 per axis
     bool lox, hix, loy, hiy, loz, hiz;
-    int ndx, ndy;                   // D_x / D_y points of the staged region
     uint64_t boff;
     int64_t bq0;
     double pmin[3], h[3];
@@ -549,58 +553,62 @@ struct FusedCtx {
     // the interior-only stage buffer (-1: a padding point)
     uint32_t goff[PREF];
     int32_t qoff[PREF];
-    double pre[PREF];
+    double pre[PD][PREF];  // fetched region of the variables of phases K+1 .. K+PD
+    double f[NV];          // this point's field values
     // LDS source offsets of this thread's D_x / D_y evaluations
     int dxs[DXIT], dys[DYIT];
     double d1v[3], d2v[6];
 
     __device__ FusedCtx(const StageIO &io_) : io(io_) {}
 
-    __device__ __forceinline__ void fetch(int v) {
+    // region of the variable of phase K into prefetch slot K % PD
+    template <int K>
+    __device__ __forceinline__ void fetch() {
+        constexpr int v = dgr::kBssnPhaseVar[K];
+        double *dst = pre[K % PD];
         const double *__restrict__ ub = io.u.p[v] + boff;
         if (io.stage >= 1) {
             const double *__restrict__ sb = io.sin.p[v] + bq0;
 #pragma unroll
-            for (int c = 0; c < PREF; c++) pre[c] = qoff[c] >= 0 ? sb[qoff[c]] : ub[goff[c]];
+            for (int c = 0; c < PREF; c++) dst[c] = qoff[c] >= 0 ? sb[qoff[c]] : ub[goff[c]];
         } else {
 #pragma unroll
-            for (int c = 0; c < PREF; c++) pre[c] = ub[goff[c]];
+            for (int c = 0; c < PREF; c++) dst[c] = ub[goff[c]];
         }
     }
 
     template <int V, int NEXT, int KPH>
     __device__ __forceinline__ void phase() {
+        static_assert(dgr::kBssnPhaseVar[KPH] == V, "phase order");
         double *t = s_tile + (KPH & 1) * NREG;
 #pragma unroll
         for (int c = 0; c < PREF; c++)
-            if (c + 1 < PREF || tid + c * FT < NREG) t[tid + c * FT] = pre[c];
-        if constexpr (NEXT >= 0) fetch(NEXT);
+            if (c + 1 < PREF || tid + c * FT < NREG) t[tid + c * FT] = pre[KPH % PD][c];
+        if constexpr (KPH + PD < NV) fetch<KPH + PD>();
         __syncthreads();
         const dgr::AxisConsts &kx = K[0], &ky = K[1], &kz = K[2];
         constexpr int m2 = d2slot(V);
         if constexpr (m2 >= 0) {
             // D_x u on tile-x x region-y x region-z and D_y u on tile-x x tile-y x region-z
             // (the reference's mixed derivatives are D_y(D_x u), D_z(D_x u), D_z(D_y u))
+            // every thread evaluates DXIT / DYIT points; those beyond the
+            // tile's region read valid LDS and land in padding nobody reads
 #pragma unroll
             for (int it = 0; it < DXIT; it++) {
                 const int e = tid + it * FT;
-                if (e < ndx) {
-                    const double *s = t + dxs[it];
-                    double d = dgr::d1_in<1>(s, kx.d1_60);
-                    if (FACE && (lox || hix)) d = dgr::d1_edge<1>(s, d, ox + e % TMAX, iex, lox, hix, kx.d1_12);
-                    s_dx[e] = d;
-                }
+                const double *s = t + dxs[it];
+                double d = dgr::d1_in<1>(s, kx.d1_60);
+                if (FACE && (lox || hix)) d = dgr::d1_edge<1>(s, d, ox + e % TMAX, iex, lox, hix, kx.d1_12);
+                s_dx[e] = d;
             }
 #pragma unroll
             for (int it = 0; it < DYIT; it++) {
                 const int e = tid + it * FT;
-                if (e < ndy) {
-                    const double *s = t + dys[it];
-                    double d = dgr::d1_in<FRX>(s, ky.d1_60);
-                    if (FACE && (loy || hiy))
-                        d = dgr::d1_edge<FRX>(s, d, oy + (e / TMAX) % TMAX, iey, loy, hiy, ky.d1_12);
-                    s_dy[e] = d;
-                }
+                const double *s = t + dys[it];
+                double d = dgr::d1_in<FRX>(s, ky.d1_60);
+                if (FACE && (loy || hiy))
+                    d = dgr::d1_edge<FRX>(s, d, oy + (e / TMAX) % TMAX, iey, loy, hiy, ky.d1_12);
+                s_dy[e] = d;
             }
             __syncthreads();
         }
@@ -684,7 +692,7 @@ struct FusedCtx {
     template <int M, int K2>
     __device__ __forceinline__ double d2() const { return d2v[K2]; }
     template <int V>
-    __device__ __forceinline__ double fld() const { return s_fld[V * FT + tid]; }
+    __device__ __forceinline__ double fld() const { return f[V]; }
     __device__ __forceinline__ void finish() {}
 };
 
@@ -708,7 +716,7 @@ struct FusedOut {
 // RHS store or RK4 stage update of one point (rkBSSN.cpp:764-860), with the
 // loads of all 24 variables issued before any store.
 template <class C>
-__device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *s_ko, const double *s_fld, int tid,
+__device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *s_ko, const double *f, int tid,
                                                uint64_t idx, int64_t q, double chi_floor) {
 #pragma clang fp contract(off)
     constexpr int FT = C::FT;
@@ -721,7 +729,7 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *
     double u0[NV], a0[NV], v14[14];
     if (s == 0) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) u0[v] = a0[v] = s_fld[v * FT + tid];
+        for (int v = 0; v < NV; v++) u0[v] = a0[v] = f[v];
     } else {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -759,7 +767,7 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *
 template <bool FACE, class C>
 __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &T, const DBlock &B,
                                            const dgr::AxisConsts *K, double *s_tile, double *s_dx, double *s_dy,
-                                           double *s_ko, double *s_fld) {
+                                           double *s_ko) {
     constexpr int FT = C::FT, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT;
     FusedCtx<FACE, C> ctx(io);
     ctx.K = K;
@@ -767,7 +775,6 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     ctx.s_dx = s_dx;
     ctx.s_dy = s_dy;
     ctx.s_ko = s_ko;
-    ctx.s_fld = s_fld;
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const int ex = T.e[0], ey = T.e[1], ez = T.e[2];
     ctx.ox = T.o[0];
@@ -778,8 +785,6 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     ctx.iez = nz - PW;
     const uint32_t bf = B.bflag;
     ctx.lox = bf & 1u; ctx.hix = bf & 2u; ctx.loy = bf & 4u; ctx.hiy = bf & 8u; ctx.loz = bf & 16u; ctx.hiz = bf & 32u;
-    ctx.ndx = TMAX * RMAX * (ez + 2 * PW);  // <= C::NDX
-    ctx.ndy = TMAX * TMAX * (ez + 2 * PW);
     ctx.boff = B.offset;
     ctx.bq0 = B.q0;
     for (int a = 0; a < 3; a++) {
@@ -822,17 +827,18 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     for (int it = 0; it < DXIT; it++) {
         const int e = tid + it * FT;
         const int i0 = e % TMAX, r = e / TMAX;
-        ctx.dxs[it] = (PW + i0) + FRX * (r % RMAX) + FRXY * (r / RMAX);
+        // (beyond the static region: a point whose taps, closures included, stay in the buffer)
+        ctx.dxs[it] = e < C::NDX ? (PW + i0) + FRX * (r % RMAX) + FRXY * (r / RMAX) : PW + 2;
     }
 #pragma unroll
     for (int it = 0; it < DYIT; it++) {
         const int e = tid + it * FT;
         const int i0 = e % TMAX, r = e / TMAX;
-        ctx.dys[it] = (PW + i0) + FRX * (PW + r % TMAX) + FRXY * (r / TMAX);
+        ctx.dys[it] = e < C::NDY ? (PW + i0) + FRX * (PW + r % TMAX) + FRXY * (r / TMAX) : PW + FRX * (PW + 2);
     }
-    // this point's field values
+    // this point's field values (registers; the compiler parks them in AGPRs)
     {
-        double f[NV];
+        double *f = ctx.f;
         if (io.stage >= 1) {
 #pragma unroll
             for (int v = 0; v < NV; v++) f[v] = io.sin.p[v][q];
@@ -840,8 +846,6 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
 #pragma unroll
             for (int v = 0; v < NV; v++) f[v] = io.u.p[v][idx];
         }
-#pragma unroll
-        for (int v = 0; v < NV; v++) s_fld[v * FT + tid] = f[v];
         if (prm.cako) {
             const double sg = sqrt(f[1]);
             ctx.sig_gauge = sg * prm.eps_gauge;
@@ -852,10 +856,13 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     }
     const double eta = rit_eta(coord(B.pmin[0], ctx.ax, B.h[0]), coord(B.pmin[1], ctx.ay, B.h[1]),
                                coord(B.pmin[2], ctx.az, B.h[2]), prm);
-    ctx.fetch(dgr::kBssnFirstPhaseVar);
+    ctx.template fetch<0>();
+    if constexpr (C::PD > 1) ctx.template fetch<1>();
+    if constexpr (C::PD > 2) ctx.template fetch<2>();
+    static_assert(C::PD <= 3, "prefetch distance");
     FusedOut<FACE, C> out(ctx);
     dgr::bssn_rhs_phased<double>(ctx, eta, prm.lam, prm.lamf, out);
-    if (active) fused_epilogue<C>(io, s_ko, s_fld, tid, idx, q, prm.chi_floor);
+    if (active) fused_epilogue<C>(io, s_ko, ctx.f, tid, idx, q, prm.chi_floor);
 }
 
 // One launch per tile kind: FACE = false over the tiles of blocks without
@@ -866,14 +873,13 @@ __global__ __launch_bounds__(C::FT) void k_fused(const StageIO io, const RhsPara
                                                 const dgr::AxisConsts *__restrict__ bconst,
                                                 const Tile *__restrict__ tiles) {
     __shared__ double s_tile[2 * C::NREG];
-    __shared__ double s_dx[C::NDX];
-    __shared__ double s_dy[C::NDY];
+    __shared__ double s_dx[C::DXIT * C::FT];
+    __shared__ double s_dy[C::DYIT * C::FT];
     __shared__ double s_ko[NV * C::FT];
-    __shared__ double s_fld[NV * C::FT];
 
     const Tile T = tiles[blockIdx.x];
     const DBlock B = blocks[T.block];
-    fused_tile<FACE, C>(io, prm, T, B, bconst + 3 * (size_t)T.block, s_tile, s_dx, s_dy, s_ko, s_fld);
+    fused_tile<FACE, C>(io, prm, T, B, bconst + 3 * (size_t)T.block, s_tile, s_dx, s_dy, s_ko);
 }
 
 // enforce_bssn_constraints on interior points (standalone entry point)
