@@ -8,10 +8,14 @@ dissipation, stage update, constraint enforcement per stage), FP64, inputs
 resident in HBM. Reports interior Mpoints/s = interior points x 4 / step time.
 
 Prints one JSON line. Also reports:
-  roofline      the RK stage (all kernels of one stage: derivatives, equations,
-                boundary) against HBM: algorithmic bytes per stage
-                (192 n^3/(n-6)^3 + 192 + 768 bytes per interior point) over the
-                stage time measured with HIP events on the launch stream
+  roofline      the dominant kernel (the fused kernel on the regular tiles of
+                blocks without flagged faces) against HBM: algorithmic bytes
+                per launch (192 n^3/(n-6)^3 + 192 + 768 bytes per interior point
+                of its tiles: padded state read once, RHS produced once, RK
+                stage update) over its average launch time, measured with HIP
+                events on the launch stream; `traffic` is the HBM bytes per
+                launch measured by rocprofv3 counters (profiles/), when a
+                summary for this workload is committed
   cpu_baseline  the oracle's OpenMP C restatement on the host cores, one RK4
                 step of a 1024-block sample of the same batch
   rhs_only      right-hand side alone (no RK update), Mpoints/s
@@ -40,6 +44,20 @@ def stage_bytes_per_point(n):
     read once, RHS written once, plus the stage update (read u0, read and write
     the accumulator, write the next stage)."""
     return 192.0 * n ** 3 / (n - 6) ** 3 + 192.0 + 768.0
+
+
+def measured_traffic(kind, nblocks):
+    """HBM bytes per launch of `kind` from the committed rocprofv3 counter
+    summary of this workload (profiles/*traffic*.txt lines
+    'traffic <kind> uniform13x<nblocks> <bytes>'), else None."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.txt"))):
+        with open(f) as fh:
+            for line in fh:
+                p = line.split()
+                if len(p) == 4 and p[0] == "traffic" and p[1] == kind and p[2] == f"uniform13x{nblocks}":
+                    return float(p[3])
+    return None
 
 
 def cpu_baseline(nblocks, threads, reps=3):
@@ -148,16 +166,34 @@ def main():
 
     stages = 4 * args.steps
     stage_ms = sum(v[0] for v in kt.values()) / stages
-    alg_bytes = stage_bytes_per_point(13) * npts
-    achieved = alg_bytes / (stage_ms * 1e-3) / 1e9
+    bpp = stage_bytes_per_point(13)
+    kind_pts = W.fused_kind_points(blocks)
+    per_kernel = {}
+    for k, (ms, nl) in kt.items():
+        if not nl:
+            continue
+        entry = {"ms_per_launch": round(ms / nl, 5), "launches_per_stage": nl / stages}
+        if k in kind_pts and kind_pts[k]:
+            entry["points"] = kind_pts[k]
+            entry["achieved_GBs"] = round(bpp * kind_pts[k] / (ms / nl * 1e-3) / 1e9, 1)
+        per_kernel[k] = entry
+    dom = max(per_kernel, key=lambda k: kt[k][0])
+    dom_ms = kt[dom][0] / kt[dom][1]
+    dom_bytes = bpp * kind_pts.get(dom, npts)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom, args.nblocks)
+    names = {"fused_wide": "k_fused<interior, 256 threads, 7x7x5 tiles>",
+             "fused_thin": "k_fused<interior, 128 threads, 7x7x2 tiles>",
+             "fused_face": "k_fused<flagged faces, 256 threads>",
+             "derivs": "k_derivs", "rhs": "k_rhs", "face": "k_face"}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": ("k_fused (one launch per RK stage)" if kt["fused"][1] else
-                           "RK stage = k_derivs + k_rhs over all chunks"),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": names.get(dom, dom), "launch_ms": round(dom_ms, 5),
+                "algorithmic_bytes_per_launch": dom_bytes,
+                "algorithmic_bytes_per_point": round(bpp, 1),
                 "stage_ms": round(stage_ms, 4),
-                "per_kernel_ms_per_stage": {k: round(v[0] / stages, 4) for k, v in kt.items()},
-                "launches_per_stage": {k: v[1] / stages for k, v in kt.items()},
-                "algorithmic_bytes_per_stage": alg_bytes}
+                "stage_GBs": round(bpp * npts / (stage_ms * 1e-3) / 1e9, 1),
+                "per_kernel": per_kernel}
     del out
     plan.close()
 

@@ -945,7 +945,7 @@ struct dgr_plan {
     // optional per-kernel timing: event pairs recorded around each launch
     bool timing = false;
     std::vector<hipEvent_t> event_pool;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[4];
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[DGR_TIMING_KINDS];
 
     hipEvent_t take_event() {
         if (event_pool.empty()) {
@@ -1210,7 +1210,7 @@ int dgr_plan_set_timing(dgr_plan *plan, int enable) {
 int dgr_plan_kernel_times(dgr_plan *plan, double *ms, int64_t *launches) {
     if (!plan || !ms || !launches) return fail(DGR_E_INVALID, "null argument");
     HIP_OK(hipSetDevice(plan->device));
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < DGR_TIMING_KINDS; k++) {
         ms[k] = 0.0;
         launches[k] = (int64_t)plan->pending[k].size();
         for (auto &pr : plan->pending[k]) {
@@ -1302,7 +1302,7 @@ template <bool FACE, class C>
 void launch_fused(dgr_plan *plan, int group, const StageIO &io, const RhsParams &prm, hipStream_t st) {
     const int n = plan->fgroup[group + 1] - plan->fgroup[group];
     if (!n) return;
-    timed(plan, 3, st, [&] {
+    timed(plan, 3 + group, st, [&] {
         hipLaunchKernelGGL((k_fused<FACE, C>), dim3(n), dim3(C::FT), 0, st, io, prm, plan->d_blocks, plan->d_bconst,
                            plan->d_ftiles + plan->fgroup[group]);
     });

@@ -145,12 +145,15 @@ class Plan:
     def set_timing(self, enable: bool = True):
         _check(load().dgr_plan_set_timing(self._h, int(enable)), "dgr_plan_set_timing")
 
+    TIMING_KINDS = ("derivs", "rhs", "face", "fused_wide", "fused_thin", "fused_face")
+
     def kernel_times(self):
         """{kind: (total ms, launches)} since the previous call."""
-        ms = (C.c_double * 4)()
-        n = (C.c_int64 * 4)()
+        nk = len(self.TIMING_KINDS)
+        ms = (C.c_double * nk)()
+        n = (C.c_int64 * nk)()
         _check(load().dgr_plan_kernel_times(self._h, ms, n), "dgr_plan_kernel_times")
-        return {k: (ms[i], n[i]) for i, k in enumerate(("derivs", "rhs", "face", "fused"))}
+        return {k: (ms[i], n[i]) for i, k in enumerate(self.TIMING_KINDS)}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -93,3 +93,22 @@ def puncture_octree(scale: float = 0.42, align: int = 1) -> np.ndarray:
         hs.append(h)
         pmins.append(chosen[e] + shift + 0.02 - PW * h)
     return _table(sizes, pmins, hs, align=align)
+
+
+def fused_kind_points(t: np.ndarray) -> dict:
+    """Interior points each launch of the fused path covers, by tile kind, with
+    the plan's tiling rule: blocks with flagged faces -> 'fused_face'; other
+    blocks are cut along z into parts of 5 points, a remainder of 1-2 points
+    forming 'fused_thin' tiles and everything else 'fused_wide' tiles."""
+    pts = {"fused_wide": 0, "fused_thin": 0, "fused_face": 0}
+    for b in range(len(t)):
+        m = (t["sz"][b].astype(np.int64) - 2 * PW)
+        xy = int(m[0] * m[1])
+        if t["bflag"][b]:
+            pts["fused_face"] += xy * int(m[2])
+            continue
+        r = int(m[2]) % 5
+        thin = r if r in (1, 2) else 0
+        pts["fused_thin"] += xy * thin
+        pts["fused_wide"] += xy * (int(m[2]) - thin)
+    return pts

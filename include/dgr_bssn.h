@@ -100,10 +100,13 @@ int64_t dgr_plan_interior_points(const dgr_plan *plan);
 /* Kernel timing (no counterpart in the reference, whose profiler_t timers wrap
  * the phases of bssnrhs, include/profile_params.h:15-53): when enabled, every
  * launch is bracketed by HIP events on the caller's stream.
- * dgr_plan_kernel_times waits for them and returns, per kernel kind
- * (0 derivatives, 1 equations, 2 boundary faces — the split path; 3 the fused
- * kernel), the summed milliseconds and the number of launches since the
- * previous call; `ms` and `launches` have 4 entries. */
+ * dgr_plan_kernel_times waits for them and returns, per kernel kind, the
+ * summed milliseconds and the number of launches since the previous call;
+ * `ms` and `launches` have DGR_TIMING_KINDS entries: 0 derivatives,
+ * 1 equations, 2 boundary faces (the split path); 3 fused kernel on regular
+ * tiles of blocks without flagged faces, 4 on thin tiles, 5 on tiles of
+ * blocks with flagged faces. */
+#define DGR_TIMING_KINDS 6
 int dgr_plan_set_timing(dgr_plan *plan, int enable);
 int dgr_plan_kernel_times(dgr_plan *plan, double *ms, int64_t *launches);
 
