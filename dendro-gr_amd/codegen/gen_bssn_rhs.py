@@ -15,7 +15,9 @@ The emitted function reads inputs through an accessor object
 and hands each right-hand side to `out.template put<V>(value)`, so a kernel
 decides where inputs come from and what happens to each output.
 
-Usage: python gen_bssn_rhs.py [--cse=local|global] [--phased] [output.h]
+Usage: python gen_bssn_rhs.py [--cse=local|global] [--phased | --ham] [output.h]
+(--phased: the right-hand side in the phase order of the fused kernel;
+ --ham: the Hamiltonian constraint)
 (local: common subexpressions are shared within one equation or intermediate
 only, which bounds the live set; global: shared across all of them.)
 """
@@ -192,6 +194,11 @@ for i, j in PAIRS:
     OUTPUTS[VIDX["At0"] + SYM6[i][j]] = At_rhs[i, j]
 
 
+# Hamiltonian constraint (CodeGen/constraints.py:250-254; dendro.sqr, dendro.py:309-318):
+# chi igt^jk R_jk - At_ij At^ij + 2/3 K^2, with the same R as the right-hand side.
+HAM = (sum(chi * igt[j, k] * R[j, k] for j, k in E_IJ) - sum(At[i, j] * AtUU[i, j] for i, j in E_IJ)
+       + Rational(2, 3) * K ** 2)
+
 # ---------------------------------------------------------------------------
 # code emission
 
@@ -293,11 +300,24 @@ def schedule(defs, outs):
     return rev[::-1]
 
 
-def eliminate():
+def _reachable_defs(outputs):
+    """The named intermediates the given outputs depend on, in definition order."""
+    by_sym = dict(DEFS)
+    need, stack = set(), [x for e in outputs for x in sp.sympify(e).free_symbols]
+    while stack:
+        x = stack.pop()
+        if x in by_sym and x not in need:
+            need.add(x)
+            stack.extend(by_sym[x].free_symbols)
+    return [(s_, e) for s_, e in DEFS if s_ in need]
+
+
+def eliminate(outputs=None):
     """Common-subexpression elimination; returns (definitions, outputs, op count)."""
-    outs = [sp.sympify(e) for e in OUTPUTS]
-    def_syms = [s for s, _ in DEFS]
-    exprs = [e for _, e in DEFS] + outs
+    outs = [sp.sympify(e) for e in (OUTPUTS if outputs is None else outputs)]
+    defs_in = DEFS if outputs is None else _reachable_defs(outs)
+    def_syms = [s for s, _ in defs_in]
+    exprs = [e for _, e in defs_in] + outs
     if CSE_MODE == "global":
         repl, reduced = sp.cse(exprs, symbols=sp.numbered_symbols("c"), optimizations="basic")
     else:  # per expression: sharing across expressions only through named intermediates
@@ -308,9 +328,9 @@ def eliminate():
             repl += r
             reduced.append(rd[0])
     defs = dict(repl)
-    for s, e in zip(def_syms, reduced[:len(DEFS)]):
+    for s, e in zip(def_syms, reduced[:len(defs_in)]):
         defs[s] = e
-    outs_r = reduced[len(DEFS):]
+    outs_r = reduced[len(defs_in):]
     ops = sum(sp.count_ops(e) for e in defs.values()) + sum(sp.count_ops(e) for e in outs_r)
     return defs, outs_r, ops
 
@@ -324,7 +344,7 @@ HEADER_TOP = """// Generated by dendro-gr_amd/codegen/gen_bssn_rhs.py -- do not 
 """
 
 
-def emit(order, defs, outs_r, input_fn, phase_line=None, inline_fields=False, pin=False):
+def emit(order, defs, outs_r, input_fn, phase_line=None, inline_fields=False, pin=False, out_names=None):
     """C++ statements for a program order of ('in'|'tmp'|'out', x) nodes, with
     optional ('phase', p) markers; returns (lines, temporaries, max live).
     With inline_fields, every use of a field reads it again (the kernel keeps
@@ -364,7 +384,7 @@ def emit(order, defs, outs_r, input_fn, phase_line=None, inline_fields=False, pi
             names[x] = nm
             live.add(x)
         else:
-            lines.append(f"    out.template put<{x}>({pr.doprint(outs_r[x])});  // {VARS[x]}")
+            lines.append(f"    out.template put<{x}>({pr.doprint(outs_r[x])});  // {(out_names or VARS)[x]}")
         max_live = max(max_live, len(live))
         live = {y for y in live if last_use.get(y, -1) > k}
     return lines, ntmp, max_live
@@ -580,6 +600,47 @@ DGR_HD DGR_INLINE void bssn_rhs_phased(Ctx &ctx, const T eta, const T *lam, cons
           f"{time.time() - t0:.1f}s")
 
 
+def generate_ham(path):
+    t0 = time.time()
+    defs, outs_r, ops_after = eliminate([HAM])
+    order = schedule(defs, outs_r)
+    lines, ntmp, max_live = emit(order, defs, outs_r, input_expr, out_names=["ham"])
+    hdr = """// Generated by dendro-gr_amd/codegen/gen_bssn_rhs.py --ham -- do not edit.
+//
+// Pointwise Hamiltonian constraint of the BSSN variables (CodeGen/constraints.py:
+// 250-254, which the reference expands into BSSN_GR/src/physconeqs.cpp):
+// chi igt^jk R_jk - At_ij At^ij + 2/3 K^2, built from the same named
+// intermediates (inverse metric, Christoffel symbols, Ricci tensor, raised At)
+// as the right-hand side.
+""" + f"""// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values live.
+//
+// In: the accessor of bssn_rhs_point (fields, first derivatives of chi, Gt,
+//     gt; second derivatives of chi and gt are the ones read).
+// Out: receives put<0>(ham).
+#pragma once
+
+#ifndef DGR_HD
+#define DGR_HD
+#endif
+#ifndef DGR_INLINE
+#define DGR_INLINE inline
+#endif
+
+namespace dgr {{
+
+template <typename T, typename In, typename Out>
+DGR_HD DGR_INLINE void bssn_ham_point(const In &in, Out &out) {{
+{chr(10).join(lines)}
+}}
+
+}}  // namespace dgr
+"""
+    with open(path, "w") as fh:
+        fh.write(hdr)
+    print(f"generated {path}: {ntmp} temporaries, ops {ops_after}, max live {max_live}, "
+          f"{time.time() - t0:.1f}s")
+
+
 CSE_MODE = "local"
 
 if __name__ == "__main__":
@@ -589,5 +650,7 @@ if __name__ == "__main__":
             CSE_MODE = a.split("=", 1)[1]
     if "--phased" in sys.argv:
         generate_phased(args[0] if args else "bssn_rhs_phased.h")
+    elif "--ham" in sys.argv:
+        generate_ham(args[0] if args else "bssn_ham_generated.h")
     else:
         generate(args[0] if args else "bssn_rhs_generated.h")

@@ -1,4 +1,5 @@
-// Host build of the generated pointwise BSSN equations, for checking the
+// Host build of the generated pointwise BSSN equations and Hamiltonian
+// constraint, for checking the
 // generated code itself on the CPU against the reference's golden vectors.
 // Not part of the device path. Input layout per point: 163 doubles
 // ([0,24) fields, [24,96) first derivatives 3*v+d, [96,162) second
@@ -6,6 +7,7 @@
 #include <cstdint>
 
 #include "bssn_rhs_generated.h"
+#include "bssn_ham_generated.h"
 
 namespace {
 template <typename T>
@@ -34,7 +36,21 @@ void run(const double *in, int64_t n, const double *lam, const double *lamf, dou
         dgr::bssn_rhs_point<T>(a, (T)in[163 * q + 162], l, lf, b);
     }
 }
+template <typename T>
+struct Out1 {
+    double *o;
+    template <int V>
+    void put(T x) { *o = (double)x; }
+};
 }  // namespace
+
+extern "C" void dgr_host_ham_points(const double *in, int64_t n, double *out) {
+    for (int64_t q = 0; q < n; q++) {
+        In163<double> a{in + 163 * q};
+        Out1<double> b{out + q};
+        dgr::bssn_ham_point<double>(a, b);
+    }
+}
 
 extern "C" void dgr_host_bssn_points(const double *in, int64_t n, const double *lam, const double *lamf,
                                      double *out) {

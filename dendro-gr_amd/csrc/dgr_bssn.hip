@@ -41,6 +41,7 @@ __host__ __device__ inline T dgr_sqrt(T x) {
 #include "bssn_rhs_generated.h"
 #include "stencils.h"
 #include "bssn_rhs_phased.h"
+#include "bssn_ham_generated.h"
 
 namespace {
 
@@ -882,6 +883,119 @@ __global__ __launch_bounds__(C::FT) void k_fused(const StageIO io, const RhsPara
     fused_tile<FACE, C>(io, prm, T, B, bconst + 3 * (size_t)T.block, s_tile, s_dx, s_dy, s_ko);
 }
 
+// -------------------------------------------------------------------------
+// Hamiltonian constraint (C_HAM of physical_constraints, physcon.cpp:12-89):
+// the generated constraint at every interior point of a chunk, from the
+// derivative workspace k_derivs fills with the same calls the reference's
+// constraint_derivs.h makes. A diagnostic: it reuses the split path rather
+// than a fused kernel of its own.
+// -------------------------------------------------------------------------
+struct HamOut {
+    double v;
+    template <int V>
+    __device__ __forceinline__ void put(double x) { v = x; }
+};
+
+__global__ __launch_bounds__(NTHREADS) void k_ham(VarPtrs u, double *__restrict__ ham,
+                                                 const DBlock *__restrict__ blocks,
+                                                 const Tile *__restrict__ tiles, const int *__restrict__ wg_tile,
+                                                 int tile_end, const double *__restrict__ W, int64_t P) {
+    const int64_t q = (int64_t)blockIdx.x * NTHREADS + threadIdx.x;
+    if (q >= P) return;
+    int t = wg_tile[blockIdx.x];
+    while (t + 1 < tile_end && q >= tiles[t + 1].q0) t++;
+    const Tile T = tiles[t];
+    const DBlock B = blocks[T.block];
+    const PointGeom g = locate(T, B, (int)(q - T.q0));
+    PointIn in{&u, g.idx, W, P, q};
+    HamOut out;
+    dgr::bssn_ham_point<double>(in, out);
+    ham[g.idx] = out.v;
+}
+
+// computeConstraintL2Norm (grUtils.tcc:37-134): per workgroup, the sum of c^2
+// and the count over interior points at distance >= r1 from BH 1 and >= r2
+// from BH 2; k_norm_final adds the partials in a fixed order.
+struct NormMask {
+    double bh[6];
+    double r1, r2;
+};
+
+__global__ __launch_bounds__(NTHREADS) void k_norm_partial(const double *__restrict__ c, NormMask m,
+                                                          const DBlock *__restrict__ blocks,
+                                                          const Tile *__restrict__ tiles,
+                                                          const int *__restrict__ wg_tile, int tile_end, int64_t P,
+                                                          double *__restrict__ psum,
+                                                          unsigned long long *__restrict__ pcnt) {
+#pragma clang fp contract(off)
+    __shared__ double s_sum[NTHREADS];
+    __shared__ unsigned long long s_cnt[NTHREADS];
+    const int64_t q = (int64_t)blockIdx.x * NTHREADS + threadIdx.x;
+    double v2 = 0.0;
+    unsigned long long cnt = 0;
+    if (q < P) {
+        int t = wg_tile[blockIdx.x];
+        while (t + 1 < tile_end && q >= tiles[t + 1].q0) t++;
+        const Tile T = tiles[t];
+        const DBlock B = blocks[T.block];
+        const PointGeom g = locate(T, B, (int)(q - T.q0));
+        const double x = coord(B.pmin[0], g.i, B.h[0]);
+        const double y = coord(B.pmin[1], g.j, B.h[1]);
+        const double z = coord(B.pmin[2], g.k, B.h[2]);
+        const double d1 = sqrt((x - m.bh[0]) * (x - m.bh[0]) + (y - m.bh[1]) * (y - m.bh[1]) +
+                               (z - m.bh[2]) * (z - m.bh[2]));
+        const double d2 = sqrt((x - m.bh[3]) * (x - m.bh[3]) + (y - m.bh[4]) * (y - m.bh[4]) +
+                               (z - m.bh[5]) * (z - m.bh[5]));
+        if (!(d1 < m.r1 || d2 < m.r2)) {
+            const double v = c[g.idx];
+            v2 = v * v;
+            cnt = 1;
+        }
+    }
+    s_sum[threadIdx.x] = v2;
+    s_cnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = NTHREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        psum[blockIdx.x] = s_sum[0];
+        pcnt[blockIdx.x] = s_cnt[0];
+    }
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_norm_final(const double *__restrict__ psum,
+                                                        const unsigned long long *__restrict__ pcnt, int n,
+                                                        double *__restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double s_sum[NTHREADS];
+    __shared__ unsigned long long s_cnt[NTHREADS];
+    double a = 0.0;
+    unsigned long long k = 0;
+    for (int i = threadIdx.x; i < n; i += NTHREADS) {
+        a += psum[i];
+        k += pcnt[i];
+    }
+    s_sum[threadIdx.x] = a;
+    s_cnt[threadIdx.x] = k;
+    __syncthreads();
+    for (int w = NTHREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s_sum[0];
+        out[1] = (double)s_cnt[0];
+    }
+}
+
 // enforce_bssn_constraints on interior points (standalone entry point)
 __global__ __launch_bounds__(NTHREADS) void k_enforce(VarPtrs u, const DBlock *__restrict__ blocks,
                                                      const Tile *__restrict__ tiles, const int *__restrict__ wg_tile,
@@ -942,6 +1056,8 @@ struct dgr_plan {
     int fgroup[NFGROUP + 1] = {};
     double *d_istage = nullptr; // k_fused RK4: 3 * 24 * interior (two stage states, accumulator)
     int *d_err = nullptr;
+    int nwg_total = 0;                    // k_rhs-style workgroups over all chunks
+    double *d_norm = nullptr;             // norm partials: nwg_total sums, nwg_total counts, 2 results
     // optional per-kernel timing: event pairs recorded around each launch
     bool timing = false;
     std::vector<hipEvent_t> event_pool;
@@ -1086,6 +1202,7 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
         plan->interior += pts;
     }
     close_chunk();
+    plan->nwg_total = (int)wg_tile.size();
 
     // k_fused tiles: x and y cut into near-equal parts of at most 7 points.
     // Along z, blocks without flagged faces are cut into parts of 5 with the
@@ -1195,6 +1312,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     hipFree(plan->d_bconst);
     hipFree(plan->d_istage);
     hipFree(plan->d_err);
+    hipFree(plan->d_norm);
     delete plan;
     return DGR_OK;
 }
@@ -1424,6 +1542,60 @@ int dgr_enforce_constraints(dgr_plan *plan, double *const *d_u, double chi_floor
                            plan->d_tiles, plan->d_wg_tile + c.wg_begin, c.tile_end, (int64_t)c.npoints, chi_floor,
                            plan->d_err);
     HIP_OK(hipGetLastError());
+    return DGR_OK;
+}
+
+int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, void *stream) {
+    g_err.clear();
+    if (!plan || !d_ham) return fail(DGR_E_INVALID, "null plan or d_ham");
+    VarPtrs u;
+    int rc;
+    if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
+    HIP_OK(hipSetDevice(plan->device));
+    if (!plan->d_W && plan->max_chunk_points) {
+        hipError_t e = hipMalloc(&plan->d_W, sizeof(double) * NSLOT * (size_t)plan->max_chunk_points);
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "derivative workspace");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    for (const Chunk &c : plan->chunks) {
+        const int ntile = c.tile_end - c.tile_begin;
+        hipLaunchKernelGGL(k_derivs, dim3(ntile, NV), dim3(NTHREADS), 0, st, u, plan->d_blocks, plan->d_tiles,
+                           c.tile_begin, plan->d_W, (int64_t)c.npoints);
+        hipLaunchKernelGGL(k_ham, dim3(c.nwg), dim3(NTHREADS), 0, st, u, d_ham, plan->d_blocks, plan->d_tiles,
+                           plan->d_wg_tile + c.wg_begin, c.tile_end, plan->d_W, (int64_t)c.npoints);
+    }
+    HIP_OK(hipGetLastError());
+    return DGR_OK;
+}
+
+int dgr_constraint_l2_norm(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2,
+                           double *norm, void *stream) {
+    g_err.clear();
+    if (!plan || !d_c || !bh || !norm) return fail(DGR_E_INVALID, "null argument");
+    HIP_OK(hipSetDevice(plan->device));
+    const int nwg = plan->nwg_total;
+    if (!plan->d_norm) {
+        hipError_t e = hipMalloc(&plan->d_norm, sizeof(double) * (2 * (size_t)nwg + 2));
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "norm partials");
+    }
+    double *psum = plan->d_norm;
+    auto *pcnt = reinterpret_cast<unsigned long long *>(plan->d_norm + nwg);
+    double *res = plan->d_norm + 2 * (size_t)nwg;
+    NormMask m;
+    for (int i = 0; i < 6; i++) m.bh[i] = bh[i];
+    m.r1 = r1;
+    m.r2 = r2;
+    hipStream_t st = (hipStream_t)stream;
+    for (const Chunk &c : plan->chunks)
+        hipLaunchKernelGGL(k_norm_partial, dim3(c.nwg), dim3(NTHREADS), 0, st, d_c, m, plan->d_blocks, plan->d_tiles,
+                           plan->d_wg_tile + c.wg_begin, c.tile_end, (int64_t)c.npoints, psum + c.wg_begin,
+                           pcnt + c.wg_begin);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NTHREADS), 0, st, psum, pcnt, nwg, res);
+    HIP_OK(hipGetLastError());
+    double h[2] = {0.0, 0.0};
+    HIP_OK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    *norm = h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : -1.0;
     return DGR_OK;
 }
 

@@ -21,6 +21,7 @@ SYMBOLS = [
     "dgr_default_params", "dgr_last_error", "dgr_plan_create", "dgr_plan_destroy",
     "dgr_plan_interior_points", "dgr_bssn_rhs", "dgr_rk4_step", "dgr_enforce_constraints",
     "dgr_plan_status", "dgr_plan_set_timing", "dgr_plan_kernel_times",
+    "dgr_ham_constraint", "dgr_constraint_l2_norm",
 ]
 
 
@@ -70,6 +71,9 @@ def load():
         L.dgr_plan_status.argtypes = [vp, vp]
         L.dgr_plan_set_timing.argtypes = [vp, C.c_int]
         L.dgr_plan_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
+        L.dgr_constraint_l2_norm.argtypes = [vp, vp, C.POINTER(C.c_double), C.c_double, C.c_double,
+                                             C.POINTER(C.c_double), vp]
         _lib = L
     return _lib
 
@@ -138,6 +142,21 @@ class Plan:
     def enforce(self, state, chi_floor: float, stream=None):
         _check(load().dgr_enforce_constraints(self._h, var_pointers(state), chi_floor,
                                               _stream_ptr(stream)), "dgr_enforce_constraints")
+
+    def ham(self, out, state, stream=None):
+        """Hamiltonian constraint at the interior points into `out` (a szpdof
+        float64 CUDA tensor)."""
+        assert out.dim() == 1 and out.is_cuda and str(out.dtype) == "torch.float64"
+        _check(load().dgr_ham_constraint(self._h, C.c_void_p(out.data_ptr()), var_pointers(state),
+                                         _stream_ptr(stream)), "dgr_ham_constraint")
+
+    def constraint_norm(self, c, bh=((3.0, 0.0, 0.0), (-3.0, 0.0, 0.0)), r1: float = 5.0,
+                        r2: float = 5.0, stream=None) -> float:
+        b = (C.c_double * 6)(*[float(v) for p in bh for v in p])
+        res = C.c_double()
+        _check(load().dgr_constraint_l2_norm(self._h, C.c_void_p(c.data_ptr()), b, r1, r2, C.byref(res),
+                                             _stream_ptr(stream)), "dgr_constraint_l2_norm")
+        return res.value
 
     def status(self, stream=None) -> int:
         return int(load().dgr_plan_status(self._h, _stream_ptr(stream)))

@@ -136,6 +136,23 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
  * floored at chi_floor. */
 int dgr_enforce_constraints(dgr_plan *plan, double *const *d_u, double chi_floor, void *stream);
 
+/* Hamiltonian constraint chi igt^jk R_jk - At_ij At^ij + 2/3 K^2 at the interior
+ * points of every block, into d_ham (one array of szpdof doubles laid out like a
+ * variable; padding not written). Replaces the C_HAM output of
+ * physical_constraints (BSSN_GR/src/physcon.cpp:12-89, generated equations
+ * physconeqs.cpp:1373, derivatives constraint_derivs.h). Asynchronous on
+ * `stream`; uses the plan's derivative workspace (allocated on first use). */
+int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, void *stream);
+
+/* computeConstraintL2Norm (BSSN_GR/include/grUtils.tcc:37-134): sqrt of the mean
+ * of c^2 over the interior points whose distance to (bh[0],bh[1],bh[2]) is >= r1
+ * and to (bh[3],bh[4],bh[5]) is >= r2 (the reference's BSSN_BH_LOC and
+ * BSSN_BH{1,2}_CONSTRAINT_R, default 5.0, parameters.cpp:71-72). `d_c` is laid
+ * out like a variable (e.g. the output of dgr_ham_constraint). Waits for
+ * `stream`; *norm = -1 when no point qualifies. */
+int dgr_constraint_l2_norm(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2,
+                           double *norm, void *stream);
+
 /* Waits for `stream`, then reports and clears constraint violations met by
  * dgr_rk4_step / dgr_enforce_constraints since the last call: DGR_E_CONSTRAINT
  * for det(gt) < 0 or a projection off by more than 1e-6 (where the reference

@@ -214,13 +214,16 @@ static int d2slot(int v) {
     }
 }
 
-void orc_bssn_point(const double *in, const orc_params *p, double *rhs) {
+/* The pointwise algebra shared by the right-hand side (rhs != NULL, needs p)
+ * and the Hamiltonian constraint (ham != NULL). */
+static void point_algebra(const double *in, const orc_params *p, double *rhs, double *ham) {
     const double *f = in;
 #define D1(v, d) in[24 + 3 * (v) + (d)]
 #define D2(v, a, b) in[96 + 6 * d2slot(v) + SYM6[a][b]]
     const double eta = in[162];
-    const double l1 = p->lambda[0], l2 = p->lambda[1], l3 = p->lambda[2], l4 = p->lambda[3];
-    const double lf0 = p->lambda_f[0], lf1 = p->lambda_f[1];
+    const double l1 = p ? p->lambda[0] : 0.0, l2 = p ? p->lambda[1] : 0.0;
+    const double l3 = p ? p->lambda[2] : 0.0, l4 = p ? p->lambda[3] : 0.0;
+    const double lf0 = p ? p->lambda_f[0] : 0.0, lf1 = p ? p->lambda_f[1] : 0.0;
 
     const double alpha = f[ORC_ALPHA], chi = f[ORC_CHI], K = f[ORC_K];
     double g[3][3], A[3][3], b[3], B[3];
@@ -368,6 +371,16 @@ void orc_bssn_point(const double *in, const orc_params *p, double *rhs) {
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) sqrA += A[i][j] * AUU[i][j];
 
+    /* Hamiltonian constraint chi igt^jk R_jk - At_ij At^ij + 2/3 K^2
+     * (CodeGen/constraints.py:250-254; dendro.sqr, dendro.py:309-318) */
+    if (ham) {
+        double s = 0.0;
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) s += chi * igt[j][k] * R[j][k];
+        *ham = s - sqrA + 2.0 / 3.0 * K * K;
+    }
+    if (!rhs) return;
+
     double divb = db[0][0] + db[1][1] + db[2][2];
     const double w = -2.0 / 3.0; /* Lie-derivative weight (bssn.py:39) */
 
@@ -438,6 +451,14 @@ void orc_bssn_point(const double *in, const orc_params *p, double *rhs) {
     }
 }
 
+void orc_bssn_point(const double *in, const orc_params *p, double *rhs) { point_algebra(in, p, rhs, NULL); }
+
+void orc_ham_point(const double *in, double *ham) { point_algebra(in, NULL, NULL, ham); }
+
+void orc_ham_points(const double *in, int64_t npts, double *ham) {
+    for (int64_t q = 0; q < npts; q++) orc_ham_point(in + q * ORC_POINT_NIN, ham + q);
+}
+
 /* The per-point algebra used by orc_bssnrhs_block. Fixture generation swaps in
  * the reference's generated equations (oracle/_ref) through this hook. */
 static orc_point_fn g_point_fn = orc_bssn_point;
@@ -506,6 +527,32 @@ static int is_gauge_var(int v) {
     return v == ORC_ALPHA || (v >= ORC_BETA0 && v <= ORC_B0 + 2);
 }
 
+/* All first derivatives and the second/mixed derivatives of the 11 D2VARS
+ * into `ws` (bssnrhs_derivs.h; the constraints use the same calls,
+ * constraint_derivs.h): 72 + 66 arrays of the block's size. */
+static void block_derivs(const double *const *u, const uint32_t *sz, uint32_t bflag, double hx, double hy,
+                         double hz, double *ws, double *d1[24][3], double *d2[11][6]) {
+    const int64_t n = (int64_t)sz[0] * sz[1] * sz[2];
+    for (int v = 0; v < 24; v++)
+        for (int d = 0; d < 3; d++) d1[v][d] = ws + (size_t)n * (3 * v + d);
+    for (int m = 0; m < 11; m++)
+        for (int c = 0; c < 6; c++) d2[m][c] = ws + (size_t)n * (72 + 6 * m + c);
+    for (int v = 0; v < 24; v++) {
+        orc_deriv644_x(d1[v][0], u[v], hx, sz, bflag);
+        orc_deriv644_y(d1[v][1], u[v], hy, sz, bflag);
+        orc_deriv644_z(d1[v][2], u[v], hz, sz, bflag);
+    }
+    for (int m = 0; m < 11; m++) {
+        const int v = D2VARS[m];
+        orc_deriv644_xx(d2[m][0], u[v], hx, sz, bflag);
+        orc_deriv644_y(d2[m][1], d1[v][0], hy, sz, bflag);  /* grad2_0_1 = D_y(D_x u) */
+        orc_deriv644_z(d2[m][2], d1[v][0], hz, sz, bflag);  /* grad2_0_2 = D_z(D_x u) */
+        orc_deriv644_yy(d2[m][3], u[v], hy, sz, bflag);
+        orc_deriv644_z(d2[m][4], d1[v][1], hz, sz, bflag);  /* grad2_1_2 = D_z(D_y u) */
+        orc_deriv644_zz(d2[m][5], u[v], hz, sz, bflag);
+    }
+}
+
 int orc_bssnrhs_block(double *const *rhs_all, const double *const *u_all, const orc_block *blk,
                       const orc_params *p) {
     const uint32_t *sz = blk->sz;
@@ -530,26 +577,8 @@ int orc_bssnrhs_block(double *const *rhs_all, const double *const *u_all, const 
     double *ws = (double *)malloc(sizeof(double) * (size_t)n * (72 + 66 + 3));
     if (!ws) return -2;
     double *d1[24][3], *d2[11][6];
-    for (int v = 0; v < 24; v++)
-        for (int d = 0; d < 3; d++) d1[v][d] = ws + (size_t)n * (3 * v + d);
-    for (int m = 0; m < 11; m++)
-        for (int c = 0; c < 6; c++) d2[m][c] = ws + (size_t)n * (72 + 6 * m + c);
+    block_derivs(u, sz, bflag, hx, hy, hz, ws, d1, d2);
     double *kox = ws + (size_t)n * 138, *koy = kox + n, *koz = koy + n;
-
-    for (int v = 0; v < 24; v++) {
-        orc_deriv644_x(d1[v][0], u[v], hx, sz, bflag);
-        orc_deriv644_y(d1[v][1], u[v], hy, sz, bflag);
-        orc_deriv644_z(d1[v][2], u[v], hz, sz, bflag);
-    }
-    for (int m = 0; m < 11; m++) {
-        const int v = D2VARS[m];
-        orc_deriv644_xx(d2[m][0], u[v], hx, sz, bflag);
-        orc_deriv644_y(d2[m][1], d1[v][0], hy, sz, bflag);  /* grad2_0_1 = D_y(D_x u) */
-        orc_deriv644_z(d2[m][2], d1[v][0], hz, sz, bflag);  /* grad2_0_2 = D_z(D_x u) */
-        orc_deriv644_yy(d2[m][3], u[v], hy, sz, bflag);
-        orc_deriv644_z(d2[m][4], d1[v][1], hz, sz, bflag);  /* grad2_1_2 = D_z(D_y u) */
-        orc_deriv644_zz(d2[m][5], u[v], hz, sz, bflag);
-    }
 
     /* pointwise equations over the interior (rhs.cpp:194-269) */
     double in[ORC_POINT_NIN], out[24];
@@ -615,6 +644,86 @@ int orc_bssnrhs_batch(double *const *rhs, const double *const *u, const orc_bloc
     for (int64_t b = 0; b < nblocks; b++) err |= (orc_bssnrhs_block(rhs, u, &blks[b], p) != 0);
     (void)nthreads;
     return err ? -1 : 0;
+}
+
+/* ======================================================================== */
+/* Hamiltonian constraint per block: physical_constraints, physcon.cpp:12-89  */
+/* (C_HAM only; interior points, padding of `ham` untouched)                  */
+/* ======================================================================== */
+int orc_ham_block(double *ham_all, const double *const *u_all, const orc_block *blk) {
+    const uint32_t *sz = blk->sz;
+    const int nx = (int)sz[0], ny = (int)sz[1], nz = (int)sz[2];
+    if (nx < 13 || ny < 13 || nz < 13) return -1;
+    const int64_t n = (int64_t)nx * ny * nz;
+    const double hx = (blk->pmax[0] - blk->pmin[0]) / (nx - 1);
+    const double hy = (blk->pmax[1] - blk->pmin[1]) / (ny - 1);
+    const double hz = (blk->pmax[2] - blk->pmin[2]) / (nz - 1);
+    const int PW = 3;
+    const double *u[24];
+    for (int v = 0; v < 24; v++) u[v] = u_all[v] + blk->offset;
+    double *ham = ham_all + blk->offset;
+    double *ws = (double *)malloc(sizeof(double) * (size_t)n * (72 + 66));
+    if (!ws) return -2;
+    double *d1[24][3], *d2[11][6];
+    block_derivs(u, sz, blk->bflag, hx, hy, hz, ws, d1, d2);
+    double in[ORC_POINT_NIN];
+    for (int k = PW; k < nz - PW; k++)
+        for (int j = PW; j < ny - PW; j++)
+            for (int i = PW; i < nx - PW; i++) {
+                const int64_t pp = IDX3(i, j, k);
+                for (int v = 0; v < 24; v++) {
+                    in[v] = u[v][pp];
+                    for (int d = 0; d < 3; d++) in[24 + 3 * v + d] = d1[v][d][pp];
+                }
+                for (int m = 0; m < 11; m++)
+                    for (int c = 0; c < 6; c++) in[96 + 6 * m + c] = d2[m][c][pp];
+                in[162] = 0.0;
+                orc_ham_point(in, &ham[pp]);
+            }
+    free(ws);
+    return 0;
+}
+
+int orc_ham_batch(double *ham, const double *const *u, const orc_block *blks, int64_t nblocks, int nthreads) {
+    int err = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+#endif
+    for (int64_t b = 0; b < nblocks; b++) err |= (orc_ham_block(ham, u, &blks[b]) != 0);
+    (void)nthreads;
+    return err ? -1 : 0;
+}
+
+/* computeConstraintL2Norm (BSSN_GR/include/grUtils.tcc:37-134): sqrt of the
+ * mean of c^2 over the nodes (here: interior points of every block) whose
+ * distance to BH 1 is >= r1 and to BH 2 is >= r2. bh = {x1,y1,z1,x2,y2,z2}.
+ * Returns -1 when no point qualifies. */
+double orc_constraint_l2_norm(const double *c_all, const orc_block *blks, int64_t nblocks, const double *bh,
+                              double r1, double r2) {
+    double l2 = 0.0;
+    int64_t cnt = 0;
+    for (int64_t b = 0; b < nblocks; b++) {
+        const orc_block *blk = &blks[b];
+        const uint32_t *sz = blk->sz;
+        const int nx = (int)sz[0], ny = (int)sz[1], nz = (int)sz[2];
+        const double hx = (blk->pmax[0] - blk->pmin[0]) / (nx - 1);
+        const double hy = (blk->pmax[1] - blk->pmin[1]) / (ny - 1);
+        const double hz = (blk->pmax[2] - blk->pmin[2]) / (nz - 1);
+        const double *c = c_all + blk->offset;
+        for (int k = 3; k < nz - 3; k++)
+            for (int j = 3; j < ny - 3; j++)
+                for (int i = 3; i < nx - 3; i++) {
+                    const double x = blk->pmin[0] + i * hx, y = blk->pmin[1] + j * hy, z = blk->pmin[2] + k * hz;
+                    const double d1 = sqrt((x - bh[0]) * (x - bh[0]) + (y - bh[1]) * (y - bh[1]) + (z - bh[2]) * (z - bh[2]));
+                    const double d2 = sqrt((x - bh[3]) * (x - bh[3]) + (y - bh[4]) * (y - bh[4]) + (z - bh[5]) * (z - bh[5]));
+                    if (d1 < r1 || d2 < r2) continue;
+                    const double v = c[IDX3(i, j, k)];
+                    l2 += v * v;
+                    cnt++;
+                }
+    }
+    return cnt ? sqrt(l2 / (double)cnt) : -1.0;
 }
 
 /* ======================================================================== */

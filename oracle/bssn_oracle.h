@@ -19,6 +19,13 @@
  *   orc_bssnrhs_block                   golden block outputs (reference algebra
  *                                       + these stencils), Minkowski => 0,
  *                                       axis-permutation covariance
+ *   orc_ham_point                       golden vectors from the reference's
+ *                                       generated physconeqs.cpp compiled in
+ *                                       place (oracle/_ref, tests/golden/);
+ *                                       Minkowski => 0; Brill-Lindquist data
+ *                                       (an exact vacuum solution) => 0 up to
+ *                                       truncation, converging at 4th order
+ *   orc_constraint_l2_norm              brute-force mask/mean in the tests
  *   orc_enforce_constraints             det(gt)=1, tr(At)=0, idempotence
  *   orc_rk4_step                        exact on polynomial-in-t RHS (closed
  *                                       form), fixture of one step
@@ -94,6 +101,17 @@ void orc_ko_deriv42_z(double *Du, const double *u, double dz, const uint32_t *sz
 void orc_default_params(orc_params *p);
 void orc_bssn_point(const double *in, const orc_params *p, double *rhs24);
 void orc_bssn_points(const double *in, int64_t npts, const orc_params *p, double *rhs);
+/* Hamiltonian constraint at a point, same input layout (eta unused). */
+void orc_ham_point(const double *in, double *ham);
+void orc_ham_points(const double *in, int64_t npts, double *ham);
+/* C_HAM of physical_constraints (physcon.cpp:12-89) at the interior points of
+ * every block; the padding of `ham` is not written. */
+int orc_ham_block(double *ham, const double *const *u, const orc_block *blk);
+int orc_ham_batch(double *ham, const double *const *u, const orc_block *blks, int64_t nblocks, int nthreads);
+/* computeConstraintL2Norm (grUtils.tcc:37-134) over interior points outside
+ * r1 of BH 1 and r2 of BH 2 (bh = x1,y1,z1,x2,y2,z2); -1 if none qualifies. */
+double orc_constraint_l2_norm(const double *c, const orc_block *blks, int64_t nblocks, const double *bh,
+                              double r1, double r2);
 /* Replace the algebra orc_bssnrhs_block evaluates per point (NULL restores
  * orc_bssn_point). Process-global: fixture generation only, single-threaded. */
 typedef void (*orc_point_fn)(const double *in, const orc_params *p, double *rhs24);

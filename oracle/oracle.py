@@ -79,6 +79,11 @@ def lib():
         L.orc_rk4_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
                                    C.POINTER(Params), C.c_int, C.c_int]
         L.orc_rk4_step.restype = C.c_int
+        L.orc_ham_points.argtypes = [dp, C.c_int64, dp]
+        L.orc_ham_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+        L.orc_ham_batch.restype = C.c_int
+        L.orc_constraint_l2_norm.argtypes = [dp, C.c_void_p, C.c_int64, dp, C.c_double, C.c_double]
+        L.orc_constraint_l2_norm.restype = C.c_double
         L.orc_enforce_constraints_point.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
         L.orc_enforce_constraints_point.restype = C.c_int
         for nm in ("orc_deriv644_x", "orc_deriv644_y", "orc_deriv644_z", "orc_deriv644_xx",
@@ -104,6 +109,7 @@ def ref_lib():
         R = C.CDLL(path)
         dp = C.POINTER(C.c_double)
         R.ref_bssn_points.argtypes = [dp, C.c_int64, dp, dp, dp]
+        R.ref_ham_points.argtypes = [dp, C.c_int64, dp]
         _ref = R
     return _ref
 
@@ -144,6 +150,23 @@ def ref_bssn_points(inp: np.ndarray, lam, lamf) -> np.ndarray:
     la = np.asarray(lam, dtype=np.float64)
     lf = np.asarray(lamf, dtype=np.float64)
     R.ref_bssn_points(_dp(inp), n, _dp(la), _dp(lf), _dp(out))
+    return out
+
+
+def ham_points(inp: np.ndarray) -> np.ndarray:
+    """Hamiltonian constraint per point (input layout of bssn_points)."""
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    out = np.zeros(inp.shape[0])
+    lib().orc_ham_points(_dp(inp), inp.shape[0], _dp(out))
+    return out
+
+
+def ref_ham_points(inp: np.ndarray) -> np.ndarray:
+    R = ref_lib()
+    assert R is not None, "oracle/_ref not built (needs the reference checkout)"
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    out = np.zeros(inp.shape[0])
+    R.ref_ham_points(_dp(inp), inp.shape[0], _dp(out))
     return out
 
 
@@ -222,6 +245,26 @@ def rk4_step(state: np.ndarray, blocks: np.ndarray, dt: float, p: Params,
                              C.byref(p), int(enforce_per_stage), nthreads)
     if err != 0:
         raise RuntimeError("orc_rk4_step failed (constraint violation)")
+
+
+def ham_batch(state: np.ndarray, blocks: np.ndarray, nthreads: int = 0) -> np.ndarray:
+    """C_HAM at the interior points of every block (padding left 0)."""
+    ham = np.zeros(state.shape[1])
+    blocks = np.ascontiguousarray(blocks)
+    err = lib().orc_ham_batch(_dp(ham), _ptrs(state_views(state)), blocks.ctypes.data, len(blocks),
+                              nthreads)
+    if err != 0:
+        raise RuntimeError("orc_ham_batch failed")
+    return ham
+
+
+def constraint_l2_norm(c: np.ndarray, blocks: np.ndarray, bh=((3.0, 0, 0), (-3.0, 0, 0)),
+                       r1: float = 5.0, r2: float = 5.0) -> float:
+    """computeConstraintL2Norm over interior points outside r1/r2 of the two BHs."""
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    blocks = np.ascontiguousarray(blocks)
+    b = np.ascontiguousarray(np.asarray(bh, dtype=np.float64).ravel())
+    return float(lib().orc_constraint_l2_norm(_dp(c), blocks.ctypes.data, len(blocks), _dp(b), r1, r2))
 
 
 def enforce_point(state: np.ndarray, node: int, chi_floor: float) -> int:

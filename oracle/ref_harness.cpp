@@ -79,3 +79,31 @@ extern "C" void ref_bssn_point_orc(const double *in, const void *params, double 
     const double *p = static_cast<const double *>(params);
     ref_bssn_points(in, 1, p, p + 4, out);
 }
+
+// The reference's generated constraint fragment (BSSN_GR/src/physconeqs.cpp,
+// included in place as physcon.cpp:78-86 does) over the same point layout:
+// only ham[pp] is returned; psi4 and the momentum constraints it also writes
+// are discarded. x, y, z (read by the psi4 part) are set to (1, 2, 3).
+extern "C" int ref_ham_points(const double *in, int64_t npts, double *ham_out) {
+    std::vector<double> soa((size_t)NIN * (size_t)npts);
+    std::vector<double> res((size_t)6 * (size_t)npts);
+    for (int64_t q = 0; q < npts; q++)
+        for (int c = 0; c < NIN; c++) soa[(size_t)c * npts + q] = in[(size_t)q * NIN + c];
+    FIELD(alpha, 0) FIELD(chi, 1) FIELD(K, 2) FIELD(Gt0, 3) FIELD(Gt1, 4) FIELD(Gt2, 5)
+    FIELD(beta0, 6) FIELD(beta1, 7) FIELD(beta2, 8) FIELD(B0, 9) FIELD(B1, 10) FIELD(B2, 11)
+    FIELD(gt0, 12) FIELD(gt1, 13) FIELD(gt2, 14) FIELD(gt3, 15) FIELD(gt4, 16) FIELD(gt5, 17)
+    FIELD(At0, 18) FIELD(At1, 19) FIELD(At2, 20) FIELD(At3, 21) FIELD(At4, 22) FIELD(At5, 23)
+    GRAD(alpha, 0) GRAD(chi, 1) GRAD(K, 2) GRAD(Gt0, 3) GRAD(Gt1, 4) GRAD(Gt2, 5)
+    GRAD(beta0, 6) GRAD(beta1, 7) GRAD(beta2, 8) GRAD(B0, 9) GRAD(B1, 10) GRAD(B2, 11)
+    GRAD(gt0, 12) GRAD(gt1, 13) GRAD(gt2, 14) GRAD(gt3, 15) GRAD(gt4, 16) GRAD(gt5, 17)
+    GRAD(At0, 18) GRAD(At1, 19) GRAD(At2, 20) GRAD(At3, 21) GRAD(At4, 22) GRAD(At5, 23)
+    GRAD2(alpha, 0) GRAD2(chi, 1) GRAD2(beta0, 2) GRAD2(beta1, 3) GRAD2(beta2, 4)
+    GRAD2(gt0, 5) GRAD2(gt1, 6) GRAD2(gt2, 7) GRAD2(gt3, 8) GRAD2(gt4, 9) GRAD2(gt5, 10)
+    OUT(ham, 0) OUT(mom0, 1) OUT(mom1, 2) OUT(mom2, 3) OUT(psi4_real, 4) OUT(psi4_img, 5)
+    const double x = 1.0, y = 2.0, z = 3.0;
+    for (int64_t pp = 0; pp < npts; pp++) {
+#include "physconeqs.cpp"
+    }
+    for (int64_t q = 0; q < npts; q++) ham_out[q] = ham[q];
+    return 0;
+}

@@ -17,6 +17,12 @@ Fixtures:
                       after one RK4 step (dt 0.025, constraints enforced per stage)
   uniform13_8.npz     the first 8 blocks of the uniform 13^3 batch, seed 2:
                       interior RHS (KO sigma 0.4, eta 2)
+  ham_points.npz      for the 200 inputs of algebra_points.npz (checksum kept),
+                      the Hamiltonian constraint of the reference's generated
+                      constraint equations (BSSN_GR/src/physconeqs.cpp,
+                      compiled in place)
+
+`python make_golden.py ham` writes ham_points.npz only.
 """
 import ctypes as C
 import os
@@ -79,8 +85,18 @@ def block_params():
     return O.default_params(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
 
 
+def write_ham():
+    inp = algebra_inputs()
+    np.savez_compressed(os.path.join(HERE, "ham_points.npz"), ham=O.ref_ham_points(inp),
+                        inputs_sum=np.array([inp.sum(), np.abs(inp).sum()]))
+
+
 def main():
     assert O.ref_lib() is not None, "build oracle/_ref first (make -C oracle)"
+    if sys.argv[1:] == ["ham"]:
+        write_ham()
+        return
+    write_ham()
     inp = algebra_inputs()
     outs = {k: O.ref_bssn_points(inp, lam, lf) for k, (lam, lf) in PARAM_SETS.items()}
     np.savez_compressed(os.path.join(HERE, "algebra_points.npz"), inputs=inp,

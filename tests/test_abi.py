@@ -77,3 +77,13 @@ def test_generated_equations_match_reference_vectors(pset):
     ref = g[f"out_{pset}"]
     err = np.abs(out - ref).max(0) / np.abs(ref).max(0)
     assert err.max() < 1e-13, err
+
+
+def test_generated_ham_matches_reference_constraint_vectors():
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    ref = np.load(os.path.join(GOLD, "ham_points.npz"))["ham"]
+    inp = np.ascontiguousarray(g["inputs"])
+    out = np.zeros(len(inp))
+    dp = C.POINTER(C.c_double)
+    _host_lib().dgr_host_ham_points(inp.ctypes.data_as(dp), len(inp), out.ctypes.data_as(dp))
+    assert np.abs(out - ref).max() / np.abs(ref).max() < 1e-13

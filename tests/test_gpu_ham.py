@@ -1,0 +1,113 @@
+"""Hamiltonian constraint and its masked norm on the GPU (through the C-ABI)
+against the oracle: per point within 1e-12 normwise relative, the norm within
+1e-12 relative (the reference sums in another order)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import dgr
+from dgr import workloads as W
+from gpu_util import host_state, log, padding_mask, to_dev
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    dgr.load()
+    yield
+
+
+def gpu_ham(t, st):
+    import torch
+    plan = dgr.Plan(t, st.shape[1])
+    ham = torch.full((st.shape[1],), float("nan"), dtype=torch.float64, device="cuda")
+    plan.ham(ham, to_dev(st))
+    torch.cuda.synchronize()
+    plan.close()
+    return ham.cpu().numpy()
+
+
+def uneven_sample():
+    full = W.puncture_octree(0.42)
+    pick = []
+    for n in (55, 31, 19, 13):
+        idx = np.nonzero(full["sz"][:, 0] == n)[0]
+        pick += list(idx[:2])
+    t = full[np.array(sorted(pick))].copy()
+    off = 0
+    for b in range(len(t)):
+        t["offset"][b] = off
+        off += int(np.prod(t["sz"][b].astype(np.int64)))
+    return t
+
+
+@pytest.mark.parametrize("bflag", [0, 63, 21])
+def test_ham_block19_matches_oracle(bflag):
+    log(f"ham block19 bflag={bflag}")
+    t = W.single_block_19(bflag)
+    st = host_state(t, seed=1, amp=1e-2)
+    got = gpu_ham(t, st)
+    ref = O.ham_batch(st, t, nthreads=1)
+    m = ~padding_mask(t, st.shape[1])
+    assert np.abs(got[m] - ref[m]).max() <= TOL * np.abs(ref[m]).max()
+    assert np.isnan(got[~m]).all(), "padding of the output was written"
+
+
+def test_ham_uneven_puncture_blocks_match_oracle():
+    """Brill-Lindquist data satisfy the constraint: ham is a truncation
+    residual (here ~1e-4) of terms of order one (chi, its derivatives, the
+    Ricci scalar) that cancel, so agreement is measured against that term
+    scale, max(|ham|, 1), rather than against the residual itself."""
+    log("ham uneven puncture blocks")
+    t = uneven_sample()
+    t["bflag"][0] = 63
+    st = host_state(t, init="puncture", seed=5)
+    got = gpu_ham(t, st)
+    ref = O.ham_batch(st, t, nthreads=8)
+    m = ~padding_mask(t, st.shape[1])
+    assert np.abs(got[m] - ref[m]).max() <= TOL * max(np.abs(ref[m]).max(), 1.0)
+
+
+def test_constraint_norm_matches_oracle():
+    import torch
+    log("constraint norm")
+    t = uneven_sample()
+    st = host_state(t, init="puncture", seed=5)
+    ref_ham = O.ham_batch(st, t, nthreads=8)
+    plan = dgr.Plan(t, st.shape[1])
+    c = torch.from_numpy(ref_ham).cuda()
+    for r in (0.0, 2.0, 5.0):
+        got = plan.constraint_norm(c, r1=r, r2=r)
+        want = O.constraint_l2_norm(ref_ham, t, r1=r, r2=r)
+        assert got == pytest.approx(want, rel=1e-12, abs=0), r
+    assert plan.constraint_norm(c, r1=1e9, r2=1e9) == -1.0
+    plan.close()
+
+
+def test_ham_norm_unchanged_over_an_rk4_step():
+    """The constraint norm of the state after one GPU RK4 step equals the
+    oracle's (same step on the CPU) to far better than 3 digits."""
+    import torch
+    log("ham norm over rk4")
+    t = uneven_sample()
+    st = host_state(t, init="puncture", seed=5)
+    p = dgr.default_params(ko_sigma=0.4)
+    plan = dgr.Plan(t, st.shape[1])
+    u = to_dev(st)
+    hmin = float(np.min((t["pmax"][:, 0] - t["pmin"][:, 0]) / (t["sz"][:, 0] - 1)))
+    dt = 0.25 * hmin
+    plan.rk4_step(u, dt, p, enforce_per_stage=True)
+    ham = torch.zeros(st.shape[1], dtype=torch.float64, device="cuda")
+    plan.ham(ham, u)
+    got = plan.constraint_norm(ham, r1=1.0, r2=1.0)
+    assert plan.status() == 0
+    ref = st.copy()
+    O.rk4_step(ref, t, dt, O.default_params(ko_sigma=0.4), enforce_per_stage=True, nthreads=8)
+    want = O.constraint_l2_norm(O.ham_batch(ref, t, nthreads=8), t, r1=1.0, r2=1.0)
+    assert want > 0.0
+    assert got == pytest.approx(want, rel=1e-9)
+    plan.close()
