@@ -1,23 +1,30 @@
 #!/usr/bin/env python
 """Benchmark of the BSSN right-hand side on one MI355X.
 
-Workload (default): 4096 uniform 13^3 padded blocks (7^3 interior each,
-1,404,928 interior points), seeded perturbed-Minkowski data, one full RK4 step
-per timed step (4 right-hand side evaluations: derivatives, equations, KO
-dissipation, stage update, constraint enforcement per stage), FP64, inputs
-resident in HBM. Reports interior Mpoints/s = interior points x 4 / step time.
+Workloads (interior points are the non-padding points of all blocks):
+  uniform13 (default)  4096 uniform 13^3 padded blocks (7^3 interior each,
+                       1,404,928 points), seeded perturbed-Minkowski data,
+                       KO sigma 0.4, eta 2, dt 0.025
+  puncture             the uneven binary-puncture octree: 13^3/19^3/31^3/55^3
+                       blocks with counts 60000/8000/1000/150 x --scale
+                       (0.42 -> ~30M points), Brill-Lindquist data plus 1e-6
+                       noise, KO sigma 0.4, RIT eta, dt 0.25 dx_min
+One timed step = one full RK4 step (4 right-hand side evaluations: derivatives,
+equations, KO dissipation, stage update, constraint enforcement per stage),
+FP64, inputs resident in HBM. value = interior points x 4 / step time.
 
 Prints one JSON line. Also reports:
   roofline      the dominant kernel (the fused kernel on the regular tiles of
                 blocks without flagged faces) against HBM: algorithmic bytes
-                per launch (192 n^3/(n-6)^3 + 192 + 768 bytes per interior point
-                of its tiles: padded state read once, RHS produced once, RK
-                stage update) over its average launch time, measured with HIP
-                events on the launch stream; `traffic` is the HBM bytes per
-                launch measured by rocprofv3 counters (profiles/), when a
-                summary for this workload is committed
+                per launch (per block 192 n^3 for reading the padded state once,
+                per interior point 192 for the RHS and 768 for the RK stage
+                update, split over the launches by the points each covers) over
+                its average launch time, measured with HIP events on the launch
+                stream; `traffic` is the HBM bytes per launch measured with
+                rocprofv3 counters (profiles/), when a summary for this
+                workload is committed
   cpu_baseline  the oracle's OpenMP C restatement on the host cores, one RK4
-                step of a 1024-block sample of the same batch
+                step of a bounded sample of the same workload
   rhs_only      right-hand side alone (no RK update), Mpoints/s
 """
 import argparse
@@ -28,63 +35,140 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dendro-gr_amd"))
 
 METRIC = "BSSN RHS Mpoints/s on 1 MI355X: plain FP64 vs batched mixed at ≤1e-8 rel err"
 HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {"fused_wide": "k_fused<interior, 256 threads, 7x7x5 tiles>",
+                "fused_thin": "k_fused<interior, 128 threads, 7x7x2 tiles>",
+                "fused_face": "k_fused<flagged faces, 256 threads>",
+                "derivs": "k_derivs", "rhs": "k_rhs", "face": "k_face"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def stage_bytes_per_point(n):
-    """Algorithmic HBM bytes per interior point per RK stage: padded FP64 state
-    read once, RHS written once, plus the stage update (read u0, read and write
-    the accumulator, write the next stage)."""
-    return 192.0 * n ** 3 / (n - 6) ** 3 + 192.0 + 768.0
+class Workload:
+    """Block table, synthetic data and parameters of one benchmark workload."""
+
+    def __init__(self, name, nblocks, scale):
+        from dgr import workloads as W
+        self.name = name
+        if name == "uniform13":
+            self.blocks = W.uniform_13(nblocks)
+            self.key = f"uniform13x{nblocks}"
+            self.desc = f"uniform_13^3 x {nblocks} blocks, one RK4 step per step"
+            self.dt = 0.25 * 0.1
+            self.params = dict(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
+        elif name == "puncture":
+            self.blocks = W.puncture_octree(scale)
+            self.key = f"puncture{scale:g}"
+            sizes = {int(n): int((self.blocks["sz"][:, 0] == n).sum()) for n in (13, 19, 31, 55)}
+            self.desc = (f"binary-puncture octree (Brill-Lindquist), blocks 13^3/19^3/31^3/55^3 = "
+                         f"{sizes[13]}/{sizes[19]}/{sizes[31]}/{sizes[55]}, one RK4 step per step")
+            h = (self.blocks["pmax"][:, 0] - self.blocks["pmin"][:, 0]) / (self.blocks["sz"][:, 0] - 1)
+            self.dt = 0.25 * float(h.min())
+            self.params = dict(ko_sigma=0.4)
+        else:
+            raise ValueError(name)
+        self.npts = W.interior_points(self.blocks)
+        self.n_pad = W.total_padded(self.blocks)
+
+    def device_state(self, dev):
+        from dgr import synth
+        if self.name == "uniform13":
+            return synth.perturbed_minkowski(self.blocks, seed=1, amp=1e-3, device=dev)
+        return synth.brill_lindquist(self.blocks, seed=1, noise=1e-6, device=dev)
+
+    def cpu_sample(self, nblocks_uniform):
+        """A bounded sample of the same workload for the CPU baseline."""
+        from dgr import workloads as W
+        if self.name == "uniform13":
+            return (W.uniform_13(nblocks_uniform),
+                    f"the first {nblocks_uniform} blocks of the same uniform 13^3 batch")
+        # every k-th block of the octree, keeping the size mix (and enough
+        # large blocks that OpenMP over blocks is not bound by one of them)
+        k = max(1, len(self.blocks) // 1400)
+        sub = self.blocks[::k].copy()
+        off = 0
+        for b in range(len(sub)):
+            sub["offset"][b] = off
+            off += int(np.prod(sub["sz"][b].astype(np.int64)))
+        return sub, f"every {k}th block of the same octree ({len(sub)} blocks)"
 
 
-def measured_traffic(kind, nblocks):
+def measured_traffic(kind, key):
     """HBM bytes per launch of `kind` from the committed rocprofv3 counter
-    summary of this workload (profiles/*traffic*.txt lines
-    'traffic <kind> uniform13x<nblocks> <bytes>'), else None."""
+    summaries (profiles/*traffic*.txt lines 'traffic <kind> <workload> <bytes>'),
+    else None."""
     import glob
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.txt"))):
         with open(f) as fh:
             for line in fh:
                 p = line.split()
-                if len(p) == 4 and p[0] == "traffic" and p[1] == kind and p[2] == f"uniform13x{nblocks}":
+                if len(p) == 4 and p[0] == "traffic" and p[1] == kind and p[2] == key:
                     return float(p[3])
     return None
 
 
-def cpu_baseline(nblocks, threads, reps=3):
+def cpu_baseline(wl, nblocks_uniform, threads, reps=3):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import oracle as O
     from dgr import workloads as W
 
-    t = W.uniform_13(nblocks)
+    t, what = wl.cpu_sample(nblocks_uniform)
     st = np.zeros((24, W.total_padded(t)))
-    O.init_perturbed_minkowski(st, t, seed=1, amp=1e-3)
-    p = O.default_params(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
+    if wl.name == "uniform13":
+        O.init_perturbed_minkowski(st, t, seed=1, amp=1e-3)
+    else:
+        O.init_brill_lindquist(st, t, seed=1, noise=1e-6)
+    p = O.default_params(**wl.params)
     pts = W.interior_points(t)
     times = []
     for r in range(reps + 1):
         s = st.copy()
         t0 = time.perf_counter()
-        O.rk4_step(s, t, 0.025, p, enforce_per_stage=True, nthreads=threads)
+        O.rk4_step(s, t, wl.dt, p, enforce_per_stage=True, nthreads=threads)
         dt = time.perf_counter() - t0
         if r > 0:
             times.append(dt)
     med = statistics.median(times)
     return {"value": round(pts * 4 / med / 1e6, 3), "unit": "Mpoints/s", "cores": threads,
             "kind": "port",
-            "sample": f"one RK4 step (4 RHS evaluations) of the first {nblocks} blocks of the "
-                      f"same uniform 13^3 batch ({pts} interior points), OpenMP over blocks, "
-                      f"median of {reps} after 1 warm-up"}
+            "sample": f"one RK4 step (4 RHS evaluations) of {what} ({pts} interior points), "
+                      f"OpenMP over blocks, median of {reps} after 1 warm-up"}
+
+
+def roofline(wl, kt, stages):
+    from dgr import workloads as W
+    kind_pts = W.fused_kind_points(wl.blocks)
+    kind_bytes = W.fused_kind_bytes(wl.blocks)
+    stage_ms = sum(v[0] for v in kt.values()) / stages
+    per_kernel = {}
+    for k, (ms, nl) in kt.items():
+        if not nl:
+            continue
+        entry = {"ms_per_launch": round(ms / nl, 5), "launches_per_stage": nl / stages}
+        if kind_pts.get(k):
+            entry["points"] = kind_pts[k]
+            entry["achieved_GBs"] = round(kind_bytes[k] / (ms / nl * 1e-3) / 1e9, 1)
+        per_kernel[k] = entry
+    dom = max(per_kernel, key=lambda k: kt[k][0])
+    dom_ms = kt[dom][0] / kt[dom][1]
+    dom_bytes = kind_bytes.get(dom) or W.stage_bytes(wl.blocks)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, wl.key),
+            "kernel": KERNEL_NAMES.get(dom, dom), "launch_ms": round(dom_ms, 5),
+            "algorithmic_bytes_per_launch": dom_bytes,
+            "algorithmic_bytes_per_point": round(W.stage_bytes(wl.blocks) / wl.npts, 1),
+            "stage_ms": round(stage_ms, 4),
+            "stage_GBs": round(W.stage_bytes(wl.blocks) / (stage_ms * 1e-3) / 1e9, 1),
+            "per_kernel": per_kernel}
 
 
 def main():
@@ -92,8 +176,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nblocks", type=int, default=4096)
-    ap.add_argument("--cpu-blocks", type=int, default=1024)
+    ap.add_argument("--workload", choices=["uniform13", "puncture"], default="uniform13")
+    ap.add_argument("--nblocks", type=int, default=4096, help="uniform13: number of blocks")
+    ap.add_argument("--scale", type=float, default=0.42, help="puncture: block-count scale")
+    ap.add_argument("--cpu-blocks", type=int, default=1024, help="uniform13: CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (3 times), to scale "
@@ -116,22 +202,18 @@ def main():
 
     import torch
     import dgr
-    from dgr import synth, workloads as W
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    blocks = W.uniform_13(args.nblocks)
-    npts = W.interior_points(blocks)
-    n_pad = W.total_padded(blocks)
-    log(f"workload: {args.nblocks} x 13^3 blocks, {npts} interior points, state {24 * n_pad * 8 / 1e9:.2f} GB")
-    state = synth.perturbed_minkowski(blocks, seed=1, amp=1e-3, device=dev)
-    plan = dgr.Plan(blocks, n_pad)
-    params = dgr.default_params(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
-    dt = 0.25 * 0.1
+    wl = Workload(args.workload, args.nblocks, args.scale)
+    log(f"workload: {wl.desc}; {wl.npts} interior points, state {24 * wl.n_pad * 8 / 1e9:.2f} GB")
+    state = wl.device_state(dev)
+    plan = dgr.Plan(wl.blocks, wl.n_pad)
+    params = dgr.default_params(**wl.params)
     stream = torch.cuda.current_stream()
 
     for _ in range(args.warmup):
-        plan.rk4_step(state, dt, params, enforce_per_stage=True, stream=stream)
+        plan.rk4_step(state, wl.dt, params, enforce_per_stage=True, stream=stream)
     torch.cuda.synchronize()
     if plan.status(stream) != 0:
         raise RuntimeError("constraint violation during warm-up")
@@ -141,7 +223,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        plan.rk4_step(state, dt, params, enforce_per_stage=True, stream=stream)
+        plan.rk4_step(state, wl.dt, params, enforce_per_stage=True, stream=stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     kt = plan.kernel_times()
@@ -150,7 +232,7 @@ def main():
         raise RuntimeError("constraint violation during the timed steps")
     elapsed = t1 - t0
     ms_step = elapsed / args.steps * 1e3
-    value = npts * 4 * args.steps / elapsed / 1e6
+    value = wl.npts * 4 * args.steps / elapsed / 1e6
 
     # right-hand side alone
     out = torch.empty_like(state)
@@ -162,46 +244,16 @@ def main():
     for _ in range(nrhs):
         plan.rhs(out, state, params, stream=stream)
     torch.cuda.synchronize()
-    rhs_only = npts * nrhs / (time.perf_counter() - r0) / 1e6
-
-    stages = 4 * args.steps
-    stage_ms = sum(v[0] for v in kt.values()) / stages
-    bpp = stage_bytes_per_point(13)
-    kind_pts = W.fused_kind_points(blocks)
-    per_kernel = {}
-    for k, (ms, nl) in kt.items():
-        if not nl:
-            continue
-        entry = {"ms_per_launch": round(ms / nl, 5), "launches_per_stage": nl / stages}
-        if k in kind_pts and kind_pts[k]:
-            entry["points"] = kind_pts[k]
-            entry["achieved_GBs"] = round(bpp * kind_pts[k] / (ms / nl * 1e-3) / 1e9, 1)
-        per_kernel[k] = entry
-    dom = max(per_kernel, key=lambda k: kt[k][0])
-    dom_ms = kt[dom][0] / kt[dom][1]
-    dom_bytes = bpp * kind_pts.get(dom, npts)
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    traffic = measured_traffic(dom, args.nblocks)
-    names = {"fused_wide": "k_fused<interior, 256 threads, 7x7x5 tiles>",
-             "fused_thin": "k_fused<interior, 128 threads, 7x7x2 tiles>",
-             "fused_face": "k_fused<flagged faces, 256 threads>",
-             "derivs": "k_derivs", "rhs": "k_rhs", "face": "k_face"}
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": names.get(dom, dom), "launch_ms": round(dom_ms, 5),
-                "algorithmic_bytes_per_launch": dom_bytes,
-                "algorithmic_bytes_per_point": round(bpp, 1),
-                "stage_ms": round(stage_ms, 4),
-                "stage_GBs": round(bpp * npts / (stage_ms * 1e-3) / 1e9, 1),
-                "per_kernel": per_kernel}
+    rhs_only = wl.npts * nrhs / (time.perf_counter() - r0) / 1e6
+    roof = roofline(wl, kt, 4 * args.steps)
     del out
     plan.close()
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: {args.cpu_blocks} blocks on {threads} threads")
-        cpu = cpu_baseline(args.cpu_blocks, threads)
+        log(f"cpu baseline on {threads} threads")
+        cpu = cpu_baseline(wl, args.cpu_blocks, threads)
         try:
             cpu["cpu_model"] = subprocess.run(["bash", "-c", "lscpu | grep 'Model name' | head -1"],
                                               capture_output=True, text=True).stdout.split(":")[-1].strip()
@@ -212,11 +264,10 @@ def main():
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic",
-            "config": {"workload": f"uniform_13^3 x {args.nblocks} blocks, one RK4 step per step",
-                       "blocks": args.nblocks, "padded_edge": 13, "interior_points": npts,
-                       "precision": "fp64", "rhs_evals_per_step": 4, "ko_sigma": 0.4, "dt": dt},
+            "config": {"workload": wl.desc, "blocks": len(wl.blocks), "interior_points": wl.npts,
+                       "precision": "fp64", "rhs_evals_per_step": 4, "dt": wl.dt, **wl.params},
             "rhs_only_mpoints_s": round(rhs_only, 2),
-            "roofline": roofline, "cpu_baseline": cpu}
+            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 

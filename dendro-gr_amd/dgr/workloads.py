@@ -95,6 +95,29 @@ def puncture_octree(scale: float = 0.42, align: int = 1) -> np.ndarray:
     return _table(sizes, pmins, hs, align=align)
 
 
+def stage_bytes(t: np.ndarray) -> float:
+    """Algorithmic HBM bytes of one fused RK stage over all blocks: each padded
+    FP64 state read once (192 n^3 per block), per interior point the RHS
+    produced once (192) and the stage update (read u0, read and write the
+    accumulator, write the next stage: 768)."""
+    n3 = np.prod(t["sz"].astype(np.float64), axis=1)
+    m3 = np.prod(t["sz"].astype(np.float64) - 2 * PW, axis=1)
+    return float((192.0 * n3 + 960.0 * m3).sum())
+
+
+def fused_kind_bytes(t: np.ndarray) -> dict:
+    """stage_bytes split over the fused launches in proportion to the interior
+    points each launch covers within every block."""
+    out = {"fused_wide": 0.0, "fused_thin": 0.0, "fused_face": 0.0}
+    for b in range(len(t)):
+        share = fused_kind_points(t[b:b + 1])
+        m = sum(share.values())
+        bb = stage_bytes(t[b:b + 1])
+        for k, v in share.items():
+            out[k] += bb * v / m
+    return out
+
+
 def fused_kind_points(t: np.ndarray) -> dict:
     """Interior points each launch of the fused path covers, by tile kind, with
     the plan's tiling rule: blocks with flagged faces -> 'fused_face'; other

@@ -527,6 +527,21 @@ static int is_gauge_var(int v) {
     return v == ORC_ALPHA || (v >= ORC_BETA0 && v <= ORC_B0 + 2);
 }
 
+/* Per-thread scratch reused across blocks: slot 0 the derivative workspace,
+ * slot 1 the RK4 block buffers. The reference allocates its workspace once
+ * (grUtils.cpp:2374-2396); a fresh allocation per block would make the large
+ * blocks page-fault bound. Held until the thread ends. */
+static _Thread_local double *g_scratch[2];
+static _Thread_local size_t g_scratch_n[2];
+static double *scratch(int slot, size_t n) {
+    if (g_scratch_n[slot] < n) {
+        free(g_scratch[slot]);
+        g_scratch[slot] = (double *)malloc(sizeof(double) * n);
+        g_scratch_n[slot] = g_scratch[slot] ? n : 0;
+    }
+    return g_scratch[slot];
+}
+
 /* All first derivatives and the second/mixed derivatives of the 11 D2VARS
  * into `ws` (bssnrhs_derivs.h; the constraints use the same calls,
  * constraint_derivs.h): 72 + 66 arrays of the block's size. */
@@ -574,7 +589,7 @@ int orc_bssnrhs_block(double *const *rhs_all, const double *const *u_all, const 
     }
 
     /* derivative workspace: 72 first + 66 second (rhs.cpp:171-180) */
-    double *ws = (double *)malloc(sizeof(double) * (size_t)n * (72 + 66 + 3));
+    double *ws = scratch(0, (size_t)n * (72 + 66 + 3));
     if (!ws) return -2;
     double *d1[24][3], *d2[11][6];
     block_derivs(u, sz, bflag, hx, hy, hz, ws, d1, d2);
@@ -630,7 +645,6 @@ int orc_bssnrhs_block(double *const *rhs_all, const double *const *u_all, const 
                     rhs[v][pp] += sigma * (kox[pp] + koy[pp] + koz[pp]);
                 }
     }
-    free(ws);
     return 0;
 }
 
@@ -662,7 +676,7 @@ int orc_ham_block(double *ham_all, const double *const *u_all, const orc_block *
     const double *u[24];
     for (int v = 0; v < 24; v++) u[v] = u_all[v] + blk->offset;
     double *ham = ham_all + blk->offset;
-    double *ws = (double *)malloc(sizeof(double) * (size_t)n * (72 + 66));
+    double *ws = scratch(0, (size_t)n * (72 + 66 + 3));
     if (!ws) return -2;
     double *d1[24][3], *d2[11][6];
     block_derivs(u, sz, blk->bflag, hx, hy, hz, ws, d1, d2);
@@ -680,7 +694,6 @@ int orc_ham_block(double *ham_all, const double *const *u_all, const orc_block *
                 in[162] = 0.0;
                 orc_ham_point(in, &ham[pp]);
             }
-    free(ws);
     return 0;
 }
 
@@ -801,7 +814,7 @@ static int rk4_block(double *const *u_all, const orc_block *blk, double dt, cons
     const int nx = (int)blk->sz[0], ny = (int)blk->sz[1], nz = (int)blk->sz[2];
     const int64_t n = (int64_t)nx * ny * nz;
     const int PW = 3;
-    double *buf = (double *)malloc(sizeof(double) * (size_t)n * 24 * 3);
+    double *buf = scratch(1, (size_t)n * 24 * 3);
     if (!buf) return -2;
     double *u0[24], *us[24], *k[24], *ucur[24];
     for (int v = 0; v < 24; v++) {
@@ -841,7 +854,6 @@ static int rk4_block(double *const *u_all, const orc_block *blk, double dt, cons
                     if (e) err = e;
                 }
     }
-    free(buf);
     return err;
 }
 
