@@ -1056,6 +1056,15 @@ struct dgr_plan {
     int fgroup[NFGROUP + 1] = {};
     double *d_istage = nullptr; // k_fused RK4: 3 * 24 * interior (two stage states, accumulator)
     int *d_err = nullptr;
+    // dgr_rk4_steps: one RK4 step captured as a graph, valid for the arguments in rk4_key
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t rk4_exec = nullptr;
+    struct {
+        double *u[NV];
+        double dt;
+        dgr_bssn_params prm;
+        int enforce;
+    } rk4_key{};
     int nwg_total = 0;                    // k_rhs-style workgroups over all chunks
     double *d_norm = nullptr;             // norm partials: nwg_total sums, nwg_total counts, 2 results
     // optional per-kernel timing: event pairs recorded around each launch
@@ -1313,6 +1322,8 @@ int dgr_plan_destroy(dgr_plan *plan) {
     hipFree(plan->d_istage);
     hipFree(plan->d_err);
     hipFree(plan->d_norm);
+    if (plan->rk4_exec) hipGraphExecDestroy(plan->rk4_exec);
+    if (plan->cap_stream) hipStreamDestroy(plan->cap_stream);
     delete plan;
     return DGR_OK;
 }
@@ -1459,6 +1470,58 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
     return run_chunks<MODE_RHS>(plan, u, r, rk, to_params(params), (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// RK4 table: include/rkBSSN.h:50-53
+const double RKC[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+const double RKU[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
+
+int ensure_istage(dgr_plan *plan) {
+    const size_t ni = (size_t)plan->interior;
+    if (!plan->d_istage && ni) {
+        hipError_t e = hipMalloc(&plan->d_istage, sizeof(double) * 3 * NV * ni);
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "stage buffers");
+    }
+    return DGR_OK;
+}
+
+// The four stage launches of one RK4 step on the fused path (rkBSSN.cpp:764-860):
+// stage s reads u (s = 0) or the previous stage buffer, writes the other one;
+// the last stage writes the result into u. Needs ensure_istage() first.
+int rk4_fused_launches(dgr_plan *plan, const VarPtrs &u, double dt, const RhsParams &prm, int enforce_per_stage,
+                       hipStream_t st) {
+    const size_t ni = (size_t)plan->interior;
+    VarPtrs sa, sb, acc;
+    for (int v = 0; v < NV; v++) {
+        sa.p[v] = plan->d_istage + (size_t)v * ni;
+        sb.p[v] = plan->d_istage + (size_t)(NV + v) * ni;
+        acc.p[v] = plan->d_istage + (size_t)(2 * NV + v) * ni;
+    }
+    const VarPtrs *ins[4] = {&sa, &sa, &sb, &sa};
+    const VarPtrs *outs[4] = {&sa, &sb, &sa, &sb};
+    for (int s = 0; s < 4; s++) {
+        StageIO io{};
+        io.u = u;
+        io.sin = *ins[s];
+        io.sout = *outs[s];
+        io.acc = acc;
+        io.cacc = RKC[s] * dt;
+        io.cnext = s < 3 ? RKU[s + 1] * dt : 0.0;
+        io.stage = s;
+        io.enforce_stage = enforce_per_stage ? 1 : 0;
+        io.err = plan->d_err;
+        int rc = run_fused(plan, io, prm, st);
+        if (rc) return rc;
+    }
+    return DGR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
                  int enforce_per_stage, void *stream) {
     g_err.clear();
@@ -1468,40 +1531,9 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
     if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
     HIP_OK(hipSetDevice(plan->device));
     hipStream_t st = (hipStream_t)stream;
-    // RK4 table: include/rkBSSN.h:50-53
-    static const double RKC[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-    static const double RKU[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
     if (plan->fused) {
-        const size_t ni = (size_t)plan->interior;
-        if (!plan->d_istage && ni) {
-            hipError_t e = hipMalloc(&plan->d_istage, sizeof(double) * 3 * NV * ni);
-            if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "stage buffers");
-        }
-        VarPtrs sa, sb, acc;
-        for (int v = 0; v < NV; v++) {
-            sa.p[v] = plan->d_istage + (size_t)v * ni;
-            sb.p[v] = plan->d_istage + (size_t)(NV + v) * ni;
-            acc.p[v] = plan->d_istage + (size_t)(2 * NV + v) * ni;
-        }
-        const RhsParams prm = to_params(params);
-        // stage s reads u (s = 0) or the previous stage buffer, writes the other one;
-        // the last stage writes the result into u (rkBSSN.cpp:764-860)
-        const VarPtrs *ins[4] = {&sa, &sa, &sb, &sa};
-        const VarPtrs *outs[4] = {&sa, &sb, &sa, &sb};
-        for (int s = 0; s < 4; s++) {
-            StageIO io{};
-            io.u = u;
-            io.sin = *ins[s];
-            io.sout = *outs[s];
-            io.acc = acc;
-            io.cacc = RKC[s] * dt;
-            io.cnext = s < 3 ? RKU[s + 1] * dt : 0.0;
-            io.stage = s;
-            io.enforce_stage = enforce_per_stage ? 1 : 0;
-            io.err = plan->d_err;
-            if ((rc = run_fused(plan, io, prm, st))) return rc;
-        }
-        return DGR_OK;
+        if ((rc = ensure_istage(plan))) return rc;
+        return rk4_fused_launches(plan, u, dt, to_params(params), enforce_per_stage, st);
     }
     const size_t n = plan->szpdof;
     if (!plan->d_stage && n) {
@@ -1527,6 +1559,57 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
         rk.err = plan->d_err;
         if ((rc = run_chunks<MODE_RK>(plan, stage, stage, rk, prm, st))) return rc;
     }
+    return DGR_OK;
+}
+
+int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
+                  int enforce_per_stage, int nsteps, void *stream) {
+    g_err.clear();
+    if (!plan || !params) return fail(DGR_E_INVALID, "null plan or params");
+    if (nsteps < 0) return fail(DGR_E_INVALID, "nsteps must be >= 0");
+    VarPtrs u;
+    int rc;
+    if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
+    // per-launch timing and the split path run the steps eagerly
+    if (!plan->fused || plan->timing) {
+        for (int i = 0; i < nsteps; i++)
+            if ((rc = dgr_rk4_step(plan, d_u, dt, params, enforce_per_stage, stream))) return rc;
+        return DGR_OK;
+    }
+    HIP_OK(hipSetDevice(plan->device));
+    if ((rc = ensure_istage(plan))) return rc;
+    auto &key = plan->rk4_key;
+    const int enf = enforce_per_stage ? 1 : 0;
+    bool same = plan->rk4_exec && key.dt == dt && key.enforce == enf &&
+                std::memcmp(&key.prm, params, sizeof(*params)) == 0;
+    for (int v = 0; same && v < NV; v++) same = key.u[v] == u.p[v];
+    if (!same) {
+        if (plan->rk4_exec) {
+            HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
+            plan->rk4_exec = nullptr;
+        }
+        if (!plan->cap_stream) HIP_OK(hipStreamCreateWithFlags(&plan->cap_stream, hipStreamNonBlocking));
+        HIP_OK(hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal));
+        rc = rk4_fused_launches(plan, u, dt, to_params(params), enf, plan->cap_stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(plan->cap_stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(DGR_E_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&plan->rk4_exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            plan->rk4_exec = nullptr;
+            return fail(DGR_E_HIP, std::string("graph instantiation: ") + hipGetErrorString(ei));
+        }
+        for (int v = 0; v < NV; v++) key.u[v] = u.p[v];
+        key.dt = dt;
+        key.prm = *params;
+        key.enforce = enf;
+    }
+    for (int i = 0; i < nsteps; i++) HIP_OK(hipGraphLaunch(plan->rk4_exec, (hipStream_t)stream));
     return DGR_OK;
 }
 

@@ -21,7 +21,7 @@ SYMBOLS = [
     "dgr_default_params", "dgr_last_error", "dgr_plan_create", "dgr_plan_destroy",
     "dgr_plan_interior_points", "dgr_bssn_rhs", "dgr_rk4_step", "dgr_enforce_constraints",
     "dgr_plan_status", "dgr_plan_set_timing", "dgr_plan_kernel_times",
-    "dgr_ham_constraint", "dgr_constraint_l2_norm",
+    "dgr_ham_constraint", "dgr_constraint_l2_norm", "dgr_rk4_steps",
 ]
 
 
@@ -71,6 +71,7 @@ def load():
         L.dgr_plan_status.argtypes = [vp, vp]
         L.dgr_plan_set_timing.argtypes = [vp, C.c_int]
         L.dgr_plan_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.dgr_rk4_steps.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), C.c_int, C.c_int, vp]
         L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
         L.dgr_constraint_l2_norm.argtypes = [vp, vp, C.POINTER(C.c_double), C.c_double, C.c_double,
                                              C.POINTER(C.c_double), vp]
@@ -138,6 +139,12 @@ class Plan:
                  stream=None):
         _check(load().dgr_rk4_step(self._h, var_pointers(state), dt, C.byref(params),
                                    int(enforce_per_stage), _stream_ptr(stream)), "dgr_rk4_step")
+
+    def rk4_steps(self, state, dt: float, params: BssnParams, nsteps: int, enforce_per_stage: bool = True,
+                  stream=None):
+        """nsteps RK4 steps replayed from a captured HIP graph."""
+        _check(load().dgr_rk4_steps(self._h, var_pointers(state), dt, C.byref(params), int(enforce_per_stage),
+                                    int(nsteps), _stream_ptr(stream)), "dgr_rk4_steps")
 
     def enforce(self, state, chi_floor: float, stream=None):
         _check(load().dgr_enforce_constraints(self._h, var_pointers(state), chi_floor,

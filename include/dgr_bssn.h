@@ -131,6 +131,15 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
 int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
                  int enforce_per_stage, void *stream);
 
+/* `nsteps` RK4 steps as dgr_rk4_step does them, with the launches of one step
+ * captured once into a HIP graph and the graph replayed on `stream` (the
+ * reference's solver loop drives the stages from the host each time,
+ * rkBSSN.cpp:764-860). The graph is kept in the plan and captured again when
+ * d_u, dt, params or enforce_per_stage differ from the previous call. With
+ * kernel timing enabled, or on the split path, the steps run without a graph. */
+int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
+                  int enforce_per_stage, int nsteps, void *stream);
+
 /* enforce_bssn_constraints (BSSN_GR/include/bssn_constraints.h:15-172) on the
  * interior points of every block: det(gt) = 1, tr(At) = 0, chi and alpha
  * floored at chi_floor. */
