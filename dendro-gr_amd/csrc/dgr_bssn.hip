@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "dgr_bssn.h"
@@ -512,6 +513,18 @@ struct FCfg {
 };
 using CfgWide = FCfg<256, 5, 3>;
 using CfgThin = FCfg<128, 2, 2>;
+// The FP32 kernels fit two waves per SIMD in 256 registers only with one
+// variable in flight.
+template <typename T>
+struct CfgOf {
+    using Wide = CfgWide;
+    using Thin = CfgThin;
+};
+template <>
+struct CfgOf<float> {
+    using Wide = FCfg<256, 5, 1>;
+    using Thin = FCfg<128, 2, 1>;
+};
 static_assert(7 * 7 * CfgWide::FZ <= CfgWide::FT && 7 * 7 * CfgThin::FZ <= CfgThin::FT, "one point per thread");
 static_assert(2 * CfgThin::LDS <= 160 * 1024, "two thin workgroups per CU");
 // tile groups of the fused path, one launch each
@@ -534,31 +547,38 @@ struct StageIO {
 
 // FACE: the tile's block has flagged faces (closures and boundary values are
 // needed); interior tiles run a body without any of that code.
-template <bool FACE, class C>
+// T: the arithmetic type of derivatives, equations and KO (double, or float
+// for the mixed-precision path, which stages each variable in LDS with a
+// per-tile offset subtracted: the value at the region's first point).
+template <bool FACE, class C, typename T>
 struct FusedCtx {
     static constexpr int FT = C::FT, NREG = C::NREG, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT, PD = C::PD;
     const StageIO &io;
     const dgr::AxisConsts *K;  // x, y, z constants of this block
-    double *s_tile, *s_dx, *s_dy, *s_ko;
+    double *s_tile, *s_dx, *s_dy;  // sized for FP64 staging; viewed as the staging type
+    T *s_ko;
     // tile (uniform)
     int ox, oy, oz, iex, iey, iez;  // tile origin, ie = n - 3 per axis
     bool lox, hix, loy, hiy, loz, hiz;
     uint64_t boff;
     int64_t bq0;
+    uint32_t corner;                // padded offset (within the block) of the region's first point
+    int32_t corner_q;               // its index in the interior-only stage buffers, -1 for padding
     double pmin[3], h[3];
     // this thread
     int tid, ax, ay, az, cofs, dxo, dyo;
     bool face;
-    double sig_gauge, sig_other;
+    T sig_gauge, sig_other;
     // region elements this thread fetches: offset in the padded block, and in
     // the interior-only stage buffer (-1: a padding point)
     uint32_t goff[PREF];
     int32_t qoff[PREF];
     double pre[PD][PREF];  // fetched region of the variables of phases K+1 .. K+PD
-    double f[NV];          // this point's field values
+    double poff[PD];       // the staging offset of each fetched variable (mixed precision)
+    T f[NV];               // this point's field values
     // LDS source offsets of this thread's D_x / D_y evaluations
     int dxs[DXIT], dys[DYIT];
-    double d1v[3], d2v[6];
+    T d1v[3], d2v[6];
 
     __device__ FusedCtx(const StageIO &io_) : io(io_) {}
 
@@ -570,9 +590,13 @@ struct FusedCtx {
         const double *__restrict__ ub = io.u.p[v] + boff;
         if (io.stage >= 1) {
             const double *__restrict__ sb = io.sin.p[v] + bq0;
+            // the offset comes from where the stage reads that point: the
+            // interior of u is being overwritten during the last stage
+            if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = corner_q >= 0 ? sb[corner_q] : ub[corner];
 #pragma unroll
             for (int c = 0; c < PREF; c++) dst[c] = qoff[c] >= 0 ? sb[qoff[c]] : ub[goff[c]];
         } else {
+            if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = ub[corner];
 #pragma unroll
             for (int c = 0; c < PREF; c++) dst[c] = ub[goff[c]];
         }
@@ -581,14 +605,31 @@ struct FusedCtx {
     template <int V, int NEXT, int KPH>
     __device__ __forceinline__ void phase() {
         static_assert(dgr::kBssnPhaseVar[KPH] == V, "phase order");
-        double *t = s_tile + (KPH & 1) * NREG;
+        // S: the type V is staged and differentiated in. The mixed path keeps
+        // alpha and chi in FP64: near the punctures they vary by O(1) within a
+        // tile, and FP32 staging errors, amplified by the second derivatives and
+        // the KO stencil, would swamp their small right-hand sides.
+        constexpr bool HI = sizeof(T) < sizeof(double) && (V == 0 || V == 1);
+        using S = std::conditional_t<HI, double, T>;
+        // buffers sit at multiples of NREG doubles whatever S is, so a phase
+        // never overlaps the other buffer
+        S *t = reinterpret_cast<S *>(s_tile + (KPH & 1) * NREG);
+        if constexpr (sizeof(S) < sizeof(double)) {
+            const double o = poff[KPH % PD];
 #pragma unroll
-        for (int c = 0; c < PREF; c++)
-            if (c + 1 < PREF || tid + c * FT < NREG) t[tid + c * FT] = pre[KPH % PD][c];
+            for (int c = 0; c < PREF; c++)
+                if (c + 1 < PREF || tid + c * FT < NREG) t[tid + c * FT] = S(pre[KPH % PD][c] - o);
+        } else {
+#pragma unroll
+            for (int c = 0; c < PREF; c++)
+                if (c + 1 < PREF || tid + c * FT < NREG) t[tid + c * FT] = pre[KPH % PD][c];
+        }
         if constexpr (KPH + PD < NV) fetch<KPH + PD>();
         __syncthreads();
         const dgr::AxisConsts &kx = K[0], &ky = K[1], &kz = K[2];
         constexpr int m2 = d2slot(V);
+        S *sdx = reinterpret_cast<S *>(s_dx);
+        S *sdy = reinterpret_cast<S *>(s_dy);
         if constexpr (m2 >= 0) {
             // D_x u on tile-x x region-y x region-z and D_y u on tile-x x tile-y x region-z
             // (the reference's mixed derivatives are D_y(D_x u), D_z(D_x u), D_z(D_y u))
@@ -597,59 +638,59 @@ struct FusedCtx {
 #pragma unroll
             for (int it = 0; it < DXIT; it++) {
                 const int e = tid + it * FT;
-                const double *s = t + dxs[it];
-                double d = dgr::d1_in<1>(s, kx.d1_60);
-                if (FACE && (lox || hix)) d = dgr::d1_edge<1>(s, d, ox + e % TMAX, iex, lox, hix, kx.d1_12);
-                s_dx[e] = d;
+                const S *s = t + dxs[it];
+                S d = dgr::d1_in<1>(s, S(kx.d1_60));
+                if (FACE && (lox || hix)) d = dgr::d1_edge<1>(s, d, ox + e % TMAX, iex, lox, hix, S(kx.d1_12));
+                sdx[e] = d;
             }
 #pragma unroll
             for (int it = 0; it < DYIT; it++) {
                 const int e = tid + it * FT;
-                const double *s = t + dys[it];
-                double d = dgr::d1_in<FRX>(s, ky.d1_60);
+                const S *s = t + dys[it];
+                S d = dgr::d1_in<FRX>(s, S(ky.d1_60));
                 if (FACE && (loy || hiy))
-                    d = dgr::d1_edge<FRX>(s, d, oy + (e / TMAX) % TMAX, iey, loy, hiy, ky.d1_12);
-                s_dy[e] = d;
+                    d = dgr::d1_edge<FRX>(s, d, oy + (e / TMAX) % TMAX, iey, loy, hiy, S(ky.d1_12));
+                sdy[e] = d;
             }
             __syncthreads();
         }
-        const double *c = t + cofs;
+        const S *c = t + cofs;
         if constexpr (m2 >= 0) {
-            const double *cx = s_dx + dxo;
-            const double *cy = s_dy + dyo;
-            d1v[0] = cx[0];
-            d1v[1] = cy[0];
-            double r;
-            r = dgr::d2_in<1>(c, kx.d2_180);
-            if (FACE && (lox || hix)) r = dgr::d2_edge<1>(c, r, ax, iex, lox, hix, kx.d2_12);
-            d2v[0] = r;
-            r = dgr::d1_in<DXS_Y>(cx, ky.d1_60);  // D_y(D_x u)
-            if (FACE && (loy || hiy)) r = dgr::d1_edge<DXS_Y>(cx, r, ay, iey, loy, hiy, ky.d1_12);
-            d2v[1] = r;
-            r = dgr::d1_in<DXS_Z>(cx, kz.d1_60);  // D_z(D_x u)
-            if (FACE && (loz || hiz)) r = dgr::d1_edge<DXS_Z>(cx, r, az, iez, loz, hiz, kz.d1_12);
-            d2v[2] = r;
-            r = dgr::d2_in<FRX>(c, ky.d2_180);
-            if (FACE && (loy || hiy)) r = dgr::d2_edge<FRX>(c, r, ay, iey, loy, hiy, ky.d2_12);
-            d2v[3] = r;
-            r = dgr::d1_in<DYS_Z>(cy, kz.d1_60);  // D_z(D_y u)
-            if (FACE && (loz || hiz)) r = dgr::d1_edge<DYS_Z>(cy, r, az, iez, loz, hiz, kz.d1_12);
-            d2v[4] = r;
-            r = dgr::d2_in<FRXY>(c, kz.d2_180);
-            if (FACE && (loz || hiz)) r = dgr::d2_edge<FRXY>(c, r, az, iez, loz, hiz, kz.d2_12);
-            d2v[5] = r;
+            const S *cx = sdx + dxo;
+            const S *cy = sdy + dyo;
+            d1v[0] = T(cx[0]);
+            d1v[1] = T(cy[0]);
+            S r;
+            r = dgr::d2_in<1>(c, S(kx.d2_180));
+            if (FACE && (lox || hix)) r = dgr::d2_edge<1>(c, r, ax, iex, lox, hix, S(kx.d2_12));
+            d2v[0] = T(r);
+            r = dgr::d1_in<DXS_Y>(cx, S(ky.d1_60));  // D_y(D_x u)
+            if (FACE && (loy || hiy)) r = dgr::d1_edge<DXS_Y>(cx, r, ay, iey, loy, hiy, S(ky.d1_12));
+            d2v[1] = T(r);
+            r = dgr::d1_in<DXS_Z>(cx, S(kz.d1_60));  // D_z(D_x u)
+            if (FACE && (loz || hiz)) r = dgr::d1_edge<DXS_Z>(cx, r, az, iez, loz, hiz, S(kz.d1_12));
+            d2v[2] = T(r);
+            r = dgr::d2_in<FRX>(c, S(ky.d2_180));
+            if (FACE && (loy || hiy)) r = dgr::d2_edge<FRX>(c, r, ay, iey, loy, hiy, S(ky.d2_12));
+            d2v[3] = T(r);
+            r = dgr::d1_in<DYS_Z>(cy, S(kz.d1_60));  // D_z(D_y u)
+            if (FACE && (loz || hiz)) r = dgr::d1_edge<DYS_Z>(cy, r, az, iez, loz, hiz, S(kz.d1_12));
+            d2v[4] = T(r);
+            r = dgr::d2_in<FRXY>(c, S(kz.d2_180));
+            if (FACE && (loz || hiz)) r = dgr::d2_edge<FRXY>(c, r, az, iez, loz, hiz, S(kz.d2_12));
+            d2v[5] = T(r);
         } else {
-            double r = dgr::d1_in<1>(c, kx.d1_60);
-            if (FACE && (lox || hix)) r = dgr::d1_edge<1>(c, r, ax, iex, lox, hix, kx.d1_12);
-            d1v[0] = r;
-            r = dgr::d1_in<FRX>(c, ky.d1_60);
-            if (FACE && (loy || hiy)) r = dgr::d1_edge<FRX>(c, r, ay, iey, loy, hiy, ky.d1_12);
-            d1v[1] = r;
+            S r = dgr::d1_in<1>(c, S(kx.d1_60));
+            if (FACE && (lox || hix)) r = dgr::d1_edge<1>(c, r, ax, iex, lox, hix, S(kx.d1_12));
+            d1v[0] = T(r);
+            r = dgr::d1_in<FRX>(c, S(ky.d1_60));
+            if (FACE && (loy || hiy)) r = dgr::d1_edge<FRX>(c, r, ay, iey, loy, hiy, S(ky.d1_12));
+            d1v[1] = T(r);
         }
         {
-            double r = dgr::d1_in<FRXY>(c, kz.d1_60);
-            if (FACE && (loz || hiz)) r = dgr::d1_edge<FRXY>(c, r, az, iez, loz, hiz, kz.d1_12);
-            d1v[2] = r;
+            S r = dgr::d1_in<FRXY>(c, S(kz.d1_60));
+            if (FACE && (loz || hiz)) r = dgr::d1_edge<FRXY>(c, r, az, iez, loz, hiz, S(kz.d1_12));
+            d1v[2] = T(r);
         }
         // computed here, not at their last use (that would keep the LDS taps live)
         DGR_PIN(d1v[0]);
@@ -659,78 +700,85 @@ struct FusedCtx {
 #pragma unroll
             for (int k = 0; k < 6; k++) DGR_PIN(d2v[k]);
         }
-        ko_term<V>(c, kx, ky, kz);
+        ko_term<V, S>(c, kx, ky, kz);
     }
 
     // KO term sigma*(ko_x+ko_y+ko_z) (rhs.cpp:434-489); on a flagged face the
     // boundary value replaces the equations (bssn_bcs, rhs.cpp:511-631) and the
     // KO term is added to it, as the reference adds KO after the boundary pass.
-    template <int V>
-    __device__ __forceinline__ void ko_term(const double *c, const dgr::AxisConsts &kx, const dgr::AxisConsts &ky,
+    template <int V, typename S>
+    __device__ __forceinline__ void ko_term(const S *c, const dgr::AxisConsts &kx, const dgr::AxisConsts &ky,
                                             const dgr::AxisConsts &kz) {
 #pragma clang fp contract(off)
-        double kox = dgr::ko_in<1>(c, kx.ko_pre);
+        S kox = dgr::ko_in<1>(c, S(kx.ko_pre));
         if (FACE && (lox || hix)) kox = dgr::ko_edge<1>(c, kox, ax, iex, lox, hix, kx);
-        double koy = dgr::ko_in<FRX>(c, ky.ko_pre);
+        S koy = dgr::ko_in<FRX>(c, S(ky.ko_pre));
         if (FACE && (loy || hiy)) koy = dgr::ko_edge<FRX>(c, koy, ay, iey, loy, hiy, ky);
-        double koz = dgr::ko_in<FRXY>(c, kz.ko_pre);
+        S koz = dgr::ko_in<FRXY>(c, S(kz.ko_pre));
         if (FACE && (loz || hiz)) koz = dgr::ko_edge<FRXY>(c, koz, az, iez, loz, hiz, kz);
-        const double kt = (is_gauge(V) ? sig_gauge : sig_other) * (kox + koy + koz);
-        double v = kt;
+        const S kt = S(is_gauge(V) ? sig_gauge : sig_other) * (kox + koy + koz);
+        T v = T(kt);
         if (FACE && face) {
+            // the field value itself (the staged one may carry an offset)
             const double x = coord(pmin[0], ax, h[0]);
             const double y = coord(pmin[1], ay, h[1]);
             const double z = coord(pmin[2], az, h[2]);
             const double inv_r = inv_radius(x, y, z);
-            const double bc = -inv_r * (x * d1v[0] + y * d1v[1] + z * d1v[2] + BC_FALLOFF[V] * (c[0] - BC_ASYM[V]));
-            v = bc + kt;
+            const T bc = T(-inv_r) * (T(x) * d1v[0] + T(y) * d1v[1] + T(z) * d1v[2] +
+                                      T(BC_FALLOFF[V]) * (f[V] - T(BC_ASYM[V])));
+            v = bc + T(kt);
         }
         s_ko[V * FT + tid] = v;
     }
 
     template <int V, int D>
-    __device__ __forceinline__ double d1() const { return d1v[D]; }
+    __device__ __forceinline__ T d1() const { return d1v[D]; }
     template <int M, int K2>
-    __device__ __forceinline__ double d2() const { return d2v[K2]; }
+    __device__ __forceinline__ T d2() const { return d2v[K2]; }
     template <int V>
-    __device__ __forceinline__ double fld() const { return f[V]; }
+    __device__ __forceinline__ T fld() const { return f[V]; }
     __device__ __forceinline__ void finish() {}
 };
 
 // The equations' outputs: on a flagged face the boundary value (already
 // holding its KO term), elsewhere equations + KO term, kept in LDS until the
 // epilogue.
-template <bool FACE, class C>
+template <bool FACE, class C, typename T>
 struct FusedOut {
     static constexpr int FT = C::FT;
-    const FusedCtx<FACE, C> &ctx;
-    __device__ explicit FusedOut(const FusedCtx<FACE, C> &c) : ctx(c) {}
+    const FusedCtx<FACE, C, T> &ctx;
+    __device__ explicit FusedOut(const FusedCtx<FACE, C, T> &c) : ctx(c) {}
     template <int V>
-    __device__ __forceinline__ void put(double val) {
+    __device__ __forceinline__ void put(T val) {
 #pragma clang fp contract(off)
-        double *s = ctx.s_ko + V * FT + ctx.tid;
-        const double kb = *s;
+        T *s = ctx.s_ko + V * FT + ctx.tid;
+        const T kb = *s;
         *s = (FACE && ctx.face) ? kb : val + kb;
     }
 };
 
 // RHS store or RK4 stage update of one point (rkBSSN.cpp:764-860), with the
 // loads of all 24 variables issued before any store.
-template <class C>
-__device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *s_ko, const double *f, int tid,
+// The RK arithmetic is FP64 in both precisions; the mixed path's FP32
+// right-hand side is widened first and its u0 read back in FP64.
+template <class C, typename T>
+__device__ __forceinline__ void fused_epilogue(const StageIO &io, const T *s_ko, const T *f, int tid,
                                                uint64_t idx, int64_t q, double chi_floor) {
 #pragma clang fp contract(off)
     constexpr int FT = C::FT;
     const int s = io.stage;
     if (s < 0) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) io.rhs.p[v][idx] = s_ko[v * FT + tid];
+        for (int v = 0; v < NV; v++) io.rhs.p[v][idx] = double(s_ko[v * FT + tid]);
         return;
     }
     double u0[NV], a0[NV], v14[14];
-    if (s == 0) {
+    if (s == 0 && sizeof(T) == sizeof(double)) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) u0[v] = a0[v] = f[v];
+        for (int v = 0; v < NV; v++) u0[v] = a0[v] = double(f[v]);
+    } else if (s == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) u0[v] = a0[v] = io.u.p[v][idx];
     } else {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -741,7 +789,7 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *
     const bool enf = s == 3 || io.enforce_stage;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-        const double k = s_ko[v * FT + tid];
+        const double k = double(s_ko[v * FT + tid]);
         const double acc = a0[v] + io.cacc * k;
         double out;
         if (s < 3) {
@@ -765,22 +813,22 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const double *
     }
 }
 
-template <bool FACE, class C>
-__device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &T, const DBlock &B,
+template <bool FACE, class C, typename T>
+__device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &tl, const DBlock &B,
                                            const dgr::AxisConsts *K, double *s_tile, double *s_dx, double *s_dy,
-                                           double *s_ko) {
+                                           T *s_ko) {
     constexpr int FT = C::FT, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT;
-    FusedCtx<FACE, C> ctx(io);
+    FusedCtx<FACE, C, T> ctx(io);
     ctx.K = K;
     ctx.s_tile = s_tile;
     ctx.s_dx = s_dx;
     ctx.s_dy = s_dy;
     ctx.s_ko = s_ko;
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
-    const int ex = T.e[0], ey = T.e[1], ez = T.e[2];
-    ctx.ox = T.o[0];
-    ctx.oy = T.o[1];
-    ctx.oz = T.o[2];
+    const int ex = tl.e[0], ey = tl.e[1], ez = tl.e[2];
+    ctx.ox = tl.o[0];
+    ctx.oy = tl.o[1];
+    ctx.oz = tl.o[2];
     ctx.iex = nx - PW;
     ctx.iey = ny - PW;
     ctx.iez = nz - PW;
@@ -788,6 +836,12 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     ctx.lox = bf & 1u; ctx.hix = bf & 2u; ctx.loy = bf & 4u; ctx.hiy = bf & 8u; ctx.loz = bf & 16u; ctx.hiz = bf & 32u;
     ctx.boff = B.offset;
     ctx.bq0 = B.q0;
+    ctx.corner = (uint32_t)((ctx.ox - PW) + nx * ((ctx.oy - PW) + ny * (ctx.oz - PW)));
+    {
+        const int ci = ctx.ox - PW, cj = ctx.oy - PW, ck = ctx.oz - PW;
+        const bool inner = ci >= PW && ci < nx - PW && cj >= PW && cj < ny - PW && ck >= PW && ck < nz - PW;
+        ctx.corner_q = inner ? (ci - PW) + (nx - 2 * PW) * ((cj - PW) + (ny - 2 * PW) * (ck - PW)) : -1;
+    }
     for (int a = 0; a < 3; a++) {
         ctx.pmin[a] = B.pmin[a];
         ctx.h[a] = B.h[a];
@@ -839,48 +893,53 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     }
     // this point's field values (registers; the compiler parks them in AGPRs)
     {
-        double *f = ctx.f;
+        T *f = ctx.f;
         if (io.stage >= 1) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) f[v] = io.sin.p[v][q];
+            for (int v = 0; v < NV; v++) f[v] = T(io.sin.p[v][q]);
         } else {
 #pragma unroll
-            for (int v = 0; v < NV; v++) f[v] = io.u.p[v][idx];
+            for (int v = 0; v < NV; v++) f[v] = T(io.u.p[v][idx]);
         }
         if (prm.cako) {
-            const double sg = sqrt(f[1]);
-            ctx.sig_gauge = sg * prm.eps_gauge;
-            ctx.sig_other = sg * prm.eps_other;
+            const T sg = sqrt(f[1]);
+            ctx.sig_gauge = sg * T(prm.eps_gauge);
+            ctx.sig_other = sg * T(prm.eps_other);
         } else {
-            ctx.sig_gauge = ctx.sig_other = prm.ko_sigma;
+            ctx.sig_gauge = ctx.sig_other = T(prm.ko_sigma);
         }
     }
-    const double eta = rit_eta(coord(B.pmin[0], ctx.ax, B.h[0]), coord(B.pmin[1], ctx.ay, B.h[1]),
-                               coord(B.pmin[2], ctx.az, B.h[2]), prm);
+    const T eta = T(rit_eta(coord(B.pmin[0], ctx.ax, B.h[0]), coord(B.pmin[1], ctx.ay, B.h[1]),
+                            coord(B.pmin[2], ctx.az, B.h[2]), prm));
+    const T lam[4] = {T(prm.lam[0]), T(prm.lam[1]), T(prm.lam[2]), T(prm.lam[3])};
+    const T lamf[2] = {T(prm.lamf[0]), T(prm.lamf[1])};
     ctx.template fetch<0>();
     if constexpr (C::PD > 1) ctx.template fetch<1>();
     if constexpr (C::PD > 2) ctx.template fetch<2>();
     static_assert(C::PD <= 3, "prefetch distance");
-    FusedOut<FACE, C> out(ctx);
-    dgr::bssn_rhs_phased<double>(ctx, eta, prm.lam, prm.lamf, out);
-    if (active) fused_epilogue<C>(io, s_ko, ctx.f, tid, idx, q, prm.chi_floor);
+    FusedOut<FACE, C, T> out(ctx);
+    dgr::bssn_rhs_phased<T>(ctx, eta, lam, lamf, out);
+    if (active) fused_epilogue<C, T>(io, s_ko, ctx.f, tid, idx, q, prm.chi_floor);
 }
 
 // One launch per tile kind: FACE = false over the tiles of blocks without
-// flagged faces, FACE = true over the others; C the tile shape.
-template <bool FACE, class C>
-__global__ __launch_bounds__(C::FT) void k_fused(const StageIO io, const RhsParams prm,
-                                                const DBlock *__restrict__ blocks,
-                                                const dgr::AxisConsts *__restrict__ bconst,
-                                                const Tile *__restrict__ tiles) {
+// flagged faces, FACE = true over the others; C the tile shape; T the
+// arithmetic type (FP32 asks for two waves per SIMD: its registers allow it).
+#ifndef DGR_FP32_WAVES
+#define DGR_FP32_WAVES 2  // waves per SIMD the FP32 kernels are built for
+#endif
+template <bool FACE, class C, typename T>
+__global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES : 1) void k_fused(
+    const StageIO io, const RhsParams prm, const DBlock *__restrict__ blocks,
+    const dgr::AxisConsts *__restrict__ bconst, const Tile *__restrict__ tiles) {
     __shared__ double s_tile[2 * C::NREG];
     __shared__ double s_dx[C::DXIT * C::FT];
     __shared__ double s_dy[C::DYIT * C::FT];
-    __shared__ double s_ko[NV * C::FT];
+    __shared__ T s_ko[NV * C::FT];
 
-    const Tile T = tiles[blockIdx.x];
-    const DBlock B = blocks[T.block];
-    fused_tile<FACE, C>(io, prm, T, B, bconst + 3 * (size_t)T.block, s_tile, s_dx, s_dy, s_ko);
+    const Tile tl = tiles[blockIdx.x];
+    const DBlock B = blocks[tl.block];
+    fused_tile<FACE, C, T>(io, prm, tl, B, bconst + 3 * (size_t)tl.block, s_tile, s_dx, s_dy, s_ko);
 }
 
 // -------------------------------------------------------------------------
@@ -1050,6 +1109,7 @@ struct dgr_plan {
     int64_t max_chunk_points = 0;
     double *d_stage = nullptr;  // split path RK4: 2 * 24 * szpdof (stage state, accumulator)
     bool fused = true;          // k_fused (default) or the split k_derivs/k_rhs path (DGR_KERNEL=split)
+    int precision = DGR_PRECISION_FP64;  // arithmetic of the fused kernel
     Tile *d_ftiles = nullptr;   // k_fused tiles, XCD-grouped, group k in [fgroup[k], fgroup[k+1])
     dgr::AxisConsts *d_bconst = nullptr;  // k_fused: x, y, z stencil constants per block
     int nftiles = 0;
@@ -1330,6 +1390,21 @@ int dgr_plan_destroy(dgr_plan *plan) {
 
 int64_t dgr_plan_interior_points(const dgr_plan *plan) { return plan ? plan->interior : -1; }
 
+int dgr_plan_set_precision(dgr_plan *plan, int precision) {
+    if (!plan) return fail(DGR_E_INVALID, "null plan");
+    if (precision != DGR_PRECISION_FP64 && precision != DGR_PRECISION_MIXED)
+        return fail(DGR_E_INVALID, "unknown precision");
+    if (precision != DGR_PRECISION_FP64 && !plan->fused)
+        return fail(DGR_E_INVALID, "mixed precision needs the fused kernel (DGR_KERNEL=split is set)");
+    if (precision != plan->precision && plan->rk4_exec) {  // the captured step used the other kernels
+        HIP_OK(hipSetDevice(plan->device));
+        HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
+        plan->rk4_exec = nullptr;
+    }
+    plan->precision = precision;
+    return DGR_OK;
+}
+
 int dgr_plan_set_timing(dgr_plan *plan, int enable) {
     if (!plan) return fail(DGR_E_INVALID, "null plan");
     plan->timing = enable != 0;
@@ -1427,20 +1502,26 @@ int run_chunks(dgr_plan *plan, const VarPtrs &state, const VarPtrs &rhs, const R
     return DGR_OK;
 }
 
-template <bool FACE, class C>
+template <bool FACE, class C, typename T>
 void launch_fused(dgr_plan *plan, int group, const StageIO &io, const RhsParams &prm, hipStream_t st) {
     const int n = plan->fgroup[group + 1] - plan->fgroup[group];
     if (!n) return;
     timed(plan, 3 + group, st, [&] {
-        hipLaunchKernelGGL((k_fused<FACE, C>), dim3(n), dim3(C::FT), 0, st, io, prm, plan->d_blocks, plan->d_bconst,
-                           plan->d_ftiles + plan->fgroup[group]);
+        hipLaunchKernelGGL((k_fused<FACE, C, T>), dim3(n), dim3(C::FT), 0, st, io, prm, plan->d_blocks,
+                           plan->d_bconst, plan->d_ftiles + plan->fgroup[group]);
     });
 }
 
+template <typename T>
+void launch_fused_all(dgr_plan *plan, const StageIO &io, const RhsParams &prm, hipStream_t st) {
+    launch_fused<false, typename CfgOf<T>::Wide, T>(plan, FG_WIDE, io, prm, st);
+    launch_fused<false, typename CfgOf<T>::Thin, T>(plan, FG_THIN, io, prm, st);
+    launch_fused<true, typename CfgOf<T>::Wide, T>(plan, FG_FACE, io, prm, st);
+}
+
 int run_fused(dgr_plan *plan, const StageIO &io, const RhsParams &prm, hipStream_t st) {
-    launch_fused<false, CfgWide>(plan, FG_WIDE, io, prm, st);
-    launch_fused<false, CfgThin>(plan, FG_THIN, io, prm, st);
-    launch_fused<true, CfgWide>(plan, FG_FACE, io, prm, st);
+    if (plan->precision == DGR_PRECISION_MIXED) launch_fused_all<float>(plan, io, prm, st);
+    else launch_fused_all<double>(plan, io, prm, st);
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }

@@ -109,9 +109,11 @@ __device__ __forceinline__ T ko(const T *s, int st, int a, int n, bool lo, bool 
 
 // Stencil constants of one axis for a whole block, computed once with the
 // same operations as Axis1/Axis2/AxisKO (so the values are identical).
-struct AxisConsts {
-    double d1_12, d1_60, d2_12, d2_180, ko_pre, ko_s3, ko_s2, ko_s1;
+template <typename T>
+struct AxisConstsT {
+    T d1_12, d1_60, d2_12, d2_180, ko_pre, ko_s3, ko_s2, ko_s1;
 };
+using AxisConsts = AxisConstsT<double>;
 
 __host__ __device__ inline AxisConsts axis_consts(double h) {
 #pragma clang fp contract(off)
@@ -135,69 +137,68 @@ __host__ __device__ inline AxisConsts axis_consts(double h) {
 // with ie = n-3) and `r` otherwise, so a caller whose tile touches no flagged
 // face on this axis skips it altogether. Expressions as in d1/d2/ko above.
 #define U(o) s[(o) * ST]
-template <int ST>
-__device__ __forceinline__ double d1_in(const double *s, double by60) {
+template <int ST, typename T>
+__device__ __forceinline__ T d1_in(const T *s, T by60) {
 #pragma clang fp contract(off)
-    return (-U(-3) + 9.0 * U(-2) - 45.0 * U(-1) + 45.0 * U(1) - 9.0 * U(2) + U(3)) * by60;
+    return (-U(-3) + T(9.0) * U(-2) - T(45.0) * U(-1) + T(45.0) * U(1) - T(9.0) * U(2) + U(3)) * by60;
 }
 
-template <int ST>
-__device__ __forceinline__ double d1_edge(const double *s, double r, int a, int ie, bool lo, bool hi, double by12) {
+template <int ST, typename T>
+__device__ __forceinline__ T d1_edge(const T *s, T r, int a, int ie, bool lo, bool hi, T by12) {
 #pragma clang fp contract(off)
     if (lo && a < 6) {
-        if (a == 3) return (-25.0 * U(0) + 48.0 * U(1) - 36.0 * U(2) + 16.0 * U(3) - 3.0 * U(4)) * by12;
-        if (a == 4) return (-3.0 * U(-1) - 10.0 * U(0) + 18.0 * U(1) - 6.0 * U(2) + U(3)) * by12;
-        return (+U(-2) - 8.0 * U(-1) + 8.0 * U(1) - U(2)) * by12;
+        if (a == 3) return (-T(25.0) * U(0) + T(48.0) * U(1) - T(36.0) * U(2) + T(16.0) * U(3) - T(3.0) * U(4)) * by12;
+        if (a == 4) return (-T(3.0) * U(-1) - T(10.0) * U(0) + T(18.0) * U(1) - T(6.0) * U(2) + U(3)) * by12;
+        return (+U(-2) - T(8.0) * U(-1) + T(8.0) * U(1) - U(2)) * by12;
     }
     if (hi && a >= ie - 3) {
-        if (a == ie - 3) return (+U(-2) - 8.0 * U(-1) + 8.0 * U(1) - U(2)) * by12;
-        if (a == ie - 2) return (-U(-3) + 6.0 * U(-2) - 18.0 * U(-1) + 10.0 * U(0) + 3.0 * U(1)) * by12;
-        return (3.0 * U(-4) - 16.0 * U(-3) + 36.0 * U(-2) - 48.0 * U(-1) + 25.0 * U(0)) * by12;
+        if (a == ie - 3) return (+U(-2) - T(8.0) * U(-1) + T(8.0) * U(1) - U(2)) * by12;
+        if (a == ie - 2) return (-U(-3) + T(6.0) * U(-2) - T(18.0) * U(-1) + T(10.0) * U(0) + T(3.0) * U(1)) * by12;
+        return (T(3.0) * U(-4) - T(16.0) * U(-3) + T(36.0) * U(-2) - T(48.0) * U(-1) + T(25.0) * U(0)) * by12;
     }
     return r;
 }
 
-template <int ST>
-__device__ __forceinline__ double d2_in(const double *s, double by180) {
+template <int ST, typename T>
+__device__ __forceinline__ T d2_in(const T *s, T by180) {
 #pragma clang fp contract(off)
-    return (2.0 * U(-3) - 27.0 * U(-2) + 270.0 * U(-1) - 490.0 * U(0) + 270.0 * U(1) - 27.0 * U(2) + 2.0 * U(3)) * by180;
+    return (T(2.0) * U(-3) - T(27.0) * U(-2) + T(270.0) * U(-1) - T(490.0) * U(0) + T(270.0) * U(1) - T(27.0) * U(2) + T(2.0) * U(3)) * by180;
 }
 
-template <int ST>
-__device__ __forceinline__ double d2_edge(const double *s, double r, int a, int ie, bool lo, bool hi, double by12) {
+template <int ST, typename T>
+__device__ __forceinline__ T d2_edge(const T *s, T r, int a, int ie, bool lo, bool hi, T by12) {
 #pragma clang fp contract(off)
     if (lo && a < 6) {
-        if (a == 3) return (45.0 * U(0) - 154.0 * U(1) + 214.0 * U(2) - 156.0 * U(3) + 61.0 * U(4) - 10.0 * U(5)) * by12;
-        if (a == 4) return (10.0 * U(-1) - 15.0 * U(0) - 4.0 * U(1) + 14.0 * U(2) - 6.0 * U(3) + U(4)) * by12;
-        return (-U(-2) + 16.0 * U(-1) - 30.0 * U(0) + 16.0 * U(1) - U(2)) * by12;
+        if (a == 3) return (T(45.0) * U(0) - T(154.0) * U(1) + T(214.0) * U(2) - T(156.0) * U(3) + T(61.0) * U(4) - T(10.0) * U(5)) * by12;
+        if (a == 4) return (T(10.0) * U(-1) - T(15.0) * U(0) - T(4.0) * U(1) + T(14.0) * U(2) - T(6.0) * U(3) + U(4)) * by12;
+        return (-U(-2) + T(16.0) * U(-1) - T(30.0) * U(0) + T(16.0) * U(1) - U(2)) * by12;
     }
     if (hi && a >= ie - 3) {
-        if (a == ie - 3) return (-U(-2) + 16.0 * U(-1) - 30.0 * U(0) + 16.0 * U(1) - U(2)) * by12;
-        if (a == ie - 2) return (U(-4) - 6.0 * U(-3) + 14.0 * U(-2) - 4.0 * U(-1) - 15.0 * U(0) + 10.0 * U(1)) * by12;
-        return (-10.0 * U(-5) + 61.0 * U(-4) - 156.0 * U(-3) + 214.0 * U(-2) - 154.0 * U(-1) + 45.0 * U(0)) * by12;
+        if (a == ie - 3) return (-U(-2) + T(16.0) * U(-1) - T(30.0) * U(0) + T(16.0) * U(1) - U(2)) * by12;
+        if (a == ie - 2) return (U(-4) - T(6.0) * U(-3) + T(14.0) * U(-2) - T(4.0) * U(-1) - T(15.0) * U(0) + T(10.0) * U(1)) * by12;
+        return (-T(10.0) * U(-5) + T(61.0) * U(-4) - T(156.0) * U(-3) + T(214.0) * U(-2) - T(154.0) * U(-1) + T(45.0) * U(0)) * by12;
     }
     return r;
 }
 
-template <int ST>
-__device__ __forceinline__ double ko_in(const double *s, double pre) {
+template <int ST, typename T>
+__device__ __forceinline__ T ko_in(const T *s, T pre) {
 #pragma clang fp contract(off)
-    return pre * (-U(-3) + 6.0 * U(-2) - 15.0 * U(-1) + 20.0 * U(0) - 15.0 * U(1) + 6.0 * U(2) - U(3));
+    return pre * (-U(-3) + T(6.0) * U(-2) - T(15.0) * U(-1) + T(20.0) * U(0) - T(15.0) * U(1) + T(6.0) * U(2) - U(3));
 }
 
-template <int ST>
-__device__ __forceinline__ double ko_edge(const double *s, double r, int a, int ie, bool lo, bool hi,
-                                          const AxisConsts &k) {
+template <int ST, typename T, typename K>
+__device__ __forceinline__ T ko_edge(const T *s, T r, int a, int ie, bool lo, bool hi, const K &k) {
 #pragma clang fp contract(off)
     if (lo && a < 6) {
-        if (a == 3) return (U(3) - 3.0 * U(2) + 3.0 * U(1) - U(0)) / k.ko_s3;
-        if (a == 4) return (U(3) - 6.0 * U(2) + 12.0 * U(1) - 10.0 * U(0) + 3.0 * U(-1)) / k.ko_s2;
-        return (U(3) - 6.0 * U(2) + 15.0 * U(1) - 19.0 * U(0) + 12.0 * U(-1) - 3.0 * U(-2)) / k.ko_s1;
+        if (a == 3) return (U(3) - T(3.0) * U(2) + T(3.0) * U(1) - U(0)) / T(k.ko_s3);
+        if (a == 4) return (U(3) - T(6.0) * U(2) + T(12.0) * U(1) - T(10.0) * U(0) + T(3.0) * U(-1)) / T(k.ko_s2);
+        return (U(3) - T(6.0) * U(2) + T(15.0) * U(1) - T(19.0) * U(0) + T(12.0) * U(-1) - T(3.0) * U(-2)) / T(k.ko_s1);
     }
     if (hi && a >= ie - 3) {
-        if (a == ie - 3) return (U(-3) - 6.0 * U(-2) + 15.0 * U(-1) - 19.0 * U(0) + 12.0 * U(1) - 3.0 * U(2)) / k.ko_s1;
-        if (a == ie - 2) return (U(-3) - 6.0 * U(-2) + 12.0 * U(-1) - 10.0 * U(0) + 3.0 * U(1)) / k.ko_s2;
-        return (U(-3) - 3.0 * U(-2) + 3.0 * U(-1) - U(0)) / k.ko_s3;
+        if (a == ie - 3) return (U(-3) - T(6.0) * U(-2) + T(15.0) * U(-1) - T(19.0) * U(0) + T(12.0) * U(1) - T(3.0) * U(2)) / T(k.ko_s1);
+        if (a == ie - 2) return (U(-3) - T(6.0) * U(-2) + T(12.0) * U(-1) - T(10.0) * U(0) + T(3.0) * U(1)) / T(k.ko_s2);
+        return (U(-3) - T(3.0) * U(-2) + T(3.0) * U(-1) - U(0)) / T(k.ko_s3);
     }
     return r;
 }

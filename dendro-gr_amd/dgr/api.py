@@ -21,8 +21,9 @@ SYMBOLS = [
     "dgr_default_params", "dgr_last_error", "dgr_plan_create", "dgr_plan_destroy",
     "dgr_plan_interior_points", "dgr_bssn_rhs", "dgr_rk4_step", "dgr_enforce_constraints",
     "dgr_plan_status", "dgr_plan_set_timing", "dgr_plan_kernel_times",
-    "dgr_ham_constraint", "dgr_constraint_l2_norm", "dgr_rk4_steps",
+    "dgr_ham_constraint", "dgr_constraint_l2_norm", "dgr_rk4_steps", "dgr_plan_set_precision",
 ]
+PRECISION = {"fp64": 0, "mixed": 1}
 
 
 class BssnParams(C.Structure):
@@ -71,6 +72,7 @@ def load():
         L.dgr_plan_status.argtypes = [vp, vp]
         L.dgr_plan_set_timing.argtypes = [vp, C.c_int]
         L.dgr_plan_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.dgr_plan_set_precision.argtypes = [vp, C.c_int]
         L.dgr_rk4_steps.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), C.c_int, C.c_int, vp]
         L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
         L.dgr_constraint_l2_norm.argtypes = [vp, vp, C.POINTER(C.c_double), C.c_double, C.c_double,
@@ -167,6 +169,10 @@ class Plan:
 
     def status(self, stream=None) -> int:
         return int(load().dgr_plan_status(self._h, _stream_ptr(stream)))
+
+    def set_precision(self, precision: str):
+        """'fp64' (default) or 'mixed' (FP32 derivatives/equations, FP64 state)."""
+        _check(load().dgr_plan_set_precision(self._h, PRECISION[precision]), "dgr_plan_set_precision")
 
     def set_timing(self, enable: bool = True):
         _check(load().dgr_plan_set_timing(self._h, int(enable)), "dgr_plan_set_timing")

@@ -94,6 +94,17 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
                     dgr_plan **out);
 int dgr_plan_destroy(dgr_plan *plan);
 
+/* Arithmetic of dgr_bssn_rhs / dgr_rk4_step(s) (no counterpart in the
+ * reference, which is FP64 only). DGR_PRECISION_FP64 (default): everything in
+ * FP64, rounding like the reference. DGR_PRECISION_MIXED: derivatives,
+ * equations and KO in FP32 on fields staged with a per-tile offset subtracted
+ * (the value at the tile region's first point), while the state, the RK stage
+ * accumulation and the constraint enforcement stay FP64; the right-hand side
+ * is widened to FP64 before it is stored or used. Fused path only. */
+#define DGR_PRECISION_FP64 0
+#define DGR_PRECISION_MIXED 1
+int dgr_plan_set_precision(dgr_plan *plan, int precision);
+
 /* Interior points (sum over blocks of (nx-6)(ny-6)(nz-6)) the plan covers. */
 int64_t dgr_plan_interior_points(const dgr_plan *plan);
 
