@@ -27,6 +27,8 @@ Prints one JSON line. Also reports:
   cpu_baseline  the oracle's OpenMP C restatement on the host cores, one RK4
                 step of a bounded sample of the same workload
   rhs_only      right-hand side alone (no RK update), Mpoints/s
+  mixed_precision  (FP64 runs) the batched mixed-precision path on the same
+                workload: its RK4 Mpoints/s, speedup, and errors against FP64
 """
 import argparse
 import json
@@ -144,6 +146,60 @@ def cpu_baseline(wl, nblocks_uniform, threads, reps=3):
                       f"OpenMP over blocks, median of {reps} after 1 warm-up"}
 
 
+def mixed_comparison(wl, state0, params, steps, fp64_value, stream):
+    """The batched mixed-precision path on the same workload: RK4 Mpoints/s
+    (graph replay), and its error against the FP64 path from identical inputs:
+    the right-hand side (max over variables of the normwise relative error
+    over interior points) and the state after one RK4 step (max |difference|
+    over the state's scale, max |u|, and per variable)."""
+    import torch
+    import dgr
+    from dgr import workloads as W
+    mask = torch.from_numpy(interior_mask(wl.blocks, wl.n_pad)).to(state0.device)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        plan = dgr.Plan(wl.blocks, wl.n_pad)
+        plan.set_precision(prec)
+        out = torch.empty_like(state0)
+        plan.rhs(out, state0, params, stream=stream)
+        u = state0.clone()
+        plan.rk4_step(u, wl.dt, params, stream=stream)
+        torch.cuda.synchronize()
+        res[prec] = (out[:, mask], u[:, mask])
+        if prec == "mixed":
+            plan.rk4_steps(u, wl.dt, params, nsteps=2, stream=stream)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plan.rk4_steps(u, wl.dt, params, nsteps=steps, stream=stream)
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - t0
+            if plan.status(stream) != 0:
+                raise RuntimeError("constraint violation in the mixed-precision steps")
+        plan.close()
+        del out, u
+    value = wl.npts * 4 * steps / elapsed / 1e6
+    (r64, u64), (r32, u32) = res["fp64"], res["mixed"]
+    e_rhs = ((r32 - r64).abs().amax(dim=1) / r64.abs().amax(dim=1).clamp_min(1e-300)).max().item()
+    e_var = ((u32 - u64).abs().amax(dim=1) / u64.abs().amax(dim=1).clamp_min(1e-300)).max().item()
+    e_scale = ((u32 - u64).abs().max() / u64.abs().max()).item()
+    return {"value": round(value, 2), "unit": "Mpoints/s", "speedup_vs_fp64": round(value / fp64_value, 3),
+            "dtype": "f32 arithmetic (alpha, chi staged and differentiated in f64), f64 state and RK",
+            "error_vs_fp64": {"state_after_one_step_vs_state_scale": e_scale,
+                              "state_after_one_step_max_per_variable": e_var,
+                              "rhs_max_normwise_per_variable": e_rhs}}
+
+
+def interior_mask(blocks, npad):
+    m = np.zeros(npad, dtype=bool)
+    for b in blocks:
+        nx, ny, nz = (int(v) for v in b["sz"])
+        blk = np.zeros((nz, ny, nx), dtype=bool)
+        blk[3:nz - 3, 3:ny - 3, 3:nx - 3] = True
+        off = int(b["offset"])
+        m[off:off + nx * ny * nz] = blk.ravel()
+    return m
+
+
 def roofline(wl, kt, stages):
     from dgr import workloads as W
     kind_pts = W.fused_kind_points(wl.blocks)
@@ -183,6 +239,8 @@ def main():
     ap.add_argument("--cpu-blocks", type=int, default=1024, help="uniform13: CPU baseline sample")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 derivatives/equations/KO on offset-subtracted fields, FP64 state")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision comparison of the FP64 run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (3 times), to scale "
@@ -211,6 +269,7 @@ def main():
     wl = Workload(args.workload, args.nblocks, args.scale)
     log(f"workload: {wl.desc}; {wl.npts} interior points, state {24 * wl.n_pad * 8 / 1e9:.2f} GB")
     state = wl.device_state(dev)
+    state0 = state.clone() if (args.precision == "fp64" and not args.no_mixed) else None
     plan = dgr.Plan(wl.blocks, wl.n_pad)
     plan.set_precision(args.precision)
     params = dgr.default_params(**wl.params)
@@ -255,6 +314,11 @@ def main():
     del out
     plan.close()
 
+    mixed = None
+    if state0 is not None:
+        mixed = mixed_comparison(wl, state0, params, args.steps, value, stream)
+    del state
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -275,7 +339,7 @@ def main():
                        "precision": args.precision, "rhs_evals_per_step": 4, "rk4_step": "hipGraph replay",
                        "dt": wl.dt, **wl.params},
             "rhs_only_mpoints_s": round(rhs_only, 2),
-            "roofline": roof, "cpu_baseline": cpu}
+            "roofline": roof, "cpu_baseline": cpu, "mixed_precision": mixed}
     print(json.dumps(line), flush=True)
 
 
