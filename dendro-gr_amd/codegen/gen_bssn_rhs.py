@@ -430,8 +430,10 @@ DGR_HD DGR_INLINE void bssn_rhs_point(const In &in, const T eta, const T *lam, c
 
 
 # Order in which the variables stream through the fused kernel's LDS.
-# Found by local search over orders for the smallest live set of the schedule.
-PHASE_ORDER = ["Gt2", "gt1", "Gt0", "chi", "Gt1", "gt3", "gt5", "gt4", "gt0", "gt2", "B2", "beta2",
+# Found by local search over orders for the smallest live set of the schedule;
+# alpha and chi sit at even positions (the mixed-precision kernel stages those
+# two in FP64, and only its even-phase LDS buffer is FP64-sized).
+PHASE_ORDER = ["Gt2", "gt1", "chi", "Gt0", "Gt1", "gt3", "gt5", "gt4", "gt0", "gt2", "B2", "beta2",
                "B0", "At5", "beta0", "beta1", "B1", "K", "alpha", "At0", "At4", "At1", "At3", "At2"]
 
 

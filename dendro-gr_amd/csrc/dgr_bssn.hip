@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -70,9 +72,10 @@ struct DBlock {  // device copy of a block with its grid spacing
 
 struct Tile {
     int block;
-    int q0;     // first workspace point of the tile within its chunk
-    int o[3];   // absolute index of the tile's first point along each axis
-    int e[3];   // tile extents
+    int q0;       // first workspace point of the tile within its chunk (split path)
+    int o[3];     // absolute index of the tile's first point along each axis
+    int e[3];     // tile extents
+    int lanetab;  // fused path: offset of the tile shape's lane -> point table
 };
 
 struct Chunk {
@@ -486,8 +489,12 @@ __global__ __launch_bounds__(NTHREADS) void k_face(VarPtrs u, VarPtrs rhs, RKArg
 // sit behind a branch that is uniform over the tile.
 // -------------------------------------------------------------------------
 constexpr int FRX = RMAX, FRXY = RMAX * RMAX;  // LDS strides of the staged region (13, 169)
-constexpr int DXS_Y = TMAX, DXS_Z = TMAX * RMAX;     // D_x u: tile x, region y, region z
-constexpr int DYS_Y = TMAX, DYS_Z = TMAX * TMAX;     // D_y u: tile x, tile y, region z
+// D_x u (tile x, region y, region z) and D_y u (tile x, tile y, region z) use
+// the same strides, so one lane -> point assignment keeps every access pattern
+// of the own-point stencils free of LDS bank conflicts (see lane_table)
+constexpr int DXS_Y = FRX, DXS_Z = FRXY;
+constexpr int DYS_Y = FRX, DYS_Z = FRXY;
+constexpr int LANE_SHADOW = 0x8000;  // lane table: the lane repeats a point and stores nothing
 
 // Tile shape of a k_fused launch: FT threads (one point each) and tiles of at
 // most 7 x 7 x FZ points. The regular shape is 256 threads / z <= 5 (one
@@ -505,11 +512,12 @@ struct FCfg {
     static constexpr int PREF = (NREG + FT - 1) / FT;       // region elements per thread
     static constexpr int NDX = TMAX * RMAX * (FZ + 2 * PW);  // D_x points
     static constexpr int NDY = TMAX * TMAX * (FZ + 2 * PW);  // D_y points
-    static constexpr int DXIT = (NDX + FT - 1) / FT;
+    static constexpr int DXIT = (NDX + FT - 1) / FT;         // D_x evaluations per thread
     static constexpr int DYIT = (NDY + FT - 1) / FT;
-    // LDS: two region buffers, D_x / D_y (padded to whole passes of the
-    // workgroup, so their evaluation needs no bounds test), KO terms
-    static constexpr size_t LDS = sizeof(double) * (2 * NREG + (DXIT + DYIT) * FT + NV * FT);
+    static constexpr int NSDX = TMAX + FRX * (RMAX - 1) + FRXY * (FZ + 2 * PW - 1);  // D_x array
+    static constexpr int NSDY = TMAX + FRX * (TMAX - 1) + FRXY * (FZ + 2 * PW - 1);  // D_y array
+    // LDS (FP64 kernel): two region buffers, D_x / D_y, KO terms
+    static constexpr size_t LDS = sizeof(double) * (2 * NREG + NSDX + NSDY + NV * FT);
 };
 using CfgWide = FCfg<256, 5, 3>;
 using CfgThin = FCfg<128, 2, 2>;
@@ -526,7 +534,12 @@ struct CfgOf<float> {
     using Thin = FCfg<128, 2, 1>;
 };
 static_assert(7 * 7 * CfgWide::FZ <= CfgWide::FT && 7 * 7 * CfgThin::FZ <= CfgThin::FT, "one point per thread");
-static_assert(2 * CfgThin::LDS <= 160 * 1024, "two thin workgroups per CU");
+static_assert(2 * CfgThin::LDS <= 160 * 1024, "two thin FP64 workgroups per CU");
+// FP32 regular tiles: one FP64 and one FP32 region buffer, FP64-sized D_x /
+// D_y arrays (alpha and chi are differentiated in FP64), FP32 KO terms
+static_assert(2 * (sizeof(double) * (CfgWide::NREG + (CfgWide::NREG + 1) / 2 + CfgWide::NSDX + CfgWide::NSDY) +
+                   sizeof(float) * NV * CfgWide::FT) <= 160 * 1024,
+              "two FP32 workgroups per CU");
 // tile groups of the fused path, one launch each
 enum { FG_WIDE = 0, FG_THIN = 1, FG_FACE = 2, NFGROUP = 3 };
 
@@ -577,7 +590,7 @@ struct FusedCtx {
     double poff[PD];       // the staging offset of each fetched variable (mixed precision)
     T f[NV];               // this point's field values
     // LDS source offsets of this thread's D_x / D_y evaluations
-    int dxs[DXIT], dys[DYIT];
+    int dxw[DXIT], dyw[DYIT];  // where this thread's D_x / D_y evaluations go (see dtable)
     T d1v[3], d2v[6];
 
     __device__ FusedCtx(const StageIO &io_) : io(io_) {}
@@ -610,6 +623,7 @@ struct FusedCtx {
         // tile, and FP32 staging errors, amplified by the second derivatives and
         // the KO stencil, would swamp their small right-hand sides.
         constexpr bool HI = sizeof(T) < sizeof(double) && (V == 0 || V == 1);
+        static_assert(!HI || KPH % 2 == 0, "FP64-staged variables need the FP64-sized (even) buffer");
         using S = std::conditional_t<HI, double, T>;
         // buffers sit at multiples of NREG doubles whatever S is, so a phase
         // never overlaps the other buffer
@@ -637,20 +651,20 @@ struct FusedCtx {
             // tile's region read valid LDS and land in padding nobody reads
 #pragma unroll
             for (int it = 0; it < DXIT; it++) {
-                const int e = tid + it * FT;
-                const S *s = t + dxs[it];
+                const int w = dxw[it];         // i0 + 13 jj + 169 kk
+                const S *s = t + (w + PW);     // region point (3+i0, jj, kk)
                 S d = dgr::d1_in<1>(s, S(kx.d1_60));
-                if (FACE && (lox || hix)) d = dgr::d1_edge<1>(s, d, ox + e % TMAX, iex, lox, hix, S(kx.d1_12));
-                sdx[e] = d;
+                if (FACE && (lox || hix)) d = dgr::d1_edge<1>(s, d, ox + w % FRX, iex, lox, hix, S(kx.d1_12));
+                sdx[w] = d;
             }
 #pragma unroll
             for (int it = 0; it < DYIT; it++) {
-                const int e = tid + it * FT;
-                const S *s = t + dys[it];
+                const int w = dyw[it];                 // i0 + 13 j0 + 169 kk
+                const S *s = t + (w + PW + FRX * PW);  // region point (3+i0, 3+j0, kk)
                 S d = dgr::d1_in<FRX>(s, S(ky.d1_60));
                 if (FACE && (loy || hiy))
-                    d = dgr::d1_edge<FRX>(s, d, oy + (e / TMAX) % TMAX, iey, loy, hiy, S(ky.d1_12));
-                sdy[e] = d;
+                    d = dgr::d1_edge<FRX>(s, d, oy + (w / FRX) % FRX, iey, loy, hiy, S(ky.d1_12));
+                sdy[w] = d;
             }
             __syncthreads();
         }
@@ -815,8 +829,8 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const T *s_ko,
 
 template <bool FACE, class C, typename T>
 __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &tl, const DBlock &B,
-                                           const dgr::AxisConsts *K, double *s_tile, double *s_dx, double *s_dy,
-                                           T *s_ko) {
+                                           const dgr::AxisConsts *K, const uint16_t *__restrict__ tabs,
+                                           int dtab, double *s_tile, double *s_dx, double *s_dy, T *s_ko) {
     constexpr int FT = C::FT, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT;
     FusedCtx<FACE, C, T> ctx(io);
     ctx.K = K;
@@ -849,9 +863,11 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     const int mx = nx - 2 * PW, my = ny - 2 * PW;
     const int tid = threadIdx.x;
     ctx.tid = tid;
-    const int npts = ex * ey * ez;
-    const bool active = tid < npts;
-    const int p = active ? tid : npts - 1;
+    // this thread's point: from the tile shape's lane table (conflict-free LDS
+    // access for the own-point stencils); shadow lanes repeat a point
+    const int lt = tabs[tl.lanetab + tid];
+    const bool active = !(lt & LANE_SHADOW);
+    const int p = lt & (LANE_SHADOW - 1);
     const int li = p % ex, lj = (p / ex) % ey, lk = p / (ex * ey);
     ctx.ax = ctx.ox + li;
     ctx.ay = ctx.oy + lj;
@@ -877,20 +893,11 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
         ctx.goff[c] = in_region ? (uint32_t)(gi + nx * (gj + ny * gk)) : 0u;
         ctx.qoff[c] = inner ? (gi - PW) + mx * ((gj - PW) + my * (gk - PW)) : -1;
     }
-    // LDS sources of the D_x points e -> (i0, jj, kk) and D_y points e -> (i0, j0, kk)
+    // where this thread's D_x / D_y evaluations go (the config's slot table)
 #pragma unroll
-    for (int it = 0; it < DXIT; it++) {
-        const int e = tid + it * FT;
-        const int i0 = e % TMAX, r = e / TMAX;
-        // (beyond the static region: a point whose taps, closures included, stay in the buffer)
-        ctx.dxs[it] = e < C::NDX ? (PW + i0) + FRX * (r % RMAX) + FRXY * (r / RMAX) : PW + 2;
-    }
+    for (int it = 0; it < DXIT; it++) ctx.dxw[it] = tabs[dtab + it * FT + tid];
 #pragma unroll
-    for (int it = 0; it < DYIT; it++) {
-        const int e = tid + it * FT;
-        const int i0 = e % TMAX, r = e / TMAX;
-        ctx.dys[it] = e < C::NDY ? (PW + i0) + FRX * (PW + r % TMAX) + FRXY * (r / TMAX) : PW + FRX * (PW + 2);
-    }
+    for (int it = 0; it < DYIT; it++) ctx.dyw[it] = tabs[dtab + (DXIT + it) * FT + tid];
     // this point's field values (registers; the compiler parks them in AGPRs)
     {
         T *f = ctx.f;
@@ -931,15 +938,18 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
 template <bool FACE, class C, typename T>
 __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES : 1) void k_fused(
     const StageIO io, const RhsParams prm, const DBlock *__restrict__ blocks,
-    const dgr::AxisConsts *__restrict__ bconst, const Tile *__restrict__ tiles) {
-    __shared__ double s_tile[2 * C::NREG];
-    __shared__ double s_dx[C::DXIT * C::FT];
-    __shared__ double s_dy[C::DYIT * C::FT];
+    const dgr::AxisConsts *__restrict__ bconst, const Tile *__restrict__ tiles,
+    const uint16_t *__restrict__ tabs, int dtab) {
+    // the region buffers sit at multiples of NREG doubles; in the FP32 kernel
+    // only the even-phase buffer holds FP64 data, the odd one FP32
+    __shared__ double s_tile[C::NREG + (sizeof(T) < sizeof(double) ? (C::NREG + 1) / 2 : C::NREG)];
+    __shared__ double s_dx[C::NSDX];
+    __shared__ double s_dy[C::NSDY];
     __shared__ T s_ko[NV * C::FT];
 
     const Tile tl = tiles[blockIdx.x];
     const DBlock B = blocks[tl.block];
-    fused_tile<FACE, C, T>(io, prm, tl, B, bconst + 3 * (size_t)tl.block, s_tile, s_dx, s_dy, s_ko);
+    fused_tile<FACE, C, T>(io, prm, tl, B, bconst + 3 * (size_t)tl.block, tabs, dtab, s_tile, s_dx, s_dy, s_ko);
 }
 
 // -------------------------------------------------------------------------
@@ -1112,6 +1122,8 @@ struct dgr_plan {
     int precision = DGR_PRECISION_FP64;  // arithmetic of the fused kernel
     Tile *d_ftiles = nullptr;   // k_fused tiles, XCD-grouped, group k in [fgroup[k], fgroup[k+1])
     dgr::AxisConsts *d_bconst = nullptr;  // k_fused: x, y, z stencil constants per block
+    uint16_t *d_tabs = nullptr;           // k_fused: lane tables per tile shape, slot tables per config
+    int dtab_wide = 0, dtab_thin = 0;     // offsets of the two configurations' slot tables
     int nftiles = 0;
     int fgroup[NFGROUP + 1] = {};
     double *d_istage = nullptr; // k_fused RK4: 3 * 24 * interior (two stage states, accumulator)
@@ -1143,6 +1155,73 @@ struct dgr_plan {
         return e;
     }
 };
+
+namespace {
+
+// Lane -> point table for a tile of ex x ey x ez points on ft lanes. Lane
+// 32g + r takes a point whose LDS address i + 13 j + 169 k is congruent to r
+// mod 32, so the 32 lanes of a group touch 32 different bank pairs at every
+// stencil tap (8-byte words: conflict-free both for 32 banks serving 16 lanes
+// and 64 banks serving 32 lanes per cycle). Points beyond ft/32 in a residue
+// class take free lanes; lanes still free repeat a point of their own residue
+// class (same address, so no conflict) and store nothing.
+void lane_table(int ex, int ey, int ez, int ft, std::vector<uint16_t> &out) {
+    const int n = ex * ey * ez, groups = ft / 32;
+    std::vector<std::vector<int>> res(32);
+    for (int p = 0; p < n; p++) {
+        const int li = p % ex, lj = (p / ex) % ey, lk = p / (ex * ey);
+        res[(li + FRX * lj + FRXY * lk) % 32].push_back(p);
+    }
+    std::vector<int> lane(ft, -1), over;
+    for (int r = 0; r < 32; r++)
+        for (int g = 0; g < (int)res[r].size(); g++) {
+            if (g < groups) lane[32 * g + r] = res[r][g];
+            else over.push_back(res[r][g]);
+        }
+    size_t o = 0;
+    for (int l = 0; l < ft && o < over.size(); l++)
+        if (lane[l] < 0) lane[l] = over[o++];
+    for (int l = 0; l < ft; l++) {
+        if (lane[l] >= 0) {
+            out.push_back((uint16_t)lane[l]);
+        } else {
+            const std::vector<int> &rr = res[l % 32];
+            out.push_back((uint16_t)((rr.empty() ? 0 : rr[0]) | LANE_SHADOW));
+        }
+    }
+}
+
+// Slot table of a tile configuration: slot (it, tid) -> the position
+// w = i0 + 13 j + 169 k it evaluates and writes, first D_x (i0 < 7, j < 13,
+// k < FZ+6) for DXIT passes, then D_y (i0 < 7, j < 7) for DYIT passes,
+// grouped like lane_table; unused slots get w = 7 (a column no stencil reads).
+template <class C>
+void dtable(std::vector<uint16_t> &out) {
+    auto fill = [&](int nj, int nit) {
+        std::vector<std::vector<int>> res(32);
+        for (int k = 0; k < C::FZ + 2 * PW; k++)
+            for (int j = 0; j < nj; j++)
+                for (int i = 0; i < TMAX; i++) {
+                    const int w = i + FRX * j + FRXY * k;
+                    res[w % 32].push_back(w);
+                }
+        const int nslot = nit * C::FT, groups = nslot / 32;
+        std::vector<int> slot(nslot, -1), over;
+        for (int r = 0; r < 32; r++)
+            for (int g = 0; g < (int)res[r].size(); g++) {
+                if (g < groups) slot[32 * g + r] = res[r][g];
+                else over.push_back(res[r][g]);
+            }
+        size_t o = 0;
+        for (int q = 0; q < nslot && o < over.size(); q++)
+            if (slot[q] < 0) slot[q] = over[o++];
+        for (int q = 0; q < nslot; q++) out.push_back((uint16_t)(slot[q] < 0 ? TMAX : slot[q]));
+    };
+    fill(RMAX, C::DXIT);
+    fill(TMAX, C::DYIT);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1258,7 +1337,7 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
         for (auto &z : pz)
             for (auto &y : py)
                 for (auto &x : px) {
-                    Tile t;
+                    Tile t{};
                     t.block = (int)b;
                     t.q0 = (int)chunk_pts;
                     t.o[0] = x.first; t.o[1] = y.first; t.o[2] = z.first;
@@ -1305,7 +1384,7 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
             for (auto &z : pz)
                 for (auto &y : py)
                     for (auto &x : px) {
-                        Tile t;
+                        Tile t{};
                         t.block = (int)b;
                         t.q0 = 0;
                         t.o[0] = x.first; t.o[1] = y.first; t.o[2] = z.first;
@@ -1331,6 +1410,27 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
         plan->fgroup[NFGROUP] = (int)ftiles.size();
     }
     plan->nftiles = (int)ftiles.size();
+    // lane tables per tile shape, slot tables per configuration
+    std::vector<uint16_t> tabs;
+    plan->dtab_wide = (int)tabs.size();
+    dtable<CfgWide>(tabs);
+    plan->dtab_thin = (int)tabs.size();
+    dtable<CfgThin>(tabs);
+    {
+        std::map<std::array<int, 4>, int> shape_off;
+        for (int g = 0; g < NFGROUP; g++)
+            for (int i = plan->fgroup[g]; i < plan->fgroup[g + 1]; i++) {
+                Tile &t = ftiles[i];
+                const int ft = g == FG_THIN ? CfgThin::FT : CfgWide::FT;
+                const std::array<int, 4> key{t.e[0], t.e[1], t.e[2], ft};
+                auto it = shape_off.find(key);
+                if (it == shape_off.end()) {
+                    it = shape_off.emplace(key, (int)tabs.size()).first;
+                    lane_table(t.e[0], t.e[1], t.e[2], ft, tabs);
+                }
+                t.lanetab = it->second;
+            }
+    }
     std::vector<dgr::AxisConsts> bconst(3 * (size_t)nblocks);
     for (uint32_t b = 0; b < nblocks; b++)
         for (int a = 0; a < 3; a++) bconst[3 * b + a] = dgr::axis_consts(hb[b].h[a]);
@@ -1342,6 +1442,8 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
     if (e == hipSuccess && !wg_tile.empty()) e = hipMalloc(&plan->d_wg_tile, sizeof(int) * wg_tile.size());
     if (e == hipSuccess && !face_tiles.empty()) e = hipMalloc(&plan->d_face_tiles, sizeof(int) * face_tiles.size());
     if (e == hipSuccess && !ftiles.empty()) e = hipMalloc(&plan->d_ftiles, sizeof(Tile) * ftiles.size());
+    if (e == hipSuccess) e = hipMalloc(&plan->d_tabs, sizeof(uint16_t) * tabs.size());
+    if (e == hipSuccess) e = hipMemcpy(plan->d_tabs, tabs.data(), sizeof(uint16_t) * tabs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && nblocks) e = hipMalloc(&plan->d_bconst, sizeof(dgr::AxisConsts) * bconst.size());
     if (e == hipSuccess && nblocks)
         e = hipMemcpy(plan->d_bconst, bconst.data(), sizeof(dgr::AxisConsts) * bconst.size(), hipMemcpyHostToDevice);
@@ -1379,6 +1481,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     hipFree(plan->d_stage);
     hipFree(plan->d_ftiles);
     hipFree(plan->d_bconst);
+    hipFree(plan->d_tabs);
     hipFree(plan->d_istage);
     hipFree(plan->d_err);
     hipFree(plan->d_norm);
@@ -1508,7 +1611,8 @@ void launch_fused(dgr_plan *plan, int group, const StageIO &io, const RhsParams 
     if (!n) return;
     timed(plan, 3 + group, st, [&] {
         hipLaunchKernelGGL((k_fused<FACE, C, T>), dim3(n), dim3(C::FT), 0, st, io, prm, plan->d_blocks,
-                           plan->d_bconst, plan->d_ftiles + plan->fgroup[group]);
+                           plan->d_bconst, plan->d_ftiles + plan->fgroup[group], plan->d_tabs,
+                           C::FT == CfgThin::FT && C::FZ == CfgThin::FZ ? plan->dtab_thin : plan->dtab_wide);
     });
 }
 
