@@ -200,7 +200,7 @@ def interior_mask(blocks, npad):
     return m
 
 
-def roofline(wl, kt, stages):
+def roofline(wl, kt, stages, precision="fp64"):
     from dgr import workloads as W
     kind_pts = W.fused_kind_points(wl.blocks)
     kind_bytes = W.fused_kind_bytes(wl.blocks)
@@ -217,9 +217,10 @@ def roofline(wl, kt, stages):
     dom = max(per_kernel, key=lambda k: kt[k][0])
     dom_ms = kt[dom][0] / kt[dom][1]
     dom_bytes = kind_bytes.get(dom) or W.stage_bytes(wl.blocks)
+    precision_tag = "" if precision == "fp64" else "_" + precision
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, wl.key),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, wl.key + precision_tag),
             "kernel": KERNEL_NAMES.get(dom, dom), "launch_ms": round(dom_ms, 5),
             "algorithmic_bytes_per_launch": dom_bytes,
             "algorithmic_bytes_per_point": round(W.stage_bytes(wl.blocks) / wl.npts, 1),
@@ -310,7 +311,7 @@ def main():
         plan.rhs(out, state, params, stream=stream)
     torch.cuda.synchronize()
     rhs_only = wl.npts * nrhs / (time.perf_counter() - r0) / 1e6
-    roof = roofline(wl, kt, 4 * ksteps)
+    roof = roofline(wl, kt, 4 * ksteps, args.precision)
     del out
     plan.close()
 
