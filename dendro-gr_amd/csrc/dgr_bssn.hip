@@ -510,7 +510,9 @@ struct FCfg {
     static constexpr int PD = PD_;                          // variables in flight ahead of the current one
     static constexpr int NREG = FRXY * (FZ + 2 * PW);       // staged region (13 x 13 x (FZ+6))
     static constexpr int PREF = (NREG + FT - 1) / FT;       // region elements per thread
-    static constexpr int NDX = TMAX * RMAX * (FZ + 2 * PW);  // D_x points
+    // D_x points: D_y(D_x u) at the tile's points reads D_x on its z planes
+    // (all 13 region rows), D_z(D_x u) on its y rows (all FZ+6 region planes)
+    static constexpr int NDX = TMAX * (RMAX * FZ + TMAX * 2 * PW);
     static constexpr int NDY = TMAX * TMAX * (FZ + 2 * PW);  // D_y points
     static constexpr int DXIT = (NDX + FT - 1) / FT;         // D_x evaluations per thread
     static constexpr int DYIT = (NDY + FT - 1) / FT;
@@ -969,6 +971,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
 #ifndef DGR_FP32_WAVES
 #define DGR_FP32_WAVES 2  // waves per SIMD the FP32 kernels are built for
 #endif
+
 template <bool FACE, class C, typename T>
 __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES : 1) void k_fused(
     const StageIO io, const RhsParams prm, const DBlock *__restrict__ blocks,
@@ -1226,19 +1229,27 @@ void lane_table(int ex, int ey, int ez, int ft, std::vector<uint16_t> &out) {
 }
 
 // Slot table of a tile configuration: slot (it, tid) -> the position
-// w = i0 + 13 j + 169 k it evaluates and writes, first D_x (i0 < 7, j < 13,
-// k < FZ+6) for DXIT passes, then D_y (i0 < 7, j < 7) for DYIT passes,
-// grouped like lane_table; unused slots get w = 7 (a column no stencil reads).
+// w = i0 + 13 j + 169 k it evaluates and writes, first D_x for DXIT passes,
+// then D_y for DYIT passes, grouped like lane_table; unused slots get w = 7
+// (a column no stencil reads). D_x: i0 < 7 and either a tile plane
+// (PW <= k < PW+FZ, any j < 13) or a tile row (PW <= j < PW+7, any k);
+// D_y: i0 < 7, j < 7, any k. (k < FZ+6 throughout; the tile sits at j, k >= PW.)
 template <class C>
-void dtable(std::vector<uint16_t> &out) {
-    auto fill = [&](int nj, int nit) {
+bool dtable(std::vector<uint16_t> &out) {
+    bool ok = true;
+    auto fill = [&](int nj, int nit, bool dx) {
         std::vector<std::vector<int>> res(32);
+        int n = 0;
         for (int k = 0; k < C::FZ + 2 * PW; k++)
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < TMAX; i++) {
+                    const bool plane = k >= PW && k < PW + C::FZ, row = j >= PW && j < PW + TMAX;
+                    if (dx && !plane && !row) continue;
                     const int w = i + FRX * j + FRXY * k;
                     res[w % 32].push_back(w);
+                    n++;
                 }
+        ok = ok && n == (dx ? C::NDX : C::NDY) && n <= nit * C::FT;
         const int nslot = nit * C::FT, groups = nslot / 32;
         std::vector<int> slot(nslot, -1), over;
         for (int r = 0; r < 32; r++)
@@ -1251,8 +1262,9 @@ void dtable(std::vector<uint16_t> &out) {
             if (slot[q] < 0) slot[q] = over[o++];
         for (int q = 0; q < nslot; q++) out.push_back((uint16_t)(slot[q] < 0 ? TMAX : slot[q]));
     };
-    fill(RMAX, C::DXIT);
-    fill(TMAX, C::DYIT);
+    fill(RMAX, C::DXIT, true);
+    fill(TMAX, C::DYIT, false);
+    return ok;
 }
 
 }  // namespace
@@ -1447,9 +1459,13 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
     // lane tables per tile shape, slot tables per configuration
     std::vector<uint16_t> tabs;
     plan->dtab_wide = (int)tabs.size();
-    dtable<CfgWide>(tabs);
+    bool tabs_ok = dtable<CfgWide>(tabs);
     plan->dtab_thin = (int)tabs.size();
-    dtable<CfgThin>(tabs);
+    tabs_ok = dtable<CfgThin>(tabs) && tabs_ok;
+    if (!tabs_ok) {
+        dgr_plan_destroy(plan);
+        return fail(DGR_E_INVALID, "internal error: D_x / D_y slot table does not match its configuration");
+    }
     {
         std::map<std::array<int, 4>, int> shape_off;
         for (int g = 0; g < NFGROUP; g++)
