@@ -45,6 +45,11 @@ sys.path.insert(0, os.path.join(ROOT, "dendro-gr_amd"))
 
 METRIC = "BSSN RHS Mpoints/s on 1 MI355X: plain FP64 vs batched mixed at ≤1e-8 rel err"
 HBM_PEAK_GBS = 8000.0
+# MI355X vector-ALU peaks (TFLOP/s) and the algorithmic FLOPs of one RK stage
+# per interior point: 6788 for the right-hand side (derivatives 864 + 462 + 396,
+# KO 1008 + 96, equations 3952, eta 10) + 96 for the stage update
+VALU_PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}
+FLOP_PER_POINT_STAGE = 6788 + 96
 KERNEL_NAMES = {"fused_wide": "k_fused<interior, 256 threads, 7x7x5 tiles>",
                 "fused_thin": "k_fused<interior, 128 threads, 7x7x2 tiles>",
                 "fused_face": "k_fused<flagged faces, 256 threads>",
@@ -218,12 +223,17 @@ def roofline(wl, kt, stages, precision="fp64"):
     dom_ms = kt[dom][0] / kt[dom][1]
     dom_bytes = kind_bytes.get(dom) or W.stage_bytes(wl.blocks)
     precision_tag = "" if precision == "fp64" else "_" + precision
+    # algorithmic FLOP rate of the dominant kernel (halo recomputation not counted)
+    tflops = kind_pts.get(dom, wl.npts) * FLOP_PER_POINT_STAGE / (dom_ms * 1e-3) / 1e12
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, wl.key + precision_tag),
             "kernel": KERNEL_NAMES.get(dom, dom), "launch_ms": round(dom_ms, 5),
             "algorithmic_bytes_per_launch": dom_bytes,
             "algorithmic_bytes_per_point": round(W.stage_bytes(wl.blocks) / wl.npts, 1),
+            "vector_alu": {"flop_per_point": FLOP_PER_POINT_STAGE, "achieved_tflops": round(tflops, 2),
+                           "peak_tflops": VALU_PEAK_TFLOPS[precision],
+                           "frac": round(tflops / VALU_PEAK_TFLOPS[precision], 4)},
             "stage_ms": round(stage_ms, 4),
             "stage_GBs": round(W.stage_bytes(wl.blocks) / (stage_ms * 1e-3) / 1e9, 1),
             "per_kernel": per_kernel}
