@@ -53,6 +53,7 @@ FLOP_PER_POINT_STAGE = 6788 + 96
 KERNEL_NAMES = {"fused_wide": "k_fused<interior, 256 threads, 7x7x5 tiles>",
                 "fused_thin": "k_fused<interior, 128 threads, 7x7x2 tiles>",
                 "fused_face": "k_fused<flagged faces, 256 threads>",
+                "fused_full": "k_fused<interior, 384 threads, 7x7x7 tiles, FP32>",
                 "derivs": "k_derivs", "rhs": "k_rhs", "face": "k_face"}
 
 
@@ -151,7 +152,7 @@ def cpu_baseline(wl, nblocks_uniform, threads, reps=3):
                       f"OpenMP over blocks, median of {reps} after 1 warm-up"}
 
 
-def mixed_comparison(wl, state0, params, steps, fp64_value, stream):
+def mixed_comparison(wl, state0, params, steps, fp64_value, stream, autotune=True):
     """The batched mixed-precision path on the same workload: RK4 Mpoints/s
     (graph replay), and its error against the FP64 path from identical inputs:
     the right-hand side (max over variables of the normwise relative error
@@ -166,6 +167,9 @@ def mixed_comparison(wl, state0, params, steps, fp64_value, stream):
         plan = dgr.Plan(wl.blocks, wl.n_pad)
         plan.set_precision(prec)
         out = torch.empty_like(state0)
+        if prec == "mixed" and autotune:
+            plan.autotune(out, state0, params, stream=stream)
+            tiling = {"x".join(str(n) for n in k): v for k, v in plan.tiling(prec).items()}
         plan.rhs(out, state0, params, stream=stream)
         u = state0.clone()
         plan.rk4_step(u, wl.dt, params, stream=stream)
@@ -188,6 +192,7 @@ def mixed_comparison(wl, state0, params, steps, fp64_value, stream):
     e_var = ((u32 - u64).abs().amax(dim=1) / u64.abs().amax(dim=1).clamp_min(1e-300)).max().item()
     e_scale = ((u32 - u64).abs().max() / u64.abs().max()).item()
     return {"value": round(value, 2), "unit": "Mpoints/s", "speedup_vs_fp64": round(value / fp64_value, 3),
+            "tiling": tiling if autotune else "default",
             "dtype": "f32 arithmetic (alpha, chi staged and differentiated in f64), f64 state and RK",
             "error_vs_fp64": {"state_after_one_step_vs_state_scale": e_scale,
                               "state_after_one_step_max_per_variable": e_var,
@@ -207,8 +212,8 @@ def interior_mask(blocks, npad):
 
 def roofline(wl, kt, stages, precision="fp64"):
     from dgr import workloads as W
-    kind_pts = W.fused_kind_points(wl.blocks)
-    kind_bytes = W.fused_kind_bytes(wl.blocks)
+    kind_pts = W.fused_kind_points(wl.blocks, precision)
+    kind_bytes = W.fused_kind_bytes(wl.blocks, precision)
     stage_ms = sum(v[0] for v in kt.values()) / stages
     per_kernel = {}
     for k, (ms, nl) in kt.items():
@@ -253,6 +258,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision comparison of the FP64 run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tiling instead of timing the tilings per block size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (3 times), to scale "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
@@ -285,6 +292,14 @@ def main():
     plan.set_precision(args.precision)
     params = dgr.default_params(**wl.params)
     stream = torch.cuda.current_stream()
+    if not args.no_autotune:
+        # tile shapes per block size, timed on this workload's own data
+        scratch = torch.empty_like(state)
+        plan.autotune(scratch, state, params, stream=stream)
+        torch.cuda.synchronize()
+        del scratch
+    tiling = {"x".join(str(n) for n in k): v for k, v in plan.tiling(args.precision).items()}
+    log(f"tiling per block size ({'autotuned' if not args.no_autotune else 'default'}): {tiling}")
 
     # the RK4 step is captured once as a HIP graph and replayed (dgr_rk4_steps)
     plan.rk4_steps(state, wl.dt, params, nsteps=max(args.warmup, 1), stream=stream)
@@ -327,7 +342,7 @@ def main():
 
     mixed = None
     if state0 is not None:
-        mixed = mixed_comparison(wl, state0, params, args.steps, value, stream)
+        mixed = mixed_comparison(wl, state0, params, args.steps, value, stream, not args.no_autotune)
     del state
 
     cpu = None
@@ -348,6 +363,7 @@ def main():
             "data": "synthetic",
             "config": {"workload": wl.desc, "blocks": len(wl.blocks), "interior_points": wl.npts,
                        "precision": args.precision, "rhs_evals_per_step": 4, "rk4_step": "hipGraph replay",
+                       "tiling": tiling, "autotuned": not args.no_autotune,
                        "dt": wl.dt, **wl.params},
             "rhs_only_mpoints_s": round(rhs_only, 2),
             "roofline": roof, "cpu_baseline": cpu, "mixed_precision": mixed}

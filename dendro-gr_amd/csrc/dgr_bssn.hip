@@ -523,6 +523,12 @@ struct FCfg {
 };
 using CfgWide = FCfg<256, 5, 3>;
 using CfgThin = FCfg<128, 2, 2>;
+// FP32 kernel on interior blocks: a whole 7 x 7 x 7 column per workgroup
+// (384 threads, ~98 KB of LDS, one workgroup per CU). Its registers allow two
+// waves per SIMD, so six waves fit where the FP64 kernel fits four: a 13^3
+// block's interior is one tile, and the staging and D_x / D_y work of the
+// z halo is shared by 7 planes instead of 5 + 2.
+using CfgFull = FCfg<384, 7, 1>;
 // The FP32 kernels fit two waves per SIMD in 256 registers only with one
 // variable in flight.
 template <typename T>
@@ -542,8 +548,19 @@ static_assert(2 * CfgThin::LDS <= 160 * 1024, "two thin FP64 workgroups per CU")
 static_assert(2 * (sizeof(double) * (CfgWide::NREG + (CfgWide::NREG + 1) / 2 + CfgWide::NSDX + CfgWide::NSDY) +
                    sizeof(float) * NV * CfgWide::FT) <= 160 * 1024,
               "two FP32 workgroups per CU");
-// tile groups of the fused path, one launch each
-enum { FG_WIDE = 0, FG_THIN = 1, FG_FACE = 2, NFGROUP = 3 };
+static_assert(7 * 7 * CfgFull::FZ <= CfgFull::FT, "one point per thread");
+static_assert(sizeof(double) * (CfgFull::NREG + (CfgFull::NREG + 1) / 2 + CfgFull::NSDX + CfgFull::NSDY) +
+                      sizeof(float) * NV * CfgFull::FT <= 160 * 1024,
+              "one FP32 7x7x7 workgroup per CU");
+// tile groups of the fused path, one launch each (FULL: FP32 only)
+enum { FG_WIDE = 0, FG_THIN = 1, FG_FACE = 2, FG_FULL = 3, NFGROUP = 4 };
+// Tilings of a block without flagged faces, chosen per block size and
+// precision (dgr_plan_autotune measures them). x and y are always cut into
+// near-equal parts of at most 7 points; along z:
+//   SCH_WT   parts of 5, a remainder of 1-2 points in thin tiles (128 threads)
+//   SCH_W    near-equal parts of at most 5, all regular tiles
+//   SCH_FULL near-equal parts of at most 7 (384 threads; FP32 only)
+enum { SCH_WT = 0, SCH_W = 1, SCH_FULL = 2, NSCHEME = 3 };
 
 // Where a stage reads its state and writes its results. Stage buffers hold
 // interior points only, indexed by DBlock::q0 + (i-3) + mx*((j-3) + my*(k-3));
@@ -1193,12 +1210,17 @@ struct dgr_plan {
     double *d_stage = nullptr;  // split path RK4: 2 * 24 * szpdof (stage state, accumulator)
     bool fused = true;          // k_fused (default) or the split k_derivs/k_rhs path (DGR_KERNEL=split)
     int precision = DGR_PRECISION_FP64;  // arithmetic of the fused kernel
-    Tile *d_ftiles = nullptr;   // k_fused tiles, XCD-grouped, group k in [fgroup[k], fgroup[k+1])
+    // k_fused tiles per precision (0 FP64, 1 mixed), XCD-grouped, group k in
+    // [fgroup[p][k], fgroup[p][k+1]), each interior block tiled by the scheme
+    // tiling[p] holds for its size (else the default of tiling_of)
+    Tile *d_ftiles[2] = {};
+    int fgroup[2][NFGROUP + 1] = {};
+    std::map<std::array<int, 3>, int> tiling[2];
+    std::vector<DBlock> hblocks;                  // host copy of the block table
+    std::map<std::array<int, 4>, int> lane_off;   // lane table offset per (ex, ey, ez, threads)
     dgr::AxisConsts *d_bconst = nullptr;  // k_fused: x, y, z stencil constants per block
     uint16_t *d_tabs = nullptr;           // k_fused: lane tables per tile shape, slot tables per config
-    int dtab_wide = 0, dtab_thin = 0;     // offsets of the two configurations' slot tables
-    int nftiles = 0;
-    int fgroup[NFGROUP + 1] = {};
+    int dtab_wide = 0, dtab_thin = 0, dtab_full = 0;  // offsets of the configurations' slot tables
     double *d_istage = nullptr; // k_fused RK4: 3 * 24 * interior (two stage states, accumulator)
     int *d_err = nullptr;
     // dgr_rk4_steps: one RK4 step captured as a graph, valid for the arguments in rk4_key
@@ -1303,6 +1325,130 @@ bool dtable(std::vector<uint16_t> &out) {
     return ok;
 }
 
+// m interior points -> ceil(m/emax) near-equal parts of at most emax points
+void split_axis_max(int m, int emax, std::vector<std::pair<int, int>> &parts) {
+    const int np = (m + emax - 1) / emax;
+    int o = PW;
+    for (int p = 0; p < np; p++) {
+        const int e = m / np + (p < m % np ? 1 : 0);
+        parts.push_back({o, e});
+        o += e;
+    }
+}
+
+int thread_count(int group) {
+    return group == FG_THIN ? CfgThin::FT : group == FG_FULL ? CfgFull::FT : CfgWide::FT;
+}
+
+// Appends the tiles of block b under scheme `sch` to per[group][slot]. Blocks
+// with flagged faces are cut along z into near-equal parts of at most 5 (each
+// >= 4 points, so the one-sided closures, which reach 5 points along their
+// axis, stay inside the staged region), whatever the scheme.
+void block_tiles(const DBlock &d, uint32_t b, int sch, std::vector<std::vector<Tile>> *per, size_t slot) {
+    std::vector<std::pair<int, int>> px, py, pz;
+    split_axis_max(d.n[0] - 6, TMAX, px);
+    split_axis_max(d.n[1] - 6, TMAX, py);
+    if (d.bflag || sch == SCH_W) {
+        split_axis_max(d.n[2] - 6, CfgWide::FZ, pz);
+    } else if (sch == SCH_FULL) {
+        split_axis_max(d.n[2] - 6, CfgFull::FZ, pz);
+    } else {
+        for (int o = PW, m = d.n[2] - 6; m > 0;) {
+            const int e = std::min(m, CfgWide::FZ);
+            pz.push_back({o, e});
+            o += e;
+            m -= e;
+        }
+    }
+    for (auto &z : pz)
+        for (auto &y : py)
+            for (auto &x : px) {
+                Tile t{};
+                t.block = (int)b;
+                t.q0 = 0;
+                t.o[0] = x.first; t.o[1] = y.first; t.o[2] = z.first;
+                t.e[0] = x.second; t.e[1] = y.second; t.e[2] = z.second;
+                const int g = d.bflag                                    ? FG_FACE
+                              : sch == SCH_FULL                          ? FG_FULL
+                              : sch == SCH_WT && t.e[2] <= CfgThin::FZ ? FG_THIN
+                                                                         : FG_WIDE;
+                per[g][slot].push_back(t);
+            }
+}
+
+// Concatenates the groups; within a group the blocks (slots) are taken eight
+// at a time and their tiles interleaved, so that with workgroups dealt
+// round-robin over the 8 XCDs the tiles of one block share an XCD and its L2
+// for their overlapping halos.
+void group_tiles(std::vector<std::vector<Tile>> *per, std::vector<Tile> &out, int *fgroup) {
+    for (int g = 0; g < NFGROUP; g++) {
+        fgroup[g] = (int)out.size();
+        std::vector<size_t> bl;
+        for (size_t b = 0; b < per[g].size(); b++)
+            if (!per[g][b].empty()) bl.push_back(b);
+        for (size_t i0 = 0; i0 < bl.size(); i0 += 8) {
+            const size_t i1 = std::min(bl.size(), i0 + 8);
+            size_t mx = 0;
+            for (size_t i = i0; i < i1; i++) mx = std::max(mx, per[g][bl[i]].size());
+            for (size_t t = 0; t < mx; t++)
+                for (size_t i = i0; i < i1; i++)
+                    if (t < per[g][bl[i]].size()) out.push_back(per[g][bl[i]][t]);
+        }
+    }
+    fgroup[NFGROUP] = (int)out.size();
+}
+
+// Tiling of an interior block for precision p (0 FP64, 1 mixed): the tuned
+// scheme of its size, else SCH_WT in FP64 and, in FP32, SCH_FULL for blocks
+// one 7-point tile deep (13^3) and SCH_WT for the others (measured defaults).
+int tiling_of(const dgr_plan *plan, int p, const DBlock &d) {
+    auto it = plan->tiling[p].find({d.n[0], d.n[1], d.n[2]});
+    if (it != plan->tiling[p].end()) return it->second;
+    if (p == 1 && d.n[2] - 6 <= CfgFull::FZ) return SCH_FULL;
+    return SCH_WT;
+}
+
+// Sets lanetab of every tile from the plan's lane tables (all shapes exist).
+bool assign_lane_tables(const dgr_plan *plan, std::vector<Tile> &tiles, const int *fgroup) {
+    for (int g = 0; g < NFGROUP; g++)
+        for (int i = fgroup[g]; i < fgroup[g + 1]; i++) {
+            Tile &t = tiles[i];
+            auto it = plan->lane_off.find({t.e[0], t.e[1], t.e[2], thread_count(g)});
+            if (it == plan->lane_off.end()) return false;
+            t.lanetab = it->second;
+        }
+    return true;
+}
+
+// (Re)builds and uploads the tile list of precision p.
+hipError_t build_ftiles(dgr_plan *plan, int p) {
+    const size_t nb = plan->hblocks.size();
+    std::vector<std::vector<Tile>> per[NFGROUP];
+    for (auto &v : per) v.resize(nb);
+    for (size_t b = 0; b < nb; b++) {
+        const DBlock &d = plan->hblocks[b];
+        block_tiles(d, (uint32_t)b, d.bflag ? SCH_WT : tiling_of(plan, p, d), per, b);
+    }
+    std::vector<Tile> tiles;
+    int fg[NFGROUP + 1];
+    group_tiles(per, tiles, fg);
+    if (!assign_lane_tables(plan, tiles, fg)) return hipErrorInvalidValue;
+    Tile *d = nullptr;
+    hipError_t e = hipSuccess;
+    if (!tiles.empty()) {
+        e = hipMalloc(&d, sizeof(Tile) * tiles.size());
+        if (e == hipSuccess) e = hipMemcpy(d, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            return e;
+        }
+    }
+    if (plan->d_ftiles[p]) (void)hipFree(plan->d_ftiles[p]);
+    plan->d_ftiles[p] = d;
+    std::copy(fg, fg + NFGROUP + 1, plan->fgroup[p]);
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1324,16 +1470,6 @@ void dgr_default_params(dgr_bssn_params *p) {
 
 const char *dgr_last_error(void) { return g_err.c_str(); }
 
-// m interior points -> ceil(m/emax) near-equal parts of at most emax points
-static void split_axis_max(int m, int emax, std::vector<std::pair<int, int>> &parts) {
-    const int np = (m + emax - 1) / emax;
-    int o = PW;
-    for (int p = 0; p < np; p++) {
-        const int e = m / np + (p < m % np ? 1 : 0);
-        parts.push_back({o, e});
-        o += e;
-    }
-}
 
 static void split_axis(int m, std::vector<std::pair<int, int>> &parts) {
     // m interior points -> ceil(m/7) near-equal parts (each >= 4 when m >= 7)
@@ -1434,88 +1570,40 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
     close_chunk();
     plan->nwg_total = (int)wg_tile.size();
 
-    // k_fused tiles: x and y cut into near-equal parts of at most 7 points.
-    // Along z, blocks without flagged faces are cut into parts of 5 with the
-    // remainder last; a remainder of 1 or 2 points makes a thin tile
-    // (128-thread launch). Blocks with flagged faces are cut into near-equal
-    // parts of at most 5 (each >= 4 points, so the one-sided closures, which
-    // reach 5 points along their axis, stay inside the staged region). Tiles
-    // are grouped by kind (interior-wide, interior-thin, face-wide); within a
-    // group, blocks are taken eight at a time and their tiles interleaved, so
-    // that (with workgroups dealt round-robin over the 8 XCDs) the tiles of
-    // one block share an XCD and its L2 for their overlapping halos.
-    std::vector<Tile> ftiles;
-    {
-        std::vector<std::vector<Tile>> per[NFGROUP];
-        for (auto &v : per) v.resize(nblocks);
-        for (uint32_t b = 0; b < nblocks; b++) {
-            const DBlock &d = hb[b];
-            std::vector<std::pair<int, int>> px, py, pz;
-            split_axis_max(d.n[0] - 6, TMAX, px);
-            split_axis_max(d.n[1] - 6, TMAX, py);
-            if (d.bflag) {
-                split_axis_max(d.n[2] - 6, CfgWide::FZ, pz);
-            } else {
-                for (int o = PW, m = d.n[2] - 6; m > 0;) {
-                    const int e = std::min(m, CfgWide::FZ);
-                    pz.push_back({o, e});
-                    o += e;
-                    m -= e;
-                }
-            }
-            for (auto &z : pz)
-                for (auto &y : py)
-                    for (auto &x : px) {
-                        Tile t{};
-                        t.block = (int)b;
-                        t.q0 = 0;
-                        t.o[0] = x.first; t.o[1] = y.first; t.o[2] = z.first;
-                        t.e[0] = x.second; t.e[1] = y.second; t.e[2] = z.second;
-                        const int kind = d.bflag ? FG_FACE : t.e[2] <= CfgThin::FZ ? FG_THIN : FG_WIDE;
-                        per[kind][b].push_back(t);
-                    }
-        }
-        for (int kind = 0; kind < NFGROUP; kind++) {
-            plan->fgroup[kind] = (int)ftiles.size();
-            std::vector<uint32_t> bl;
-            for (uint32_t b = 0; b < nblocks; b++)
-                if (!per[kind][b].empty()) bl.push_back(b);
-            for (size_t g = 0; g < bl.size(); g += 8) {
-                const size_t ge = std::min(bl.size(), g + 8);
-                size_t mx = 0;
-                for (size_t i = g; i < ge; i++) mx = std::max(mx, per[kind][bl[i]].size());
-                for (size_t t = 0; t < mx; t++)
-                    for (size_t i = g; i < ge; i++)
-                        if (t < per[kind][bl[i]].size()) ftiles.push_back(per[kind][bl[i]][t]);
-            }
-        }
-        plan->fgroup[NFGROUP] = (int)ftiles.size();
-    }
-    plan->nftiles = (int)ftiles.size();
-    // lane tables per tile shape, slot tables per configuration
+    // k_fused: slot tables per configuration, lane tables for every tile
+    // shape any tiling can produce (one block per size and kind suffices);
+    // the tile lists themselves are built per precision after the uploads
+    plan->hblocks = hb;
     std::vector<uint16_t> tabs;
     plan->dtab_wide = (int)tabs.size();
     bool tabs_ok = dtable<CfgWide>(tabs);
     plan->dtab_thin = (int)tabs.size();
     tabs_ok = dtable<CfgThin>(tabs) && tabs_ok;
+    plan->dtab_full = (int)tabs.size();
+    tabs_ok = dtable<CfgFull>(tabs) && tabs_ok;
     if (!tabs_ok) {
         dgr_plan_destroy(plan);
         return fail(DGR_E_INVALID, "internal error: D_x / D_y slot table does not match its configuration");
     }
     {
-        std::map<std::array<int, 4>, int> shape_off;
-        for (int g = 0; g < NFGROUP; g++)
-            for (int i = plan->fgroup[g]; i < plan->fgroup[g + 1]; i++) {
-                Tile &t = ftiles[i];
-                const int ft = g == FG_THIN ? CfgThin::FT : CfgWide::FT;
-                const std::array<int, 4> key{t.e[0], t.e[1], t.e[2], ft};
-                auto it = shape_off.find(key);
-                if (it == shape_off.end()) {
-                    it = shape_off.emplace(key, (int)tabs.size()).first;
-                    lane_table(t.e[0], t.e[1], t.e[2], ft, tabs);
-                }
-                t.lanetab = it->second;
+        std::map<std::array<int, 4>, bool> seen;
+        for (uint32_t b = 0; b < nblocks; b++) {
+            const DBlock &d = hb[b];
+            if (!seen.emplace(std::array<int, 4>{d.n[0], d.n[1], d.n[2], d.bflag ? 1 : 0}, true).second)
+                continue;
+            for (int sch = 0; sch < (d.bflag ? 1 : NSCHEME); sch++) {
+                std::vector<std::vector<Tile>> per[NFGROUP];
+                for (auto &v : per) v.resize(1);
+                block_tiles(d, b, sch, per, 0);
+                for (int g = 0; g < NFGROUP; g++)
+                    for (const Tile &t : per[g][0]) {
+                        const std::array<int, 4> key{t.e[0], t.e[1], t.e[2], thread_count(g)};
+                        if (plan->lane_off.count(key)) continue;
+                        plan->lane_off[key] = (int)tabs.size();
+                        lane_table(t.e[0], t.e[1], t.e[2], thread_count(g), tabs);
+                    }
             }
+        }
     }
     std::vector<dgr::AxisConsts> bconst(3 * (size_t)nblocks);
     for (uint32_t b = 0; b < nblocks; b++)
@@ -1527,7 +1615,6 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
     if (e == hipSuccess && !tiles.empty()) e = hipMalloc(&plan->d_tiles, sizeof(Tile) * tiles.size());
     if (e == hipSuccess && !wg_tile.empty()) e = hipMalloc(&plan->d_wg_tile, sizeof(int) * wg_tile.size());
     if (e == hipSuccess && !face_tiles.empty()) e = hipMalloc(&plan->d_face_tiles, sizeof(int) * face_tiles.size());
-    if (e == hipSuccess && !ftiles.empty()) e = hipMalloc(&plan->d_ftiles, sizeof(Tile) * ftiles.size());
     if (e == hipSuccess) e = hipMalloc(&plan->d_tabs, sizeof(uint16_t) * tabs.size());
     if (e == hipSuccess) e = hipMemcpy(plan->d_tabs, tabs.data(), sizeof(uint16_t) * tabs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && nblocks) e = hipMalloc(&plan->d_bconst, sizeof(dgr::AxisConsts) * bconst.size());
@@ -1539,8 +1626,7 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
     if (e == hipSuccess && !wg_tile.empty()) e = hipMemcpy(plan->d_wg_tile, wg_tile.data(), sizeof(int) * wg_tile.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !face_tiles.empty())
         e = hipMemcpy(plan->d_face_tiles, face_tiles.data(), sizeof(int) * face_tiles.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !ftiles.empty())
-        e = hipMemcpy(plan->d_ftiles, ftiles.data(), sizeof(Tile) * ftiles.size(), hipMemcpyHostToDevice);
+    for (int p = 0; p < 2 && e == hipSuccess; p++) e = build_ftiles(plan, p);
     if (e == hipSuccess) e = hipMemset(plan->d_err, 0, sizeof(int));
     if (e != hipSuccess) {
         dgr_plan_destroy(plan);
@@ -1565,7 +1651,8 @@ int dgr_plan_destroy(dgr_plan *plan) {
     hipFree(plan->d_face_tiles);
     hipFree(plan->d_W);
     hipFree(plan->d_stage);
-    hipFree(plan->d_ftiles);
+    hipFree(plan->d_ftiles[0]);
+    hipFree(plan->d_ftiles[1]);
     hipFree(plan->d_bconst);
     hipFree(plan->d_tabs);
     hipFree(plan->d_istage);
@@ -1692,26 +1779,32 @@ int run_chunks(dgr_plan *plan, const VarPtrs &state, const VarPtrs &rhs, const R
 }
 
 template <bool FACE, class C, typename T>
-void launch_fused(dgr_plan *plan, int group, const StageIO &io, const RhsParams &prm, hipStream_t st) {
-    const int n = plan->fgroup[group + 1] - plan->fgroup[group];
+void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, int dtab, const StageIO &io,
+                  const RhsParams &prm, hipStream_t st) {
+    const int n = fg[group + 1] - fg[group];
     if (!n) return;
     timed(plan, 3 + group, st, [&] {
         hipLaunchKernelGGL((k_fused<FACE, C, T>), dim3(n), dim3(C::FT), 0, st, io, prm, plan->d_blocks,
-                           plan->d_bconst, plan->d_ftiles + plan->fgroup[group], plan->d_tabs,
-                           C::FT == CfgThin::FT && C::FZ == CfgThin::FZ ? plan->dtab_thin : plan->dtab_wide);
+                           plan->d_bconst, tiles + fg[group], plan->d_tabs, dtab);
     });
 }
 
+// The launches of one tile list (groups as in fg).
 template <typename T>
-void launch_fused_all(dgr_plan *plan, const StageIO &io, const RhsParams &prm, hipStream_t st) {
-    launch_fused<false, typename CfgOf<T>::Wide, T>(plan, FG_WIDE, io, prm, st);
-    launch_fused<false, typename CfgOf<T>::Thin, T>(plan, FG_THIN, io, prm, st);
-    launch_fused<true, typename CfgOf<T>::Wide, T>(plan, FG_FACE, io, prm, st);
+void launch_fused_all(dgr_plan *plan, const Tile *tiles, const int *fg, const StageIO &io, const RhsParams &prm,
+                      hipStream_t st) {
+    launch_fused<false, typename CfgOf<T>::Wide, T>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
+    launch_fused<false, typename CfgOf<T>::Thin, T>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
+    if constexpr (sizeof(T) < sizeof(double))
+        launch_fused<false, CfgFull, T>(plan, tiles, fg, FG_FULL, plan->dtab_full, io, prm, st);
+    launch_fused<true, typename CfgOf<T>::Wide, T>(plan, tiles, fg, FG_FACE, plan->dtab_wide, io, prm, st);
 }
 
 int run_fused(dgr_plan *plan, const StageIO &io, const RhsParams &prm, hipStream_t st) {
-    if (plan->precision == DGR_PRECISION_MIXED) launch_fused_all<float>(plan, io, prm, st);
-    else launch_fused_all<double>(plan, io, prm, st);
+    if (plan->precision == DGR_PRECISION_MIXED)
+        launch_fused_all<float>(plan, plan->d_ftiles[1], plan->fgroup[1], io, prm, st);
+    else
+        launch_fused_all<double>(plan, plan->d_ftiles[0], plan->fgroup[0], io, prm, st);
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }
@@ -1739,6 +1832,130 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
     }
     RKArgs rk{};
     return run_chunks<MODE_RHS>(plan, u, r, rk, to_params(params), (hipStream_t)stream);
+}
+
+int dgr_plan_autotune(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
+                      const dgr_bssn_params *params, void *stream) {
+    g_err.clear();
+    if (!plan || !params) return fail(DGR_E_INVALID, "null plan or params");
+    if (!plan->fused) return fail(DGR_E_INVALID, "autotuning needs the fused kernel (DGR_KERNEL=split is set)");
+    VarPtrs u, r;
+    int rc;
+    if ((rc = load_ptrs(u, d_u, "d_u")) || (rc = load_ptrs(r, d_rhs, "d_rhs"))) return rc;
+    HIP_OK(hipSetDevice(plan->device));
+    const hipStream_t st = (hipStream_t)stream;
+    const int p = plan->precision == DGR_PRECISION_MIXED ? 1 : 0;
+    StageIO io{};
+    io.u = u;
+    io.rhs = r;
+    io.stage = -1;
+    io.err = plan->d_err;
+    const RhsParams prm = to_params(params);
+    std::map<std::array<int, 3>, std::vector<uint32_t>> sizes;  // blocks without flagged faces by size
+    for (uint32_t b = 0; b < (uint32_t)plan->hblocks.size(); b++) {
+        const DBlock &d = plan->hblocks[b];
+        if (!d.bflag) sizes[{d.n[0], d.n[1], d.n[2]}].push_back(b);
+    }
+    const bool timing = plan->timing;
+    plan->timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Tile *d_t = nullptr;
+    size_t cap = 0;
+    auto tune = [&]() -> int {
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        for (auto &kv : sizes) {
+            // a sample of at most 2048 blocks of this size
+            const size_t ns = std::min<size_t>(kv.second.size(), 2048);
+            int best = -1;
+            float best_ms = 0.f;
+            for (int sch = 0; sch < NSCHEME; sch++) {
+                if (sch == SCH_FULL && p == 0) continue;
+                std::vector<std::vector<Tile>> per[NFGROUP];
+                for (auto &v : per) v.resize(ns);
+                for (size_t i = 0; i < ns; i++) block_tiles(plan->hblocks[kv.second[i]], kv.second[i], sch, per, i);
+                std::vector<Tile> tiles;
+                int fg[NFGROUP + 1];
+                group_tiles(per, tiles, fg);
+                if (!assign_lane_tables(plan, tiles, fg)) return fail(DGR_E_INVALID, "internal error: no lane table");
+                HIP_OK(hipStreamSynchronize(st));  // earlier candidates' launches read d_t
+                if (tiles.size() > cap) {
+                    HIP_OK(hipFree(d_t));
+                    d_t = nullptr;
+                    HIP_OK(hipMalloc(&d_t, sizeof(Tile) * tiles.size()));
+                    cap = tiles.size();
+                }
+                HIP_OK(hipMemcpy(d_t, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+                float ms = 0.f;
+                for (int rep = 0; rep < 4; rep++) {  // a warm-up, then the best of three
+                    HIP_OK(hipEventRecord(e0, st));
+                    if (p) launch_fused_all<float>(plan, d_t, fg, io, prm, st);
+                    else launch_fused_all<double>(plan, d_t, fg, io, prm, st);
+                    HIP_OK(hipGetLastError());
+                    HIP_OK(hipEventRecord(e1, st));
+                    HIP_OK(hipEventSynchronize(e1));
+                    float t = 0.f;
+                    HIP_OK(hipEventElapsedTime(&t, e0, e1));
+                    if (rep == 1 || (rep > 1 && t < ms)) ms = t;
+                }
+                if (best < 0 || ms < best_ms) {
+                    best = sch;
+                    best_ms = ms;
+                }
+            }
+            plan->tiling[p][kv.first] = best;
+        }
+        HIP_OK(build_ftiles(plan, p));
+        if (plan->rk4_exec) {  // the captured step used the previous tile list
+            HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
+            plan->rk4_exec = nullptr;
+        }
+        return DGR_OK;
+    };
+    rc = tune();
+    plan->timing = timing;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (d_t) (void)hipFree(d_t);
+    return rc;
+}
+
+int dgr_plan_set_tiling(dgr_plan *plan, int precision, uint32_t nx, uint32_t ny, uint32_t nz, int scheme) {
+    g_err.clear();
+    if (!plan) return fail(DGR_E_INVALID, "null plan");
+    if (precision != DGR_PRECISION_FP64 && precision != DGR_PRECISION_MIXED)
+        return fail(DGR_E_INVALID, "unknown precision");
+    if (scheme < 0 || scheme >= NSCHEME || (scheme == SCH_FULL && precision == DGR_PRECISION_FP64))
+        return fail(DGR_E_INVALID, "unknown tiling for this precision");
+    const int p = precision == DGR_PRECISION_MIXED ? 1 : 0;
+    plan->tiling[p][{(int)nx, (int)ny, (int)nz}] = scheme;
+    HIP_OK(hipSetDevice(plan->device));
+    HIP_OK(build_ftiles(plan, p));
+    if (plan->rk4_exec) {  // the captured step used the previous tile list
+        HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
+        plan->rk4_exec = nullptr;
+    }
+    return DGR_OK;
+}
+
+int dgr_plan_tiling(const dgr_plan *plan, int precision, int max_sizes, uint32_t *sizes, int *scheme) {
+    if (!plan) return fail(DGR_E_INVALID, "null plan");
+    if (precision != DGR_PRECISION_FP64 && precision != DGR_PRECISION_MIXED)
+        return fail(DGR_E_INVALID, "unknown precision");
+    if (max_sizes > 0 && (!sizes || !scheme)) return fail(DGR_E_INVALID, "null output");
+    const int p = precision == DGR_PRECISION_MIXED ? 1 : 0;
+    std::map<std::array<int, 3>, int> seen;
+    for (const DBlock &d : plan->hblocks)
+        if (!d.bflag) seen.emplace(std::array<int, 3>{d.n[0], d.n[1], d.n[2]}, tiling_of(plan, p, d));
+    int i = 0;
+    for (auto &kv : seen) {
+        if (i < max_sizes) {
+            for (int a = 0; a < 3; a++) sizes[3 * i + a] = (uint32_t)kv.first[a];
+            scheme[i] = kv.second;
+        }
+        i++;
+    }
+    return i;
 }
 
 }  // extern "C"

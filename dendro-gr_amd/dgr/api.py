@@ -22,8 +22,10 @@ SYMBOLS = [
     "dgr_plan_interior_points", "dgr_bssn_rhs", "dgr_rk4_step", "dgr_enforce_constraints",
     "dgr_plan_status", "dgr_plan_set_timing", "dgr_plan_kernel_times",
     "dgr_ham_constraint", "dgr_constraint_l2_norm", "dgr_rk4_steps", "dgr_plan_set_precision",
+    "dgr_plan_autotune", "dgr_plan_tiling", "dgr_plan_set_tiling",
 ]
 PRECISION = {"fp64": 0, "mixed": 1}
+TILINGS = ("wt", "w", "full")  # DGR_TILING_WT, _W, _FULL
 
 
 class BssnParams(C.Structure):
@@ -75,6 +77,9 @@ def load():
         L.dgr_plan_set_precision.argtypes = [vp, C.c_int]
         L.dgr_rk4_steps.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), C.c_int, C.c_int, vp]
         L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
+        L.dgr_plan_autotune.argtypes = [vp, vp, vp, C.POINTER(BssnParams), vp]
+        L.dgr_plan_tiling.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]
+        L.dgr_plan_set_tiling.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
         L.dgr_constraint_l2_norm.argtypes = [vp, vp, C.POINTER(C.c_double), C.c_double, C.c_double,
                                              C.POINTER(C.c_double), vp]
         _lib = L
@@ -174,10 +179,31 @@ class Plan:
         """'fp64' (default) or 'mixed' (FP32 derivatives/equations, FP64 state)."""
         _check(load().dgr_plan_set_precision(self._h, PRECISION[precision]), "dgr_plan_set_precision")
 
+    def autotune(self, out, state, params: BssnParams, stream=None):
+        """Times the tilings per block size for the current precision and keeps
+        the fastest (dgr_plan_autotune); `out` is overwritten."""
+        _check(load().dgr_plan_autotune(self._h, var_pointers(out), var_pointers(state), C.byref(params),
+                                        _stream_ptr(stream)), "dgr_plan_autotune")
+
+    def set_tiling(self, precision: str, size, tiling: str):
+        """Pins the tiling ('wt', 'w' or 'full') of blocks of `size` (nx, ny, nz)."""
+        _check(load().dgr_plan_set_tiling(self._h, PRECISION[precision], *[int(n) for n in size],
+                                          TILINGS.index(tiling)), "dgr_plan_set_tiling")
+
+    def tiling(self, precision: str = "fp64") -> dict:
+        """{(nx, ny, nz): 'wt' | 'w' | 'full'} for the blocks without flagged faces."""
+        L = load()
+        n = L.dgr_plan_tiling(self._h, PRECISION[precision], 0, None, None)
+        _check(min(n, 0), "dgr_plan_tiling")
+        sizes = (C.c_uint32 * (3 * max(n, 1)))()
+        sch = (C.c_int * max(n, 1))()
+        L.dgr_plan_tiling(self._h, PRECISION[precision], n, sizes, sch)
+        return {(sizes[3 * i], sizes[3 * i + 1], sizes[3 * i + 2]): TILINGS[sch[i]] for i in range(n)}
+
     def set_timing(self, enable: bool = True):
         _check(load().dgr_plan_set_timing(self._h, int(enable)), "dgr_plan_set_timing")
 
-    TIMING_KINDS = ("derivs", "rhs", "face", "fused_wide", "fused_thin", "fused_face")
+    TIMING_KINDS = ("derivs", "rhs", "face", "fused_wide", "fused_thin", "fused_face", "fused_full")
 
     def kernel_times(self):
         """{kind: (total ms, launches)} since the previous call."""

@@ -105,12 +105,12 @@ def stage_bytes(t: np.ndarray) -> float:
     return float((192.0 * n3 + 960.0 * m3).sum())
 
 
-def fused_kind_bytes(t: np.ndarray) -> dict:
+def fused_kind_bytes(t: np.ndarray, precision: str = "fp64") -> dict:
     """stage_bytes split over the fused launches in proportion to the interior
     points each launch covers within every block."""
-    out = {"fused_wide": 0.0, "fused_thin": 0.0, "fused_face": 0.0}
+    out = {k: 0.0 for k in fused_kind_points(t[:0], precision)}
     for b in range(len(t)):
-        share = fused_kind_points(t[b:b + 1])
+        share = fused_kind_points(t[b:b + 1], precision)
         m = sum(share.values())
         bb = stage_bytes(t[b:b + 1])
         for k, v in share.items():
@@ -118,17 +118,24 @@ def fused_kind_bytes(t: np.ndarray) -> dict:
     return out
 
 
-def fused_kind_points(t: np.ndarray) -> dict:
+def fused_kind_points(t: np.ndarray, precision: str = "fp64") -> dict:
     """Interior points each launch of the fused path covers, by tile kind, with
-    the plan's tiling rule: blocks with flagged faces -> 'fused_face'; other
-    blocks are cut along z into parts of 5 points, a remainder of 1-2 points
-    forming 'fused_thin' tiles and everything else 'fused_wide' tiles."""
-    pts = {"fused_wide": 0, "fused_thin": 0, "fused_face": 0}
+    the plan's tiling rule: blocks with flagged faces -> 'fused_face'; in FP64
+    other blocks are cut along z into parts of 5 points, a remainder of 1-2
+    points forming 'fused_thin' tiles and everything else 'fused_wide' tiles;
+    in mixed precision other blocks go to 'fused_full' (7 x 7 x 7 tiles)."""
+    if precision == "fp64":
+        pts = {"fused_wide": 0, "fused_thin": 0, "fused_face": 0}
+    else:
+        pts = {"fused_full": 0, "fused_face": 0}
     for b in range(len(t)):
         m = (t["sz"][b].astype(np.int64) - 2 * PW)
         xy = int(m[0] * m[1])
         if t["bflag"][b]:
             pts["fused_face"] += xy * int(m[2])
+            continue
+        if precision != "fp64":
+            pts["fused_full"] += xy * int(m[2])
             continue
         r = int(m[2]) % 5
         thin = r if r in (1, 2) else 0

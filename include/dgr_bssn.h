@@ -105,6 +105,44 @@ int dgr_plan_destroy(dgr_plan *plan);
 #define DGR_PRECISION_MIXED 1
 int dgr_plan_set_precision(dgr_plan *plan, int precision);
 
+/* Tile shapes of the fused kernel (no counterpart in the reference, whose
+ * kernels take fixed shapes). Every block is cut into columns of at most 7 x 7
+ * points in x and y; along z a block without flagged faces is cut by one of:
+ *   DGR_TILING_WT    parts of 5, a 1-2 point remainder in 128-thread tiles
+ *   DGR_TILING_W     near-equal parts of at most 5
+ *   DGR_TILING_FULL  near-equal parts of at most 7 (384-thread tiles; mixed
+ *                    precision only)
+ * Blocks with flagged faces always use near-equal parts of at most 5. Each
+ * precision keeps one tiling per block size; before tuning: WT in FP64; in
+ * mixed precision FULL for blocks with at most 7 interior points along z and
+ * WT for the others. */
+#define DGR_TILING_WT 0
+#define DGR_TILING_W 1
+#define DGR_TILING_FULL 2
+
+/* Tile autotuner: for the plan's current precision and every size of the
+ * blocks without flagged faces, times one right-hand-side evaluation of a
+ * sample of those blocks (at most 2048) under each applicable tiling (best of
+ * three after a warm-up, HIP events on `stream`) and keeps the fastest. The
+ * evaluations read d_u and overwrite d_rhs (both caller-owned, laid out as
+ * for dgr_bssn_rhs). Allocates temporary device memory, waits for `stream`,
+ * and drops a captured dgr_rk4_steps graph. Results do not depend on the
+ * tiling in FP64 (bitwise); in mixed precision they move within its error
+ * (the staging offset is per tile). */
+int dgr_plan_autotune(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
+                      const dgr_bssn_params *params, void *stream);
+
+/* Sets the tiling of blocks of size nx x ny x nz (without flagged faces) for
+ * `precision` and rebuilds that precision's tile list (DGR_TILING_FULL is for
+ * mixed precision only). Drops a captured dgr_rk4_steps graph. */
+int dgr_plan_set_tiling(dgr_plan *plan, int precision, uint32_t nx, uint32_t ny, uint32_t nz, int scheme);
+
+/* The tiling per block size for `precision`: returns the number of distinct
+ * sizes of blocks without flagged faces and writes the first max_sizes of
+ * them (sizes[3i..3i+2] = nx, ny, nz; scheme[i] = DGR_TILING_*), in
+ * increasing (nx, ny, nz) order. */
+int dgr_plan_tiling(const dgr_plan *plan, int precision, int max_sizes, uint32_t *sizes, int *scheme);
+
 /* Interior points (sum over blocks of (nx-6)(ny-6)(nz-6)) the plan covers. */
 int64_t dgr_plan_interior_points(const dgr_plan *plan);
 
@@ -116,8 +154,9 @@ int64_t dgr_plan_interior_points(const dgr_plan *plan);
  * `ms` and `launches` have DGR_TIMING_KINDS entries: 0 derivatives,
  * 1 equations, 2 boundary faces (the split path); 3 fused kernel on regular
  * tiles of blocks without flagged faces, 4 on thin tiles, 5 on tiles of
- * blocks with flagged faces. */
-#define DGR_TIMING_KINDS 6
+ * blocks with flagged faces, 6 the mixed-precision kernel on 7 x 7 x 7 tiles
+ * of blocks without flagged faces (it replaces 3 and 4 in that precision). */
+#define DGR_TIMING_KINDS 7
 int dgr_plan_set_timing(dgr_plan *plan, int enable);
 int dgr_plan_kernel_times(dgr_plan *plan, double *ms, int64_t *launches);
 
