@@ -565,10 +565,18 @@ enum { SCH_WT = 0, SCH_W = 1, SCH_FULL = 2, NSCHEME = 3 };
 // Where a stage reads its state and writes its results. Stage buffers hold
 // interior points only, indexed by DBlock::q0 + (i-3) + mx*((j-3) + my*(k-3));
 // padding always comes from the caller's state `u`.
+// Buffers of one fused launch. An RK4 step alternates two padded states: the
+// caller's u and the plan's work state W, whose padding holds a copy of u's
+// (k_pad_copy, once per dgr_rk4_step / dgr_rk4_steps call). Stage 0 reads u
+// and writes the next stage state into W's interior; stage 1 reads W and
+// writes u's interior; stage 2 reads u and writes W; stage 3 reads W and
+// writes the result into u. A stage therefore reads its whole region from
+// one array, and u's padding is never written. u0 (needed by stages 1 and 2
+// once u's interior is reused) and the accumulator are interior-only.
 struct StageIO {
-    VarPtrs u;     // caller's padded state (padding; interior at stage 0 and in RHS mode)
-    VarPtrs sin;   // interior-only stage input (stages 1..3)
-    VarPtrs sout;  // interior-only next-stage state (stages 0..2)
+    VarPtrs src;   // padded stage input (u or W; u in RHS mode)
+    VarPtrs dst;   // padded next-stage state, interior points (W or u)
+    VarPtrs u0;    // interior-only copy of the step's initial state
     VarPtrs acc;   // interior-only accumulator
     VarPtrs rhs;   // RHS mode: padded output
     double cacc, cnext;
@@ -593,18 +601,14 @@ struct FusedCtx {
     int ox, oy, oz, iex, iey, iez;  // tile origin, ie = n - 3 per axis
     bool lox, hix, loy, hiy, loz, hiz;
     uint64_t boff;
-    int64_t bq0;
     uint32_t corner;                // padded offset (within the block) of the region's first point
-    int32_t corner_q;               // its index in the interior-only stage buffers, -1 for padding
     double pmin[3], h[3];
     // this thread
     int tid, ax, ay, az, cofs, dxo, dyo;
     bool face;
     T sig_gauge, sig_other;
-    // region elements this thread fetches: offset in the padded block, and in
-    // the interior-only stage buffer (-1: a padding point)
+    // region elements this thread fetches: offset in the padded block
     uint32_t goff[PREF];
-    int32_t qoff[PREF];
     double pre[PD][PREF];  // fetched region of the variables of phases K+1 .. K+PD
     double poff[PD];       // the staging offset of each fetched variable (mixed precision)
     T f[NV];               // this point's field values
@@ -654,19 +658,10 @@ struct FusedCtx {
     __device__ __forceinline__ void fetch() {
         constexpr int v = dgr::kBssnPhaseVar[K];
         double *dst = pre[K % PD];
-        const double *__restrict__ ub = io.u.p[v] + boff;
-        if (io.stage >= 1) {
-            const double *__restrict__ sb = io.sin.p[v] + bq0;
-            // the offset comes from where the stage reads that point: the
-            // interior of u is being overwritten during the last stage
-            if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = corner_q >= 0 ? sb[corner_q] : ub[corner];
+        const double *__restrict__ sb = io.src.p[v] + boff;
+        if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = sb[corner];
 #pragma unroll
-            for (int c = 0; c < PREF; c++) dst[c] = qoff[c] >= 0 ? sb[qoff[c]] : ub[goff[c]];
-        } else {
-            if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = ub[corner];
-#pragma unroll
-            for (int c = 0; c < PREF; c++) dst[c] = ub[goff[c]];
-        }
+        for (int c = 0; c < PREF; c++) dst[c] = sb[goff[c]];
     }
 
     // The type variable V is staged and differentiated in. The mixed path
@@ -859,7 +854,9 @@ struct FusedOut {
 };
 
 // RHS store or RK4 stage update of one point (rkBSSN.cpp:764-860), with the
-// loads of all 24 variables issued before any store.
+// loads of all 24 variables issued before any store. Stage s (U = {0, 1/2,
+// 1/2, 1}, C = {1/6, 1/3, 1/3, 1/6}): acc = (s ? acc : u0) + C[s] dt k;
+// next = u0 + U[s+1] dt k (s < 3) or the final state acc (s = 3).
 // The RK arithmetic is FP64 in both precisions; the mixed path's FP32
 // right-hand side is widened first and its u0 read back in FP64.
 template <class C, typename T>
@@ -879,17 +876,18 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const T *s_ko,
         for (int v = 0; v < NV; v++) u0[v] = a0[v] = double(f[v]);
     } else if (s == 0) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) u0[v] = a0[v] = io.u.p[v][idx];
+        for (int v = 0; v < NV; v++) u0[v] = a0[v] = io.src.p[v][idx];
     } else {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            if (s < 3) u0[v] = io.u.p[v][idx];
+            if (s < 3) u0[v] = io.u0.p[v][q];
             a0[v] = io.acc.p[v][q];
         }
     }
     const bool enf = s == 3 || io.enforce_stage;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
+        if (s == 0) io.u0.p[v][q] = u0[v];
         const double k = double(s_ko[v * FT + tid]);
         const double acc = a0[v] + io.cacc * k;
         double out;
@@ -900,8 +898,7 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const T *s_ko,
             out = acc;
         }
         if (slot14(v) >= 0 && enf) v14[slot14(v)] = out;
-        else if (s < 3) io.sout.p[v][q] = out;
-        else io.u.p[v][idx] = out;
+        else io.dst.p[v][idx] = out;
     }
     if (!enf) return;
     const int e = enforce_point(v14, chi_floor);
@@ -909,8 +906,7 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const T *s_ko,
 #pragma unroll
     for (int c = 0; c < 14; c++) {
         const int v = c < 2 ? c : c + 10;
-        if (s < 3) io.sout.p[v][q] = v14[c];
-        else io.u.p[v][idx] = v14[c];
+        io.dst.p[v][idx] = v14[c];
     }
 }
 
@@ -937,13 +933,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     const uint32_t bf = B.bflag;
     ctx.lox = bf & 1u; ctx.hix = bf & 2u; ctx.loy = bf & 4u; ctx.hiy = bf & 8u; ctx.loz = bf & 16u; ctx.hiz = bf & 32u;
     ctx.boff = B.offset;
-    ctx.bq0 = B.q0;
     ctx.corner = (uint32_t)((ctx.ox - PW) + nx * ((ctx.oy - PW) + ny * (ctx.oz - PW)));
-    {
-        const int ci = ctx.ox - PW, cj = ctx.oy - PW, ck = ctx.oz - PW;
-        const bool inner = ci >= PW && ci < nx - PW && cj >= PW && cj < ny - PW && ck >= PW && ck < nz - PW;
-        ctx.corner_q = inner ? (ci - PW) + (nx - 2 * PW) * ((cj - PW) + (ny - 2 * PW) * (ck - PW)) : -1;
-    }
     for (int a = 0; a < 3; a++) {
         ctx.pmin[a] = B.pmin[a];
         ctx.h[a] = B.h[a];
@@ -977,9 +967,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
         const int jj = r / FRX, ii = r - jj * FRX;
         const bool in_region = ii < ex + 2 * PW && jj < ey + 2 * PW && kk < ez + 2 * PW;
         const int gi = ctx.ox - PW + ii, gj = ctx.oy - PW + jj, gk = ctx.oz - PW + kk;
-        const bool inner = in_region && gi >= PW && gi < nx - PW && gj >= PW && gj < ny - PW && gk >= PW && gk < nz - PW;
         ctx.goff[c] = in_region ? (uint32_t)(gi + nx * (gj + ny * gk)) : 0u;
-        ctx.qoff[c] = inner ? (gi - PW) + mx * ((gj - PW) + my * (gk - PW)) : -1;
     }
     // where this thread's D_x / D_y evaluations go (the config's slot table)
 #pragma unroll
@@ -989,13 +977,8 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     // this point's field values (registers; the compiler parks them in AGPRs)
     {
         T *f = ctx.f;
-        if (io.stage >= 1) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) f[v] = T(io.sin.p[v][q]);
-        } else {
-#pragma unroll
-            for (int v = 0; v < NV; v++) f[v] = T(io.u.p[v][idx]);
-        }
+        for (int v = 0; v < NV; v++) f[v] = T(io.src.p[v][idx]);
         if (prm.cako) {
             const T sg = sqrt(f[1]);
             ctx.sig_gauge = sg * T(prm.eps_gauge);
@@ -1040,6 +1023,22 @@ __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES 
     const Tile tl = tiles[blockIdx.x];
     const DBlock B = blocks[tl.block];
     fused_tile<FACE, C, T>(io, prm, tl, B, bconst + 3 * (size_t)tl.block, tabs, dtab, s_tile, s_dx, s_dy, s_ko);
+}
+
+// Copies the padding points of every block of src into dst (one workgroup per
+// block and variable): the RK4 work state's padding, refreshed from u once per
+// call of the RK4 entry points (u's padding is frozen while they run).
+__global__ __launch_bounds__(256) void k_pad_copy(VarPtrs dst, VarPtrs src, const DBlock *__restrict__ blocks) {
+    const DBlock &B = blocks[blockIdx.x];
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int n = nx * ny * nz;
+    const double *__restrict__ s = src.p[blockIdx.y] + B.offset;
+    double *__restrict__ d = dst.p[blockIdx.y] + B.offset;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const int i = p % nx, j = (p / nx) % ny, k = p / (nx * ny);
+        const bool inner = i >= PW && i < nx - PW && j >= PW && j < ny - PW && k >= PW && k < nz - PW;
+        if (!inner) d[p] = s[p];
+    }
 }
 
 // -------------------------------------------------------------------------
@@ -1221,7 +1220,8 @@ struct dgr_plan {
     dgr::AxisConsts *d_bconst = nullptr;  // k_fused: x, y, z stencil constants per block
     uint16_t *d_tabs = nullptr;           // k_fused: lane tables per tile shape, slot tables per config
     int dtab_wide = 0, dtab_thin = 0, dtab_full = 0;  // offsets of the configurations' slot tables
-    double *d_istage = nullptr; // k_fused RK4: 3 * 24 * interior (two stage states, accumulator)
+    double *d_istage = nullptr; // k_fused RK4: 2 * 24 * interior (u0 copy, accumulator)
+    double *d_wstate = nullptr; // k_fused RK4: 24 * szpdof, the padded work state W
     int *d_err = nullptr;
     // dgr_rk4_steps: one RK4 step captured as a graph, valid for the arguments in rk4_key
     hipStream_t cap_stream = nullptr;
@@ -1656,6 +1656,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     hipFree(plan->d_bconst);
     hipFree(plan->d_tabs);
     hipFree(plan->d_istage);
+    hipFree(plan->d_wstate);
     hipFree(plan->d_err);
     hipFree(plan->d_norm);
     if (plan->rk4_exec) hipGraphExecDestroy(plan->rk4_exec);
@@ -1824,7 +1825,7 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
     HIP_OK(hipSetDevice(plan->device));
     if (plan->fused) {
         StageIO io{};
-        io.u = u;
+        io.src = u;
         io.rhs = r;
         io.stage = -1;
         io.err = plan->d_err;
@@ -1846,7 +1847,7 @@ int dgr_plan_autotune(dgr_plan *plan, double *const *d_rhs, const double *const 
     const hipStream_t st = (hipStream_t)stream;
     const int p = plan->precision == DGR_PRECISION_MIXED ? 1 : 0;
     StageIO io{};
-    io.u = u;
+    io.src = u;
     io.rhs = r;
     io.stage = -1;
     io.err = plan->d_err;
@@ -1969,31 +1970,44 @@ const double RKU[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
 int ensure_istage(dgr_plan *plan) {
     const size_t ni = (size_t)plan->interior;
     if (!plan->d_istage && ni) {
-        hipError_t e = hipMalloc(&plan->d_istage, sizeof(double) * 3 * NV * ni);
+        hipError_t e = hipMalloc(&plan->d_istage, sizeof(double) * 2 * NV * ni);
         if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "stage buffers");
+    }
+    if (!plan->d_wstate && ni) {
+        hipError_t e = hipMalloc(&plan->d_wstate, sizeof(double) * NV * plan->szpdof);
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "work state");
     }
     return DGR_OK;
 }
 
+// W's padding <- u's (before the steps of one RK4 entry-point call)
+int launch_pad_copy(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
+    const uint32_t nb = (uint32_t)plan->hblocks.size();
+    if (!nb) return DGR_OK;
+    VarPtrs w;
+    for (int v = 0; v < NV; v++) w.p[v] = plan->d_wstate + (size_t)v * plan->szpdof;
+    hipLaunchKernelGGL(k_pad_copy, dim3(nb, NV), dim3(256), 0, st, w, u, plan->d_blocks);
+    HIP_OK(hipGetLastError());
+    return DGR_OK;
+}
+
 // The four stage launches of one RK4 step on the fused path (rkBSSN.cpp:764-860):
-// stage s reads u (s = 0) or the previous stage buffer, writes the other one;
-// the last stage writes the result into u. Needs ensure_istage() first.
+// stages read u, W, u, W and write the next state into W, u, W, u (see
+// StageIO). Needs ensure_istage() first.
 int rk4_fused_launches(dgr_plan *plan, const VarPtrs &u, double dt, const RhsParams &prm, int enforce_per_stage,
                        hipStream_t st) {
     const size_t ni = (size_t)plan->interior;
-    VarPtrs sa, sb, acc;
+    VarPtrs w, u0, acc;
     for (int v = 0; v < NV; v++) {
-        sa.p[v] = plan->d_istage + (size_t)v * ni;
-        sb.p[v] = plan->d_istage + (size_t)(NV + v) * ni;
-        acc.p[v] = plan->d_istage + (size_t)(2 * NV + v) * ni;
+        w.p[v] = plan->d_wstate + (size_t)v * plan->szpdof;
+        u0.p[v] = plan->d_istage + (size_t)v * ni;
+        acc.p[v] = plan->d_istage + (size_t)(NV + v) * ni;
     }
-    const VarPtrs *ins[4] = {&sa, &sa, &sb, &sa};
-    const VarPtrs *outs[4] = {&sa, &sb, &sa, &sb};
     for (int s = 0; s < 4; s++) {
         StageIO io{};
-        io.u = u;
-        io.sin = *ins[s];
-        io.sout = *outs[s];
+        io.src = (s & 1) ? w : u;
+        io.dst = (s & 1) ? u : w;
+        io.u0 = u0;
         io.acc = acc;
         io.cacc = RKC[s] * dt;
         io.cnext = s < 3 ? RKU[s + 1] * dt : 0.0;
@@ -2020,7 +2034,7 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
     HIP_OK(hipSetDevice(plan->device));
     hipStream_t st = (hipStream_t)stream;
     if (plan->fused) {
-        if ((rc = ensure_istage(plan))) return rc;
+        if ((rc = ensure_istage(plan)) || (rc = launch_pad_copy(plan, u, st))) return rc;
         return rk4_fused_launches(plan, u, dt, to_params(params), enforce_per_stage, st);
     }
     const size_t n = plan->szpdof;
@@ -2097,6 +2111,7 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         key.prm = *params;
         key.enforce = enf;
     }
+    if (nsteps > 0 && (rc = launch_pad_copy(plan, u, (hipStream_t)stream))) return rc;
     for (int i = 0; i < nsteps; i++) HIP_OK(hipGraphLaunch(plan->rk4_exec, (hipStream_t)stream));
     return DGR_OK;
 }

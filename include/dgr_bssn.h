@@ -175,9 +175,13 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
  * advanced without re-filling padding between stages). With
  * enforce_per_stage != 0 the algebraic constraints are enforced on every stage
  * state as rkBSSN.cpp:810-812 does; they are always enforced on the result
- * (rkBSSN.cpp:858, bssnCtx.cpp:1174-1182). Uses plan-owned stage buffers,
- * allocated on first use: three interior-only states (two stage states and
- * the accumulator, 3 x 24 x interior-points doubles). */
+ * (rkBSSN.cpp:858, bssnCtx.cpp:1174-1182). Uses plan-owned buffers,
+ * allocated on first use: a padded work state (24 x szpdof doubles, its
+ * padding refreshed from d_u's at the start of every call) and two
+ * interior-only states (the step's initial state and the accumulator,
+ * 2 x 24 x interior-points doubles). The stages alternate between d_u and the
+ * work state, so d_u's interior holds an intermediate stage state while the
+ * call runs (its padding is never written). */
 int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
                  int enforce_per_stage, void *stream);
 
