@@ -298,6 +298,9 @@ def main():
         plan.autotune(scratch, state, params, stream=stream)
         torch.cuda.synchronize()
         del scratch
+    # the plan allocates its RK4 buffers with hipMalloc: hand torch's cached
+    # blocks back first (the 120 M-point octree needs ~200 GB of them)
+    torch.cuda.empty_cache()
     tiling = {"x".join(str(n) for n in k): v for k, v in plan.tiling(args.precision).items()}
     log(f"tiling per block size ({'autotuned' if not args.no_autotune else 'default'}): {tiling}")
 
