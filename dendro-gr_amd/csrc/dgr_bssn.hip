@@ -692,14 +692,11 @@ struct FusedCtx {
         }
     }
 
-    // Where a phase stages its variable. Early (FP64): at the start of the
-    // phase, then a barrier. Late (FP32): the previous phase stages it at its
-    // end, so the equations that follow a phase share a barrier-free stretch
-    // with the next phase's D_x / D_y evaluations and the scheduler can
-    // overlap them with those LDS round trips. Measured: late staging is ~1%
-    // faster in FP32 and ~1% slower in FP64, where the equations' registers
-    // leave the LDS loads no room to run ahead.
-    static constexpr bool LATE = sizeof(T) < sizeof(double);
+    // Each phase stages the next phase's variable at its end (after its own
+    // stencils), so the equations that follow a phase share a barrier-free
+    // stretch with the next phase's D_x / D_y evaluations and the scheduler
+    // can overlap them with those LDS round trips. (Staging at the start of
+    // each phase instead measured ~1% slower in both precisions.)
 
     // Stages phase K and synchronizes (the kernel has fetched phases 0 .. PD-1).
     template <int K>
@@ -710,7 +707,7 @@ struct FusedCtx {
     }
 
     __device__ __forceinline__ void begin() {
-        if constexpr (LATE) stage_next<0>();
+        stage_next<0>();
     }
 
     // Phase KPH: variable V in LDS buffer KPH & 1; its derivatives, KO term
@@ -719,7 +716,6 @@ struct FusedCtx {
     __device__ __forceinline__ void phase() {
         static_assert(dgr::kBssnPhaseVar[KPH] == V, "phase order");
         using S = Stage<V>;
-        if constexpr (!LATE) stage_next<KPH>();
         S *t = reinterpret_cast<S *>(s_tile + (KPH & 1) * NREG);
         const dgr::AxisConsts &kx = K[0], &ky = K[1], &kz = K[2];
         constexpr int m2 = d2slot(V);
@@ -747,33 +743,40 @@ struct FusedCtx {
                     d = dgr::d1_edge<FRX>(s, d, oy + (w / FRX) % FRX, iey, loy, hiy, S(ky.d1_12));
                 sdy[w] = d;
             }
-            __syncthreads();
         }
         const S *c = t + cofs;
         if constexpr (m2 >= 0) {
-            const S *cx = sdx + dxo;
-            const S *cy = sdy + dyo;
-            d1v[0] = T(cx[0]);
-            d1v[1] = T(cy[0]);
+            // the pure second derivatives, D_z and (interior tiles) the KO
+            // term read only the staged region: they go before the barrier
+            // that publishes D_x / D_y, in one stretch with those LDS loads
             S r;
             r = dgr::d2_in<1>(c, c_d2<0, S>());
             if (FACE && (lox || hix)) r = dgr::d2_edge<1>(c, r, ax, iex, lox, hix, S(kx.d2_12));
             d2v[0] = T(r);
+            r = dgr::d2_in<FRX>(c, c_d2<1, S>());
+            if (FACE && (loy || hiy)) r = dgr::d2_edge<FRX>(c, r, ay, iey, loy, hiy, S(ky.d2_12));
+            d2v[3] = T(r);
+            r = dgr::d2_in<FRXY>(c, c_d2<2, S>());
+            if (FACE && (loz || hiz)) r = dgr::d2_edge<FRXY>(c, r, az, iez, loz, hiz, S(kz.d2_12));
+            d2v[5] = T(r);
+            r = dgr::d1_in<FRXY>(c, c_d1<2, S>());
+            if (FACE && (loz || hiz)) r = dgr::d1_edge<FRXY>(c, r, az, iez, loz, hiz, S(kz.d1_12));
+            d1v[2] = T(r);
+            if constexpr (!FACE) ko_term<V, S>(c, kx, ky, kz);
+            __syncthreads();
+            const S *cx = sdx + dxo;
+            const S *cy = sdy + dyo;
+            d1v[0] = T(cx[0]);
+            d1v[1] = T(cy[0]);
             r = dgr::d1_in<DXS_Y>(cx, c_d1<1, S>());  // D_y(D_x u)
             if (FACE && (loy || hiy)) r = dgr::d1_edge<DXS_Y>(cx, r, ay, iey, loy, hiy, S(ky.d1_12));
             d2v[1] = T(r);
             r = dgr::d1_in<DXS_Z>(cx, c_d1<2, S>());  // D_z(D_x u)
             if (FACE && (loz || hiz)) r = dgr::d1_edge<DXS_Z>(cx, r, az, iez, loz, hiz, S(kz.d1_12));
             d2v[2] = T(r);
-            r = dgr::d2_in<FRX>(c, c_d2<1, S>());
-            if (FACE && (loy || hiy)) r = dgr::d2_edge<FRX>(c, r, ay, iey, loy, hiy, S(ky.d2_12));
-            d2v[3] = T(r);
             r = dgr::d1_in<DYS_Z>(cy, c_d1<2, S>());  // D_z(D_y u)
             if (FACE && (loz || hiz)) r = dgr::d1_edge<DYS_Z>(cy, r, az, iez, loz, hiz, S(kz.d1_12));
             d2v[4] = T(r);
-            r = dgr::d2_in<FRXY>(c, c_d2<2, S>());
-            if (FACE && (loz || hiz)) r = dgr::d2_edge<FRXY>(c, r, az, iez, loz, hiz, S(kz.d2_12));
-            d2v[5] = T(r);
         } else {
             S r = dgr::d1_in<1>(c, c_d1<0, S>());
             if (FACE && (lox || hix)) r = dgr::d1_edge<1>(c, r, ax, iex, lox, hix, S(kx.d1_12));
@@ -781,9 +784,7 @@ struct FusedCtx {
             r = dgr::d1_in<FRX>(c, c_d1<1, S>());
             if (FACE && (loy || hiy)) r = dgr::d1_edge<FRX>(c, r, ay, iey, loy, hiy, S(ky.d1_12));
             d1v[1] = T(r);
-        }
-        {
-            S r = dgr::d1_in<FRXY>(c, c_d1<2, S>());
+            r = dgr::d1_in<FRXY>(c, c_d1<2, S>());
             if (FACE && (loz || hiz)) r = dgr::d1_edge<FRXY>(c, r, az, iez, loz, hiz, S(kz.d1_12));
             d1v[2] = T(r);
         }
@@ -795,8 +796,9 @@ struct FusedCtx {
 #pragma unroll
             for (int k = 0; k < 6; k++) DGR_PIN(d2v[k]);
         }
-        ko_term<V, S>(c, kx, ky, kz);
-        if constexpr (LATE && NEXT >= 0) stage_next<KPH + 1>();
+        // on a flagged face the boundary value needs the first derivatives
+        if constexpr (m2 < 0 || FACE) ko_term<V, S>(c, kx, ky, kz);
+        if constexpr (NEXT >= 0) stage_next<KPH + 1>();
     }
 
     // KO term sigma*(ko_x+ko_y+ko_z) (rhs.cpp:434-489); on a flagged face the
