@@ -549,9 +549,13 @@ static_assert(2 * (sizeof(double) * (CfgWide::NREG + (CfgWide::NREG + 1) / 2 + C
                    sizeof(float) * NV * CfgWide::FT) <= 160 * 1024,
               "two FP32 workgroups per CU");
 static_assert(7 * 7 * CfgFull::FZ <= CfgFull::FT, "one point per thread");
-static_assert(sizeof(double) * (CfgFull::NREG + (CfgFull::NREG + 1) / 2 + CfgFull::NSDX + CfgFull::NSDY) +
+// three FP64-sized region buffers and two D_x / D_y buffers (one barrier per phase)
+static_assert(sizeof(double) * (3 * CfgFull::NREG + 2 * (CfgFull::NSDX + CfgFull::NSDY)) +
                       sizeof(float) * NV * CfgFull::FT <= 160 * 1024,
               "one FP32 7x7x7 workgroup per CU");
+static_assert(sizeof(double) * (3 * CfgWide::NREG + 2 * (CfgWide::NSDX + CfgWide::NSDY) + NV * CfgWide::FT) <=
+                  160 * 1024,
+              "one FP64 regular workgroup per CU");
 // tile groups of the fused path, one launch each (FULL: FP32 only)
 enum { FG_WIDE = 0, FG_THIN = 1, FG_FACE = 2, FG_FULL = 3, NFGROUP = 4 };
 // Tilings of a block without flagged faces, chosen per block size and
@@ -671,15 +675,29 @@ struct FusedCtx {
     template <int V>
     using Stage = std::conditional_t<(sizeof(T) < sizeof(double) && (V == 0 || V == 1)), double, T>;
 
-    // region of the variable of phase K (prefetch slot K % PD) into LDS buffer
-    // K & 1; buffers sit at multiples of NREG doubles whatever the staging type
+    // One barrier per phase where LDS allows it (DEEP: the FP64 256-thread and
+    // FP32 384-thread kernels, which hold one workgroup per CU anyway): three
+    // region buffers and two D_x / D_y buffers, so phase K stages variable K+1
+    // before its single barrier, and phase K+1's D_x / D_y and region stencils
+    // follow phase K's equations with no barrier between them (the scheduler
+    // overlaps the equations' arithmetic with those LDS round trips). Buffer
+    // reuse: region buffer (K+1)%3 was last read by phase K-2 and D_x / D_y
+    // buffer K&1 by phase K-2, both before barrier K-1, which every wave has
+    // passed when it writes them in phase K. Other kernels keep two region
+    // buffers and two barriers in phases with mixed derivatives.
+    static constexpr bool DEEP = sizeof(T) == sizeof(double) ? C::FT == CfgWide::FT : C::FT == CfgFull::FT;
+    template <int K>
+    static constexpr int rbuf() { return DEEP ? K % 3 : K & 1; }
+
+    // region of the variable of phase K (prefetch slot K % PD) into its LDS
+    // buffer; buffers sit at multiples of NREG doubles whatever the staging type
     template <int K>
     __device__ __forceinline__ void stash() {
         constexpr int V = dgr::kBssnPhaseVar[K];
         using S = Stage<V>;
-        static_assert(sizeof(S) == sizeof(T) || K % 2 == 0,
+        static_assert(sizeof(S) == sizeof(T) || DEEP || K % 2 == 0,
                       "FP64-staged variables need the FP64-sized (even) buffer");
-        S *t = reinterpret_cast<S *>(s_tile + (K & 1) * NREG);
+        S *t = reinterpret_cast<S *>(s_tile + rbuf<K>() * NREG);
         if constexpr (sizeof(S) < sizeof(double)) {
             const double o = poff[K % PD];
 #pragma unroll
@@ -716,11 +734,11 @@ struct FusedCtx {
     __device__ __forceinline__ void phase() {
         static_assert(dgr::kBssnPhaseVar[KPH] == V, "phase order");
         using S = Stage<V>;
-        S *t = reinterpret_cast<S *>(s_tile + (KPH & 1) * NREG);
+        S *t = reinterpret_cast<S *>(s_tile + rbuf<KPH>() * NREG);
         const dgr::AxisConsts &kx = K[0], &ky = K[1], &kz = K[2];
         constexpr int m2 = d2slot(V);
-        S *sdx = reinterpret_cast<S *>(s_dx);
-        S *sdy = reinterpret_cast<S *>(s_dy);
+        S *sdx = reinterpret_cast<S *>(s_dx + (DEEP ? (KPH & 1) * C::NSDX : 0));
+        S *sdy = reinterpret_cast<S *>(s_dy + (DEEP ? (KPH & 1) * C::NSDY : 0));
         if constexpr (m2 >= 0) {
             // D_x u on tile-x x region-y x region-z and D_y u on tile-x x tile-y x region-z
             // (the reference's mixed derivatives are D_y(D_x u), D_z(D_x u), D_z(D_y u))
@@ -763,6 +781,10 @@ struct FusedCtx {
             if (FACE && (loz || hiz)) r = dgr::d1_edge<FRXY>(c, r, az, iez, loz, hiz, S(kz.d1_12));
             d1v[2] = T(r);
             if constexpr (!FACE) ko_term<V, S>(c, kx, ky, kz);
+            if constexpr (DEEP && NEXT >= 0) {
+                stash<KPH + 1>();
+                if constexpr (KPH + 1 + PD < NV) fetch<KPH + 1 + PD>();
+            }
             __syncthreads();
             const S *cx = sdx + dxo;
             const S *cy = sdy + dyo;
@@ -798,7 +820,9 @@ struct FusedCtx {
         }
         // on a flagged face the boundary value needs the first derivatives
         if constexpr (m2 < 0 || FACE) ko_term<V, S>(c, kx, ky, kz);
-        if constexpr (NEXT >= 0) stage_next<KPH + 1>();
+        if constexpr (!DEEP || m2 < 0) {
+            if constexpr (NEXT >= 0) stage_next<KPH + 1>();
+        }
     }
 
     // KO term sigma*(ko_x+ko_y+ko_z) (rhs.cpp:434-489); on a flagged face the
@@ -1015,11 +1039,12 @@ __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES 
     const StageIO io, const RhsParams prm, const DBlock *__restrict__ blocks,
     const dgr::AxisConsts *__restrict__ bconst, const Tile *__restrict__ tiles,
     const uint16_t *__restrict__ tabs, int dtab) {
-    // the region buffers sit at multiples of NREG doubles; in the FP32 kernel
-    // only the even-phase buffer holds FP64 data, the odd one FP32
-    __shared__ double s_tile[C::NREG + (sizeof(T) < sizeof(double) ? (C::NREG + 1) / 2 : C::NREG)];
-    __shared__ double s_dx[C::NSDX];
-    __shared__ double s_dy[C::NSDY];
+    // the region buffers sit at multiples of NREG doubles; with two buffers the
+    // FP32 kernel keeps FP64 data in the even-phase one only, the odd one FP32
+    constexpr bool DEEP = FusedCtx<FACE, C, T>::DEEP;
+    __shared__ double s_tile[DEEP ? 3 * C::NREG : C::NREG + (sizeof(T) < sizeof(double) ? (C::NREG + 1) / 2 : C::NREG)];
+    __shared__ double s_dx[(DEEP ? 2 : 1) * C::NSDX];
+    __shared__ double s_dy[(DEEP ? 2 : 1) * C::NSDY];
     __shared__ T s_ko[NV * C::FT];
 
     const Tile tl = tiles[blockIdx.x];
