@@ -167,6 +167,39 @@ def test_rk4_step_matches_oracle(enforce):
     plan.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_rk4_after_padding_change_matches_oracle(graph):
+    """The RK4 stages alternate between u and a plan-owned work state whose
+    padding is copied from u at every call: a caller that refills u's padding
+    between calls (as the solver's unzip does) must see the new padding used."""
+    import torch
+    log(f"rk4 after padding change, graph={graph}")
+    t = W.uniform_13(8)
+    t["bflag"][5] = 21
+    st = host_state(t, seed=13)
+    p = block_params()
+    plan = dgr.Plan(t, st.shape[1])
+    u = to_dev(st)
+    step = (lambda x: plan.rk4_steps(x, 0.025, p, nsteps=1)) if graph else (lambda x: plan.rk4_step(x, 0.025, p))
+    step(u)
+    torch.cuda.synchronize()
+    # new padding: the state of another seed; the interior keeps the first step's result
+    pad = padding_mask(t, st.shape[1])
+    other = host_state(t, seed=14)
+    mid = u.cpu().numpy()
+    mid[:, pad] = other[:, pad]
+    u.copy_(to_dev(mid))
+    step(u)
+    torch.cuda.synchronize()
+    assert plan.status() == 0
+    got = u.cpu().numpy()
+    ref = mid.copy()
+    O.rk4_step(ref, t, 0.025, oracle_params(p), nthreads=8)
+    assert np.array_equal(got[:, pad], mid[:, pad])
+    assert normwise(interior_values(got - mid, t), interior_values(ref - mid, t)).max() <= 1e-11
+    plan.close()
+
+
 def test_rk4_matches_reference_fixture():
     import torch
     g = np.load(os.path.join(GOLD, "block19.npz"))
