@@ -1665,29 +1665,30 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
 
 int dgr_plan_destroy(dgr_plan *plan) {
     if (!plan) return DGR_OK;
-    hipSetDevice(plan->device);
+    // best-effort release: errors here have no one to report to
+    (void)hipSetDevice(plan->device);
     for (auto &v : plan->pending)
         for (auto &pr : v) {
-            hipEventDestroy(pr.first);
-            hipEventDestroy(pr.second);
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
         }
-    for (hipEvent_t e : plan->event_pool) hipEventDestroy(e);
-    hipFree(plan->d_blocks);
-    hipFree(plan->d_tiles);
-    hipFree(plan->d_wg_tile);
-    hipFree(plan->d_face_tiles);
-    hipFree(plan->d_W);
-    hipFree(plan->d_stage);
-    hipFree(plan->d_ftiles[0]);
-    hipFree(plan->d_ftiles[1]);
-    hipFree(plan->d_bconst);
-    hipFree(plan->d_tabs);
-    hipFree(plan->d_istage);
-    hipFree(plan->d_wstate);
-    hipFree(plan->d_err);
-    hipFree(plan->d_norm);
-    if (plan->rk4_exec) hipGraphExecDestroy(plan->rk4_exec);
-    if (plan->cap_stream) hipStreamDestroy(plan->cap_stream);
+    for (hipEvent_t e : plan->event_pool) (void)hipEventDestroy(e);
+    (void)hipFree(plan->d_blocks);
+    (void)hipFree(plan->d_tiles);
+    (void)hipFree(plan->d_wg_tile);
+    (void)hipFree(plan->d_face_tiles);
+    (void)hipFree(plan->d_W);
+    (void)hipFree(plan->d_stage);
+    (void)hipFree(plan->d_ftiles[0]);
+    (void)hipFree(plan->d_ftiles[1]);
+    (void)hipFree(plan->d_bconst);
+    (void)hipFree(plan->d_tabs);
+    (void)hipFree(plan->d_istage);
+    (void)hipFree(plan->d_wstate);
+    (void)hipFree(plan->d_err);
+    (void)hipFree(plan->d_norm);
+    if (plan->rk4_exec) (void)hipGraphExecDestroy(plan->rk4_exec);
+    if (plan->cap_stream) (void)hipStreamDestroy(plan->cap_stream);
     delete plan;
     return DGR_OK;
 }
@@ -1771,10 +1772,15 @@ void timed(dgr_plan *plan, int kind, hipStream_t st, F launch) {
         return;
     }
     hipEvent_t a = plan->take_event(), b = plan->take_event();
-    if (a) hipEventRecord(a, st);
+    const bool ra = a && hipEventRecord(a, st) == hipSuccess;
     launch();
-    if (b) hipEventRecord(b, st);
-    if (a && b) plan->pending[kind].push_back({a, b});
+    const bool rb = b && hipEventRecord(b, st) == hipSuccess;
+    if (ra && rb) {
+        plan->pending[kind].push_back({a, b});
+    } else {  // an unrecorded pair would fail in dgr_plan_kernel_times: drop it
+        if (a) plan->event_pool.push_back(a);
+        if (b) plan->event_pool.push_back(b);
+    }
 }
 
 template <int MODE>
@@ -2123,12 +2129,12 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(plan->cap_stream, &g);
         if (rc) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return rc;
         }
         if (ec != hipSuccess) return fail(DGR_E_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
         const hipError_t ei = hipGraphInstantiate(&plan->rk4_exec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         if (ei != hipSuccess) {
             plan->rk4_exec = nullptr;
             return fail(DGR_E_HIP, std::string("graph instantiation: ") + hipGetErrorString(ei));
