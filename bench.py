@@ -156,13 +156,15 @@ def mixed_comparison(wl, state0, params, steps, fp64_value, stream, autotune=Tru
     """The batched mixed-precision path on the same workload: RK4 Mpoints/s
     (graph replay), and its error against the FP64 path from identical inputs:
     the right-hand side (max over variables of the normwise relative error
-    over interior points) and the state after one RK4 step (max |difference|
-    over the state's scale, max |u|, and per variable)."""
+    over interior points, max-norm and L2) and the state after one RK4 step
+    (max |difference| over the state's scale, max |u|, and per variable, max-
+    norm and L2); on the puncture workload also the Hamiltonian-constraint
+    norm after 10 RK4 steps (masked r < 5 around each hole) in both precisions."""
     import torch
     import dgr
     from dgr import workloads as W
     mask = torch.from_numpy(interior_mask(wl.blocks, wl.n_pad)).to(state0.device)
-    res = {}
+    res, hams = {}, {}
     for prec in ("fp64", "mixed"):
         plan = dgr.Plan(wl.blocks, wl.n_pad)
         plan.set_precision(prec)
@@ -175,6 +177,13 @@ def mixed_comparison(wl, state0, params, steps, fp64_value, stream, autotune=Tru
         plan.rk4_step(u, wl.dt, params, stream=stream)
         torch.cuda.synchronize()
         res[prec] = (out[:, mask], u[:, mask])
+        if wl.name == "puncture":
+            u.copy_(state0)
+            plan.rk4_steps(u, wl.dt, params, nsteps=10, stream=stream)
+            ham = out[0]  # one padded array as the constraint's output
+            plan.ham(ham, u, stream=stream)
+            hams[prec] = plan.constraint_norm(ham, stream=stream)
+            u.copy_(state0)
         if prec == "mixed":
             plan.rk4_steps(u, wl.dt, params, nsteps=2, stream=stream)
             torch.cuda.synchronize()
@@ -191,12 +200,21 @@ def mixed_comparison(wl, state0, params, steps, fp64_value, stream, autotune=Tru
     e_rhs = ((r32 - r64).abs().amax(dim=1) / r64.abs().amax(dim=1).clamp_min(1e-300)).max().item()
     e_var = ((u32 - u64).abs().amax(dim=1) / u64.abs().amax(dim=1).clamp_min(1e-300)).max().item()
     e_scale = ((u32 - u64).abs().max() / u64.abs().max()).item()
+    e_rhs_l2 = ((r32 - r64).norm(dim=1) / r64.norm(dim=1).clamp_min(1e-300)).max().item()
+    e_var_l2 = ((u32 - u64).norm(dim=1) / u64.norm(dim=1).clamp_min(1e-300)).max().item()
+    err = {"state_after_one_step_vs_state_scale": e_scale,
+           "state_after_one_step_max_per_variable": e_var,
+           "state_after_one_step_l2_per_variable": e_var_l2,
+           "rhs_max_normwise_per_variable": e_rhs,
+           "rhs_l2_per_variable": e_rhs_l2}
+    if hams:
+        err["ham_constraint_norm_after_10_steps"] = {
+            "fp64": hams["fp64"], "mixed": hams["mixed"],
+            "relative_difference": abs(hams["mixed"] - hams["fp64"]) / abs(hams["fp64"])}
     return {"value": round(value, 2), "unit": "Mpoints/s", "speedup_vs_fp64": round(value / fp64_value, 3),
             "tiling": tiling if autotune else "default",
             "dtype": "f32 arithmetic (alpha, chi staged and differentiated in f64), f64 state and RK",
-            "error_vs_fp64": {"state_after_one_step_vs_state_scale": e_scale,
-                              "state_after_one_step_max_per_variable": e_var,
-                              "rhs_max_normwise_per_variable": e_rhs}}
+            "error_vs_fp64": err}
 
 
 def interior_mask(blocks, npad):
