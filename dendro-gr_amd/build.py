@@ -34,16 +34,30 @@ def regenerate(verbose=False):
           os.path.join(HERE, "csrc", "bssn_rhs_generated.h")], verbose)
 
 
+# The FP64 kernels (and the rest of the library) are scheduled with the
+# iterative ILP strategy; the FP32 fused kernels, compiled from the same
+# source with DGR_F32_KERNELS in a unit of their own, keep the default one.
+MAIN_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]
+
+
+def build_bssn(src: str, out: str, verbose: bool = False, extra=()) -> None:
+    """libdgr_bssn.so from the HIP source `src` (two units) and the C++ shim."""
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc")]
+    objs = [out + ".main.o", out + ".f32.o"]
+    _run([HIPCC, *HIP_FLAGS, *MAIN_FLAGS, *extra, "-c", *inc, src, "-o", objs[0]], verbose)
+    _run([HIPCC, *HIP_FLAGS, *extra, "-DDGR_F32_KERNELS", "-c", *inc, src, "-o", objs[1]], verbose)
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", *inc, *objs, os.path.join(HERE, "host", "bssn_shim.cpp"), "-o", tmp],
+         verbose)
+    os.replace(tmp, out)
+    for o in objs:
+        os.remove(o)
+
+
 def build(verbose: bool = False) -> None:
     os.makedirs(LIB, exist_ok=True)
-    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc")]
-    out = os.path.join(LIB, "libdgr_bssn.so")
-    tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", *inc,
-          os.path.join(HERE, "csrc", "dgr_bssn.hip"),
-          os.path.join(HERE, "host", "bssn_shim.cpp"),
-          "-o", tmp], verbose)
-    os.replace(tmp, out)
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc")]  # for the host library
+    build_bssn(os.path.join(HERE, "csrc", "dgr_bssn.hip"), os.path.join(LIB, "libdgr_bssn.so"), verbose)
     host = os.path.join(LIB, "libdgr_algebra_host.so")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", *inc,
           os.path.join(HERE, "csrc", "algebra_host.cpp"), "-o", host + ".tmp"], verbose)
