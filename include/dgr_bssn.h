@@ -83,8 +83,9 @@ const char *dgr_last_error(void);
 
 /* Validates the block table (sizes >= 13 per axis, blocks inside [0, szpdof)),
  * splits every block into tiles and orders the work for the device. The
- * default is the single-kernel path (tiles of at most 7 x 7 x 5 points, no
- * workspace); the environment variable DGR_KERNEL=split selects the
+ * default is the single-kernel path (tiles of at most 7 x 7 x 5 points, or
+ * 7 x 7 x 7 in mixed precision, see DGR_TILING_*; no derivative workspace);
+ * the environment variable DGR_KERNEL=split selects the
  * two-kernel path (tiles of at most 7^3 points; a derivative workspace of 162
  * doubles per point in chunks of DGR_CHUNK_POINTS points).
  * Replaces the reference's global derivative workspace and its per-block
