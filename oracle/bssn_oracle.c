@@ -214,9 +214,10 @@ static int d2slot(int v) {
     }
 }
 
-/* The pointwise algebra shared by the right-hand side (rhs != NULL, needs p)
- * and the Hamiltonian constraint (ham != NULL). */
-static void point_algebra(const double *in, const orc_params *p, double *rhs, double *ham) {
+/* The pointwise algebra shared by the right-hand side (rhs != NULL, needs p),
+ * the Hamiltonian constraint (ham != NULL) and the momentum constraints
+ * (mom != NULL, three values). */
+static void point_algebra(const double *in, const orc_params *p, double *rhs, double *ham, double *mom) {
     const double *f = in;
 #define D1(v, d) in[24 + 3 * (v) + (d)]
 #define D2(v, a, b) in[96 + 6 * d2slot(v) + SYM6[a][b]]
@@ -379,6 +380,23 @@ static void point_algebra(const double *in, const orc_params *p, double *rhs, do
             for (int k = 0; k < 3; k++) s += chi * igt[j][k] * R[j][k];
         *ham = s - sqrA + 2.0 / 3.0 * K * K;
     }
+    /* momentum constraints (CodeGen/constraints.py:256-282):
+     * mom_i = igt^jk (d_k At_ij - C2^m_ki At_jm) - Gt^j At_ij
+     *         - 3/2 igt^jk At_ki d_j chi / chi - 2/3 d_i K, with the evolved Gt^j */
+    if (mom) {
+        for (int i = 0; i < 3; i++) {
+            double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) {
+                    double cA = 0.0;
+                    for (int m = 0; m < 3; m++) cA += C2[m][k][i] * A[j][m];
+                    s1 += igt[j][k] * (dA[k][i][j] - cA);
+                    s3 += igt[j][k] * A[k][i] * dchi[j] / chi;
+                }
+            for (int j = 0; j < 3; j++) s2 += f[ORC_GT0 + j] * A[i][j];
+            mom[i] = s1 - s2 - 1.5 * s3 - 2.0 / 3.0 * dK[i];
+        }
+    }
     if (!rhs) return;
 
     double divb = db[0][0] + db[1][1] + db[2][2];
@@ -451,9 +469,15 @@ static void point_algebra(const double *in, const orc_params *p, double *rhs, do
     }
 }
 
-void orc_bssn_point(const double *in, const orc_params *p, double *rhs) { point_algebra(in, p, rhs, NULL); }
+void orc_bssn_point(const double *in, const orc_params *p, double *rhs) { point_algebra(in, p, rhs, NULL, NULL); }
 
-void orc_ham_point(const double *in, double *ham) { point_algebra(in, NULL, NULL, ham); }
+void orc_ham_point(const double *in, double *ham) { point_algebra(in, NULL, NULL, ham, NULL); }
+
+void orc_constraint_point(const double *in, double *ham, double *mom) { point_algebra(in, NULL, NULL, ham, mom); }
+
+void orc_constraint_points(const double *in, int64_t npts, double *ham, double *mom) {
+    for (int64_t q = 0; q < npts; q++) orc_constraint_point(in + q * ORC_POINT_NIN, ham + q, mom + 3 * q);
+}
 
 void orc_ham_points(const double *in, int64_t npts, double *ham) {
     for (int64_t q = 0; q < npts; q++) orc_ham_point(in + q * ORC_POINT_NIN, ham + q);
@@ -664,7 +688,16 @@ int orc_bssnrhs_batch(double *const *rhs, const double *const *u, const orc_bloc
 /* Hamiltonian constraint per block: physical_constraints, physcon.cpp:12-89  */
 /* (C_HAM only; interior points, padding of `ham` untouched)                  */
 /* ======================================================================== */
+static int constraint_block(double *ham_all, double *const *mom_all, const double *const *u_all,
+                            const orc_block *blk);
+
 int orc_ham_block(double *ham_all, const double *const *u_all, const orc_block *blk) {
+    return constraint_block(ham_all, NULL, u_all, blk);
+}
+
+/* C_HAM and (mom_all != NULL: three arrays laid out like a variable) C_MOM */
+static int constraint_block(double *ham_all, double *const *mom_all, const double *const *u_all,
+                            const orc_block *blk) {
     const uint32_t *sz = blk->sz;
     const int nx = (int)sz[0], ny = (int)sz[1], nz = (int)sz[2];
     if (nx < 13 || ny < 13 || nz < 13) return -1;
@@ -692,9 +725,24 @@ int orc_ham_block(double *ham_all, const double *const *u_all, const orc_block *
                 for (int m = 0; m < 11; m++)
                     for (int c = 0; c < 6; c++) in[96 + 6 * m + c] = d2[m][c][pp];
                 in[162] = 0.0;
-                orc_ham_point(in, &ham[pp]);
+                double m[3];
+                orc_constraint_point(in, &ham[pp], m);
+                if (mom_all)
+                    for (int c = 0; c < 3; c++) mom_all[c][blk->offset + pp] = m[c];
             }
     return 0;
+}
+
+int orc_constraints_batch(double *ham, double *const *mom, const double *const *u, const orc_block *blks,
+                          int64_t nblocks, int nthreads) {
+    int err = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+#endif
+    for (int64_t b = 0; b < nblocks; b++) err |= (constraint_block(ham, mom, u, &blks[b]) != 0);
+    (void)nthreads;
+    return err ? -1 : 0;
 }
 
 int orc_ham_batch(double *ham, const double *const *u, const orc_block *blks, int64_t nblocks, int nthreads) {

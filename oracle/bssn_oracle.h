@@ -104,10 +104,18 @@ void orc_bssn_points(const double *in, int64_t npts, const orc_params *p, double
 /* Hamiltonian constraint at a point, same input layout (eta unused). */
 void orc_ham_point(const double *in, double *ham);
 void orc_ham_points(const double *in, int64_t npts, double *ham);
+/* C_HAM and the momentum constraints mom0..2 (CodeGen/constraints.py:256-282)
+ * of one point / npts points (mom: three per point). */
+void orc_constraint_point(const double *in, double *ham, double *mom);
+void orc_constraint_points(const double *in, int64_t npts, double *ham, double *mom);
 /* C_HAM of physical_constraints (physcon.cpp:12-89) at the interior points of
  * every block; the padding of `ham` is not written. */
 int orc_ham_block(double *ham, const double *const *u, const orc_block *blk);
 int orc_ham_batch(double *ham, const double *const *u, const orc_block *blks, int64_t nblocks, int nthreads);
+/* orc_ham_batch plus the momentum constraints into mom[0..2] (laid out like a
+ * variable; padding not written). */
+int orc_constraints_batch(double *ham, double *const *mom, const double *const *u, const orc_block *blks,
+                          int64_t nblocks, int nthreads);
 /* computeConstraintL2Norm (grUtils.tcc:37-134) over interior points outside
  * r1 of BH 1 and r2 of BH 2 (bh = x1,y1,z1,x2,y2,z2); -1 if none qualifies. */
 double orc_constraint_l2_norm(const double *c, const orc_block *blks, int64_t nblocks, const double *bh,

@@ -82,6 +82,9 @@ def lib():
         L.orc_ham_points.argtypes = [dp, C.c_int64, dp]
         L.orc_ham_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
         L.orc_ham_batch.restype = C.c_int
+        L.orc_constraint_points.argtypes = [dp, C.c_int64, dp, dp]
+        L.orc_constraints_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+        L.orc_constraints_batch.restype = C.c_int
         L.orc_constraint_l2_norm.argtypes = [dp, C.c_void_p, C.c_int64, dp, C.c_double, C.c_double]
         L.orc_constraint_l2_norm.restype = C.c_double
         L.orc_enforce_constraints_point.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
@@ -110,6 +113,7 @@ def ref_lib():
         dp = C.POINTER(C.c_double)
         R.ref_bssn_points.argtypes = [dp, C.c_int64, dp, dp, dp]
         R.ref_ham_points.argtypes = [dp, C.c_int64, dp]
+        R.ref_constraint_points.argtypes = [dp, C.c_int64, dp, dp]
         _ref = R
     return _ref
 
@@ -168,6 +172,26 @@ def ref_ham_points(inp: np.ndarray) -> np.ndarray:
     out = np.zeros(inp.shape[0])
     R.ref_ham_points(_dp(inp), inp.shape[0], _dp(out))
     return out
+
+
+def constraint_points(inp: np.ndarray):
+    """(ham, mom) per point: C_HAM and the momentum constraints, mom shaped (n, 3)."""
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    ham = np.zeros(inp.shape[0])
+    mom = np.zeros((inp.shape[0], 3))
+    lib().orc_constraint_points(_dp(inp), inp.shape[0], _dp(ham), _dp(mom))
+    return ham, mom
+
+
+def ref_constraint_points(inp: np.ndarray):
+    """The reference's physconeqs.cpp: (ham, mom) with mom shaped (n, 3)."""
+    R = ref_lib()
+    assert R is not None, "oracle/_ref not built (needs the reference checkout)"
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    ham = np.zeros(inp.shape[0])
+    mom = np.zeros((inp.shape[0], 3))
+    R.ref_constraint_points(_dp(inp), inp.shape[0], _dp(ham), _dp(mom))
+    return ham, mom
 
 
 def stencil(name: str, u: np.ndarray, h: float, sz, bflag: int) -> np.ndarray:
@@ -256,6 +280,20 @@ def ham_batch(state: np.ndarray, blocks: np.ndarray, nthreads: int = 0) -> np.nd
     if err != 0:
         raise RuntimeError("orc_ham_batch failed")
     return ham
+
+
+def constraints_batch(state: np.ndarray, blocks: np.ndarray, nthreads: int = 0):
+    """(ham, mom) at the interior points of every block: C_HAM (szpdof) and the
+    momentum constraints (3, szpdof); padding left 0."""
+    ham = np.zeros(state.shape[1])
+    mom = np.zeros((3, state.shape[1]))
+    blocks = np.ascontiguousarray(blocks)
+    mp = (C.c_void_p * 3)(*[mom[c].ctypes.data for c in range(3)])
+    err = lib().orc_constraints_batch(_dp(ham), mp, _ptrs(state_views(state)), blocks.ctypes.data,
+                                      len(blocks), nthreads)
+    if err != 0:
+        raise RuntimeError("orc_constraints_batch failed")
+    return ham, mom
 
 
 def constraint_l2_norm(c: np.ndarray, blocks: np.ndarray, bh=((3.0, 0, 0), (-3.0, 0, 0)),

@@ -82,9 +82,9 @@ extern "C" void ref_bssn_point_orc(const double *in, const void *params, double 
 
 // The reference's generated constraint fragment (BSSN_GR/src/physconeqs.cpp,
 // included in place as physcon.cpp:78-86 does) over the same point layout:
-// only ham[pp] is returned; psi4 and the momentum constraints it also writes
-// are discarded. x, y, z (read by the psi4 part) are set to (1, 2, 3).
-extern "C" int ref_ham_points(const double *in, int64_t npts, double *ham_out) {
+// ham[pp] (and, with mom_out, mom0..2[pp], three per point) are returned;
+// psi4 is discarded. x, y, z (read by the psi4 part) are set to (1, 2, 3).
+extern "C" int ref_constraint_points(const double *in, int64_t npts, double *ham_out, double *mom_out) {
     std::vector<double> soa((size_t)NIN * (size_t)npts);
     std::vector<double> res((size_t)6 * (size_t)npts);
     for (int64_t q = 0; q < npts; q++)
@@ -105,5 +105,15 @@ extern "C" int ref_ham_points(const double *in, int64_t npts, double *ham_out) {
 #include "physconeqs.cpp"
     }
     for (int64_t q = 0; q < npts; q++) ham_out[q] = ham[q];
+    if (mom_out)
+        for (int64_t q = 0; q < npts; q++) {
+            mom_out[3 * q] = mom0[q];
+            mom_out[3 * q + 1] = mom1[q];
+            mom_out[3 * q + 2] = mom2[q];
+        }
     return 0;
+}
+
+extern "C" int ref_ham_points(const double *in, int64_t npts, double *ham_out) {
+    return ref_constraint_points(in, npts, ham_out, nullptr);
 }

@@ -18,7 +18,8 @@ Fixtures:
   uniform13_8.npz     the first 8 blocks of the uniform 13^3 batch, seed 2:
                       interior RHS (KO sigma 0.4, eta 2)
   ham_points.npz      for the 200 inputs of algebra_points.npz (checksum kept),
-                      the Hamiltonian constraint of the reference's generated
+                      the Hamiltonian constraint (ham) and the momentum
+                      constraints (mom, 200 x 3) of the reference's generated
                       constraint equations (BSSN_GR/src/physconeqs.cpp,
                       compiled in place)
 
@@ -87,7 +88,8 @@ def block_params():
 
 def write_ham():
     inp = algebra_inputs()
-    np.savez_compressed(os.path.join(HERE, "ham_points.npz"), ham=O.ref_ham_points(inp),
+    ham, mom = O.ref_constraint_points(inp)
+    np.savez_compressed(os.path.join(HERE, "ham_points.npz"), ham=ham, mom=mom,
                         inputs_sum=np.array([inp.sum(), np.abs(inp).sum()]))
 
 

@@ -125,3 +125,70 @@ def test_constraint_norm_masks_and_averages_like_the_reference():
     assert len(vals) > 0 and skipped > 0
     assert got == pytest.approx(np.sqrt(np.mean(np.square(vals))), rel=1e-12)
     assert O.constraint_l2_norm(c, t, r1=1e9, r2=1e9) == -1.0
+
+
+# --- momentum constraints (CodeGen/constraints.py:256-282) -----------------
+
+def test_mom_matches_reference_constraint_equations():
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    h = np.load(os.path.join(GOLD, "ham_points.npz"))
+    _, mom = O.constraint_points(g["inputs"])
+    ref = h["mom"]
+    assert ref.shape == (g["inputs"].shape[0], 3)
+    assert (np.abs(mom - ref).max(axis=0) / np.abs(ref).max(axis=0)).max() < 1e-13
+
+
+def test_mom_fixture_still_matches_reference_build():
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    inp = np.load(os.path.join(GOLD, "algebra_points.npz"))["inputs"]
+    assert np.array_equal(O.ref_constraint_points(inp)[1], np.load(os.path.join(GOLD, "ham_points.npz"))["mom"])
+
+
+def _flat_block_state(fill):
+    """One 13^3 block of flat data (alpha = chi = 1, gt = delta) plus fill(st, x, y, z)."""
+    t = W.uniform_13(1)
+    n = 13
+    st = np.zeros((24, W.total_padded(t)))
+    for v in (0, 1, 12, 15, 17):
+        st[v] = 1.0
+    h = (t["pmax"][0] - t["pmin"][0]) / (n - 1)
+    ax = [t["pmin"][0][a] + h[a] * np.arange(n) for a in range(3)]
+    z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    fill(st, x.ravel(), y.ravel(), z.ravel())
+    return t, st
+
+
+def test_mom_of_minkowski_is_exactly_zero():
+    t, st = _flat_block_state(lambda st, x, y, z: None)
+    ham, mom = O.constraints_batch(st, t, nthreads=1)
+    assert np.count_nonzero(mom) == 0 and np.count_nonzero(ham) == 0
+
+
+def test_mom_of_linear_K_is_minus_two_thirds_grad_K():
+    """Flat metric, At = 0, K = c.x: mom_i = -2/3 c_i (the stencils are exact on
+    linear data; a wrong sign or coefficient of the d_i K term fails)."""
+    c = np.array([0.3, -0.7, 1.1])
+
+    def fill(st, x, y, z):
+        st[2] = c[0] * x + c[1] * y + c[2] * z
+    t, st = _flat_block_state(fill)
+    _, mom = O.constraints_batch(st, t, nthreads=1)
+    for i in range(3):
+        got = interior_flat(mom[i], t)
+        assert np.abs(got - (-2.0 / 3.0) * c[i]).max() < 1e-12
+
+
+def test_mom_divergence_term_contracts_the_right_index():
+    """Flat metric, At_xy = a y (everything else flat): mom_x = d_y At_xy = a and
+    mom_y = d_x At_yx = 0, mom_z = 0. Differentiating along the wrong index of
+    At_ij gives mom_x = 0."""
+    a = 0.37
+
+    def fill(st, x, y, z):
+        st[19] = a * y  # At01
+    t, st = _flat_block_state(fill)
+    _, mom = O.constraints_batch(st, t, nthreads=1)
+    assert np.abs(interior_flat(mom[0], t) - a).max() < 1e-12
+    assert np.abs(interior_flat(mom[1], t)).max() < 1e-12
+    assert np.abs(interior_flat(mom[2], t)).max() < 1e-12
