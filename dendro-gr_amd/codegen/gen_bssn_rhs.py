@@ -199,6 +199,14 @@ for i, j in PAIRS:
 HAM = (sum(chi * igt[j, k] * R[j, k] for j, k in E_IJ) - sum(At[i, j] * AtUU[i, j] for i, j in E_IJ)
        + Rational(2, 3) * K ** 2)
 
+# Momentum constraints (CodeGen/constraints.py:256-282), with the evolved Gt^j:
+# igt^jk (d_k At_ij - C2^m_ki At_jm) - Gt^j At_ij - 3/2 igt^jk At_ki d_j chi / chi
+# - 2/3 d_i K
+MOM = [sum(igt[j, k] * (d(k, At_n[i][j]) - sum(C2[(m, k, i)] * At[j, m] for m in E_I)) for j, k in E_IJ)
+       - sum(fld[Gt_n[j]] * At[i, j] for j in E_I)
+       - Rational(3, 2) * sum(igt[j, k] * At[k, i] * d(j, "chi") for j, k in E_IJ) * ichi
+       - Rational(2, 3) * d(i, "K") for i in E_I]
+
 # ---------------------------------------------------------------------------
 # code emission
 
@@ -608,21 +616,23 @@ DGR_HD DGR_INLINE void bssn_rhs_phased(Ctx &ctx, const T eta, const T *lam, cons
 
 def generate_ham(path):
     t0 = time.time()
-    defs, outs_r, ops_after = eliminate([HAM])
+    defs, outs_r, ops_after = eliminate([HAM] + MOM)
     order = schedule(defs, outs_r)
-    lines, ntmp, max_live = emit(order, defs, outs_r, input_expr, out_names=["ham"])
+    lines, ntmp, max_live = emit(order, defs, outs_r, input_expr, out_names=["ham", "mom0", "mom1", "mom2"])
     hdr = """// Generated by dendro-gr_amd/codegen/gen_bssn_rhs.py --ham -- do not edit.
 //
-// Pointwise Hamiltonian constraint of the BSSN variables (CodeGen/constraints.py:
-// 250-254, which the reference expands into BSSN_GR/src/physconeqs.cpp):
-// chi igt^jk R_jk - At_ij At^ij + 2/3 K^2, built from the same named
+// Pointwise constraints of the BSSN variables (CodeGen/constraints.py:250-282,
+// which the reference expands into BSSN_GR/src/physconeqs.cpp): the
+// Hamiltonian constraint chi igt^jk R_jk - At_ij At^ij + 2/3 K^2 and the
+// momentum constraints igt^jk (d_k At_ij - C2^m_ki At_jm) - Gt^j At_ij
+// - 3/2 igt^jk At_ki d_j chi / chi - 2/3 d_i K, built from the same named
 // intermediates (inverse metric, Christoffel symbols, Ricci tensor, raised At)
 // as the right-hand side.
 """ + f"""// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values live.
 //
-// In: the accessor of bssn_rhs_point (fields, first derivatives of chi, Gt,
-//     gt; second derivatives of chi and gt are the ones read).
-// Out: receives put<0>(ham).
+// In: the accessor of bssn_rhs_point (fields, first derivatives of chi, K,
+//     Gt, gt, At; second derivatives of chi and gt are the ones read).
+// Out: receives put<0>(ham), put<1..3>(mom0..2).
 #pragma once
 
 #ifndef DGR_HD

@@ -37,10 +37,18 @@ void run(const double *in, int64_t n, const double *lam, const double *lamf, dou
     }
 }
 template <typename T>
-struct Out1 {
+struct Out1 {  // keeps the Hamiltonian constraint (output 0) only
     double *o;
     template <int V>
-    void put(T x) { *o = (double)x; }
+    void put(T x) {
+        if (V == 0) *o = (double)x;
+    }
+};
+template <typename T>
+struct Out4 {  // ham, mom0, mom1, mom2
+    double *o;
+    template <int V>
+    void put(T x) { o[V] = (double)x; }
 };
 }  // namespace
 
@@ -48,6 +56,14 @@ extern "C" void dgr_host_ham_points(const double *in, int64_t n, double *out) {
     for (int64_t q = 0; q < n; q++) {
         In163<double> a{in + 163 * q};
         Out1<double> b{out + q};
+        dgr::bssn_ham_point<double>(a, b);
+    }
+}
+
+extern "C" void dgr_host_constraint_points(const double *in, int64_t n, double *out) {
+    for (int64_t q = 0; q < n; q++) {
+        In163<double> a{in + 163 * q};
+        Out4<double> b{out + 4 * q};
         dgr::bssn_ham_point<double>(a, b);
     }
 }

@@ -1075,16 +1075,28 @@ __global__ __launch_bounds__(256) void k_pad_copy(VarPtrs dst, VarPtrs src, cons
 // constraint_derivs.h makes. A diagnostic: it reuses the split path rather
 // than a fused kernel of its own.
 // -------------------------------------------------------------------------
-struct HamOut {
-    double v;
+// physical_constraints (physcon.cpp:12-89): the generated constraint header
+// produces ham (output 0) and mom0..2 (outputs 1-3); with MOM false only ham
+// is kept and the compiler drops the momentum expressions.
+template <bool MOM>
+struct ConsOut {
+    double v[4];
     template <int V>
-    __device__ __forceinline__ void put(double x) { v = x; }
+    __device__ __forceinline__ void put(double x) {
+        if (V == 0 || MOM) v[V] = x;
+    }
 };
 
-__global__ __launch_bounds__(NTHREADS) void k_ham(VarPtrs u, double *__restrict__ ham,
-                                                 const DBlock *__restrict__ blocks,
-                                                 const Tile *__restrict__ tiles, const int *__restrict__ wg_tile,
-                                                 int tile_end, const double *__restrict__ W, int64_t P) {
+struct ConsPtrs {
+    double *c[4];  // ham, mom0, mom1, mom2 (any may be null)
+};
+
+template <bool MOM>
+__global__ __launch_bounds__(NTHREADS) void k_constraints(VarPtrs u, ConsPtrs o,
+                                                          const DBlock *__restrict__ blocks,
+                                                          const Tile *__restrict__ tiles,
+                                                          const int *__restrict__ wg_tile, int tile_end,
+                                                          const double *__restrict__ W, int64_t P) {
     const int64_t q = (int64_t)blockIdx.x * NTHREADS + threadIdx.x;
     if (q >= P) return;
     int t = wg_tile[blockIdx.x];
@@ -1093,9 +1105,14 @@ __global__ __launch_bounds__(NTHREADS) void k_ham(VarPtrs u, double *__restrict_
     const DBlock B = blocks[T.block];
     const PointGeom g = locate(T, B, (int)(q - T.q0));
     PointIn in{&u, g.idx, W, P, q};
-    HamOut out;
+    ConsOut<MOM> out;
     dgr::bssn_ham_point<double>(in, out);
-    ham[g.idx] = out.v;
+    if (o.c[0]) o.c[0][g.idx] = out.v[0];
+    if (MOM) {
+#pragma unroll
+        for (int i = 1; i < 4; i++)
+            if (o.c[i]) o.c[i][g.idx] = out.v[i];
+    }
 }
 
 // computeConstraintL2Norm (grUtils.tcc:37-134): per workgroup, the sum of c^2
@@ -2207,9 +2224,17 @@ int dgr_enforce_constraints(dgr_plan *plan, double *const *d_u, double chi_floor
     return DGR_OK;
 }
 
-int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, void *stream) {
+int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, const double *const *d_u, void *stream) {
     g_err.clear();
-    if (!plan || !d_ham) return fail(DGR_E_INVALID, "null plan or d_ham");
+    if (!plan) return fail(DGR_E_INVALID, "null plan");
+    ConsPtrs o{{d_ham, nullptr, nullptr, nullptr}};
+    bool mom = false;
+    if (d_mom)
+        for (int i = 0; i < 3; i++) {
+            o.c[1 + i] = d_mom[i];
+            mom = mom || d_mom[i];
+        }
+    if (!d_ham && !mom) return fail(DGR_E_INVALID, "no output: d_ham and every d_mom[i] are null");
     VarPtrs u;
     int rc;
     if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
@@ -2223,11 +2248,25 @@ int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, 
         const int ntile = c.tile_end - c.tile_begin;
         hipLaunchKernelGGL(k_derivs, dim3(ntile, NV), dim3(NTHREADS), 0, st, u, plan->d_blocks, plan->d_tiles,
                            c.tile_begin, plan->d_W, (int64_t)c.npoints);
-        hipLaunchKernelGGL(k_ham, dim3(c.nwg), dim3(NTHREADS), 0, st, u, d_ham, plan->d_blocks, plan->d_tiles,
-                           plan->d_wg_tile + c.wg_begin, c.tile_end, plan->d_W, (int64_t)c.npoints);
+        if (mom)
+            hipLaunchKernelGGL(k_constraints<true>, dim3(c.nwg), dim3(NTHREADS), 0, st, u, o, plan->d_blocks,
+                               plan->d_tiles, plan->d_wg_tile + c.wg_begin, c.tile_end, plan->d_W,
+                               (int64_t)c.npoints);
+        else
+            hipLaunchKernelGGL(k_constraints<false>, dim3(c.nwg), dim3(NTHREADS), 0, st, u, o, plan->d_blocks,
+                               plan->d_tiles, plan->d_wg_tile + c.wg_begin, c.tile_end, plan->d_W,
+                               (int64_t)c.npoints);
     }
     HIP_OK(hipGetLastError());
     return DGR_OK;
+}
+
+int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, void *stream) {
+    if (!d_ham) {
+        g_err.clear();
+        return fail(DGR_E_INVALID, "null plan or d_ham");
+    }
+    return dgr_constraints(plan, d_ham, nullptr, d_u, stream);
 }
 
 int dgr_constraint_l2_norm(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2,

@@ -22,7 +22,7 @@ SYMBOLS = [
     "dgr_plan_interior_points", "dgr_bssn_rhs", "dgr_rk4_step", "dgr_enforce_constraints",
     "dgr_plan_status", "dgr_plan_set_timing", "dgr_plan_kernel_times",
     "dgr_ham_constraint", "dgr_constraint_l2_norm", "dgr_rk4_steps", "dgr_plan_set_precision",
-    "dgr_plan_autotune", "dgr_plan_tiling", "dgr_plan_set_tiling",
+    "dgr_plan_autotune", "dgr_plan_tiling", "dgr_plan_set_tiling", "dgr_constraints",
 ]
 PRECISION = {"fp64": 0, "mixed": 1}
 TILINGS = ("wt", "w", "full")  # DGR_TILING_WT, _W, _FULL
@@ -77,6 +77,7 @@ def load():
         L.dgr_plan_set_precision.argtypes = [vp, C.c_int]
         L.dgr_rk4_steps.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), C.c_int, C.c_int, vp]
         L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
+        L.dgr_constraints.argtypes = [vp, vp, vp, vp, vp]
         L.dgr_plan_autotune.argtypes = [vp, vp, vp, C.POINTER(BssnParams), vp]
         L.dgr_plan_tiling.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]
         L.dgr_plan_set_tiling.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
@@ -163,6 +164,21 @@ class Plan:
         assert out.dim() == 1 and out.is_cuda and str(out.dtype) == "torch.float64"
         _check(load().dgr_ham_constraint(self._h, C.c_void_p(out.data_ptr()), var_pointers(state),
                                          _stream_ptr(stream)), "dgr_ham_constraint")
+
+    def constraints(self, ham, mom, state, stream=None):
+        """Hamiltonian constraint into `ham` and the three momentum constraints
+        into `mom` (a (3, szpdof) tensor or a sequence of three 1-D tensors,
+        entries may be None); either argument may be None."""
+        def ptr(t):
+            if t is None:
+                return None
+            assert t.dim() == 1 and t.is_cuda and str(t.dtype) == "torch.float64"
+            return t.data_ptr()
+        mp = None
+        if mom is not None:
+            mp = (C.c_void_p * 3)(*[ptr(mom[i]) for i in range(3)])
+        _check(load().dgr_constraints(self._h, C.c_void_p(ptr(ham)), mp, var_pointers(state),
+                                      _stream_ptr(stream)), "dgr_constraints")
 
     def constraint_norm(self, c, bh=((3.0, 0.0, 0.0), (-3.0, 0.0, 0.0)), r1: float = 5.0,
                         r2: float = 5.0, stream=None) -> float:

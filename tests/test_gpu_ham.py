@@ -1,6 +1,7 @@
-"""Hamiltonian constraint and its masked norm on the GPU (through the C-ABI)
-against the oracle: per point within 1e-12 normwise relative, the norm within
-1e-12 relative (the reference sums in another order)."""
+"""Hamiltonian and momentum constraints and the masked norm on the GPU
+(through the C-ABI) against the oracle: per point within 1e-12 normwise
+relative, the norm within 1e-12 relative (the reference sums in another
+order)."""
 import numpy as np
 import pytest
 
@@ -110,4 +111,74 @@ def test_ham_norm_unchanged_over_an_rk4_step():
     want = O.constraint_l2_norm(O.ham_batch(ref, t, nthreads=8), t, r1=1.0, r2=1.0)
     assert want > 0.0
     assert got == pytest.approx(want, rel=1e-9)
+    plan.close()
+
+
+def gpu_constraints(t, st, which=(True, True, True, True)):
+    import torch
+    plan = dgr.Plan(t, st.shape[1])
+    out = torch.full((4, st.shape[1]), float("nan"), dtype=torch.float64, device="cuda")
+    sel = [out[i] if w else None for i, w in enumerate(which)]
+    plan.constraints(sel[0], sel[1:], to_dev(st))
+    torch.cuda.synchronize()
+    plan.close()
+    return out.cpu().numpy()
+
+
+def check_constraints(got, t, st, scale_floor=0.0):
+    ham, mom = O.constraints_batch(st, t, nthreads=8)
+    m = ~padding_mask(t, st.shape[1])
+    for i, ref in enumerate([ham] + list(mom)):
+        err = np.abs(got[i][m] - ref[m]).max()
+        scale = max(np.abs(ref[m]).max(), scale_floor)
+        log(f"  output {i}: max err {err:.3e} scale {scale:.3e}")
+        assert scale > 0.0
+        assert err <= TOL * scale, i
+    assert np.isnan(got[:, ~m]).all(), "padding of the output was written"
+
+
+@pytest.mark.parametrize("bflag", [0, 63])
+def test_momentum_block19_matches_oracle(bflag):
+    """dgr_constraints (ham and mom0..2) on a perturbed flat block against the
+    oracle's physical_constraints restatement."""
+    log(f"momentum block19 bflag={bflag}")
+    t = W.single_block_19(bflag)
+    st = host_state(t, seed=2, amp=1e-2)
+    check_constraints(gpu_constraints(t, st), t, st)
+
+
+def test_momentum_uneven_puncture_blocks_match_oracle():
+    """Brill-Lindquist data with 1e-3 noise on every variable, so that At, K
+    and the metric carry momentum-constraint violations of their own; ham is
+    measured against max(|ham|, 1) as in the Hamiltonian-only test."""
+    log("momentum uneven puncture blocks")
+    t = uneven_sample()
+    t["bflag"][0] = 63
+    st = host_state(t, init="puncture", seed=6, noise=1e-3)
+    got = gpu_constraints(t, st)
+    ham, mom = O.constraints_batch(st, t, nthreads=8)
+    m = ~padding_mask(t, st.shape[1])
+    assert np.abs(got[0][m] - ham[m]).max() <= TOL * max(np.abs(ham[m]).max(), 1.0)
+    for i in range(3):
+        assert np.abs(got[1 + i][m] - mom[i][m]).max() <= TOL * np.abs(mom[i][m]).max(), i
+
+
+def test_constraints_skip_null_outputs():
+    import torch
+    log("constraints null outputs")
+    t = W.single_block_19(21)
+    st = host_state(t, seed=3, amp=1e-2)
+    got = gpu_constraints(t, st, which=(False, True, False, True))
+    assert np.isnan(got[0]).all() and np.isnan(got[2]).all()
+    full = gpu_constraints(t, st)
+    m = ~padding_mask(t, st.shape[1])
+    for i in (1, 3):
+        assert np.abs(got[i][m] - full[i][m]).max() <= TOL * np.abs(full[i][m]).max()
+    ham_only = gpu_ham(t, st)
+    assert np.abs(ham_only[m] - full[0][m]).max() <= TOL * np.abs(full[0][m]).max()
+    plan = dgr.Plan(t, st.shape[1])
+    with pytest.raises(RuntimeError):
+        plan.constraints(None, None, to_dev(st))
+    with pytest.raises(RuntimeError):
+        plan.constraints(None, [None, None, None], to_dev(st))
     plan.close()
