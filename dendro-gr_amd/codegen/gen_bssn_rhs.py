@@ -46,7 +46,8 @@ E_IJ = [(i, j) for i in E_I for j in E_I]
 fld = {n: sp.Symbol(f"f_{n}", real=True) for n in VARS}
 d1s = {(n, d): sp.Symbol(f"d1_{n}_{d}", real=True) for n in VARS for d in E_I}
 d2s = {(n, c): sp.Symbol(f"d2_{n}_{c}", real=True) for n in D2VARS for c in range(6)}
-INPUTS = set(fld.values()) | set(d1s.values()) | set(d2s.values())
+XYZ = [sp.Symbol(f"x_{i}", real=True) for i in range(3)]  # point coordinates (psi4 only)
+INPUTS = set(fld.values()) | set(d1s.values()) | set(d2s.values()) | set(XYZ)
 eta = sp.Symbol("eta", real=True)
 l1, l2, l3, l4 = sp.symbols("lam0 lam1 lam2 lam3", real=True)
 lf0, lf1 = sp.symbols("lamf0 lamf1", real=True)
@@ -207,6 +208,61 @@ MOM = [sum(igt[j, k] * (d(k, At_n[i][j]) - sum(C2[(m, k, i)] * At[j, m] for m in
        - Rational(3, 2) * sum(igt[j, k] * At[k, i] * d(j, "chi") for j, k in E_IJ) * ichi
        - Rational(2, 3) * d(i, "K") for i in E_I]
 
+
+def psi4_exprs():
+    """psi4 (CodeGen/constraints.py:42-244): Gram-Schmidt in the physical
+    metric gt/chi on r = (x,y,z), theta = (xz, yz, -(x^2+y^2)) and
+    phi = (-y, x, 0) gives the tetrad r, m = (theta + i phi)/sqrt(2). Every
+    term is quadratic in m, so the two factors 1/sqrt(2) are applied as one
+    1/2. Returns [real, imaginary] with the sign convention of
+    constraints.py:238-244. The intermediates are created here (not at import)
+    so that the right-hand side's definition list stays without them."""
+    x, y, z = XYZ
+    half = Rational(1, 2)
+
+    def gdot(u, v):
+        return ichi * sum(gt[i, j] * u[i] * v[j] for i, j in E_IJ)
+
+    r0 = [x, y, z]
+    nr = let("ps_nr", 1 / sp.sqrt(gdot(r0, r0)))
+    r = [let(f"ps_r{i}", r0[i] * nr) for i in E_I]
+    th0 = [x * z, y * z, -(x * x + y * y)]
+    t1, t2 = let("ps_t1", gdot(th0, th0)), let("ps_t2", gdot(r, th0))
+    nth = let("ps_nth", 1 / sp.sqrt(t1 - t2 * t2))
+    th = [let(f"ps_th{i}", (th0[i] - t2 * r[i]) * nth) for i in E_I]
+    ph0 = [-y, x, sp.Integer(0)]
+    p1, p2, p3 = let("ps_p1", gdot(ph0, ph0)), let("ps_p2", gdot(r, ph0)), let("ps_p3", gdot(th, ph0))
+    nph = let("ps_nph", 1 / sp.sqrt(p1 - p2 * p2 - p3 * p3))
+    ph = [let(f"ps_ph{i}", (ph0[i] - p2 * r[i] - p3 * th[i]) * nph) for i in E_I]
+    # 2 MM_ij, 2 NN_ij (symmetric); sqrt(2) MR_ij, sqrt(2) NR_ij (antisymmetric)
+    MM = {(i, j): th[i] * th[j] - ph[i] * ph[j] for i, j in E_IJ}
+    NN = {(i, j): th[i] * ph[j] + th[j] * ph[i] for i, j in E_IJ}
+    MR = {(i, j): th[i] * r[j] - th[j] * r[i] for i, j in E_IJ}
+    NR = {(i, j): ph[i] * r[j] - ph[j] * r[i] for i, j in E_IJ}
+    Av = [let(f"ps_A{i}", sum(At[i, j] * r[j] for j in E_I)) for i in E_I]
+    # d_j At_ki + C2^m_ki At_mj, contracted with sqrt(2) m_real (U) and sqrt(2) m_imag (V)
+    Wt = {(k, i, j): let(f"ps_W{k}{i}{j}", d(j, At_n[k][i]) + sum(C2[(m, k, i)] * At[m, j] for m in E_I))
+          for k in E_I for i, j in E_IJ}
+    U = {(i, j): sum(th[k] * Wt[(k, i, j)] for k in E_I) for i, j in E_IJ}
+    V = {(i, j): sum(ph[k] * Wt[(k, i, j)] for k in E_I) for i, j in E_IJ}
+    rdchi = sum(r[i] * d(i, "chi") for i in E_I)
+    Atmp = let("ps_Atmp", ichi * ichi * (sum(Av[i] * r[i] for i in E_I) + K * chi * Rational(1, 3)
+                                         + half * rdchi))
+    p1r = half * sum(R[i, j] * MM[(i, j)] for i, j in E_IJ)
+    p1i = half * sum(R[i, j] * NN[(i, j)] for i, j in E_IJ)
+    p2r = Atmp * half * sum(At[i, j] * MM[(i, j)] for i, j in E_IJ)
+    p2i = Atmp * half * sum(At[i, j] * NN[(i, j)] for i, j in E_IJ)
+    p3r = half * ichi * sum(MR[(i, j)] * U[(i, j)] - NR[(i, j)] * V[(i, j)] for i, j in E_IJ)
+    p3i = half * ichi * sum(NR[(i, j)] * U[(i, j)] + MR[(i, j)] * V[(i, j)] for i, j in E_IJ)
+    tA = let("ps_tA", sum(th[i] * Av[i] for i in E_I))
+    pA = let("ps_pA", sum(ph[i] * Av[i] for i in E_I))
+    td = sum(th[i] * d(i, "chi") for i in E_I)
+    pd = sum(ph[i] * d(i, "chi") for i in E_I)
+    p4r = half * ichi * ichi * (tA * (tA + half * td) - pA * (pA + half * pd))
+    p4i = half * ichi * ichi * (tA * (pA + half * pd) + pA * (tA + half * td))
+    return [-(p1r + p2r - p3r - p4r), p1i + p2i - p3i - p4i]
+
+
 # ---------------------------------------------------------------------------
 # code emission
 
@@ -220,6 +276,8 @@ def input_expr(sym: sp.Symbol) -> str:
     if n.startswith("d2_"):
         var, c = n[3:].rsplit("_", 1)
         return f"in.template d2<{D2IDX[var]}, {c}>()"
+    if n.startswith("x_"):
+        return f"in.template crd<{n[2:]}>()"
     raise KeyError(n)
 
 
@@ -253,6 +311,10 @@ class Printer(C99CodePrinter):
         bs = self.parenthesize(base, PRECEDENCE["Pow"])
         if ex == -1:
             return f"(T(1.0)/{bs})"
+        if ex == Rational(1, 2):
+            return f"sqrt({self._print(base)})"
+        if ex == Rational(-1, 2):
+            return f"(T(1.0)/sqrt({self._print(base)}))"
         if ex.is_Integer and 2 <= int(ex) <= 4:
             return "(" + "*".join([bs] * int(ex)) + ")"
         if ex.is_Integer and -4 <= int(ex) <= -2:
@@ -616,23 +678,28 @@ DGR_HD DGR_INLINE void bssn_rhs_phased(Ctx &ctx, const T eta, const T *lam, cons
 
 def generate_ham(path):
     t0 = time.time()
-    defs, outs_r, ops_after = eliminate([HAM] + MOM)
+    defs, outs_r, ops_after = eliminate([HAM] + MOM + psi4_exprs())
     order = schedule(defs, outs_r)
-    lines, ntmp, max_live = emit(order, defs, outs_r, input_expr, out_names=["ham", "mom0", "mom1", "mom2"])
+    lines, ntmp, max_live = emit(order, defs, outs_r, input_expr,
+                                 out_names=["ham", "mom0", "mom1", "mom2", "psi4_real", "psi4_img"])
     hdr = """// Generated by dendro-gr_amd/codegen/gen_bssn_rhs.py --ham -- do not edit.
 //
 // Pointwise constraints of the BSSN variables (CodeGen/constraints.py:250-282,
 // which the reference expands into BSSN_GR/src/physconeqs.cpp): the
 // Hamiltonian constraint chi igt^jk R_jk - At_ij At^ij + 2/3 K^2 and the
 // momentum constraints igt^jk (d_k At_ij - C2^m_ki At_jm) - Gt^j At_ij
-// - 3/2 igt^jk At_ki d_j chi / chi - 2/3 d_i K, built from the same named
-// intermediates (inverse metric, Christoffel symbols, Ricci tensor, raised At)
-// as the right-hand side.
+// - 3/2 igt^jk At_ki d_j chi / chi - 2/3 d_i K and the Weyl scalar psi4
+// (CodeGen/constraints.py:42-244) in the tetrad built from the point's
+// coordinates, from the same named intermediates (inverse metric, Christoffel
+// symbols, Ricci tensor, raised At) as the right-hand side.
 """ + f"""// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values live.
 //
 // In: the accessor of bssn_rhs_point (fields, first derivatives of chi, K,
-//     Gt, gt, At; second derivatives of chi and gt are the ones read).
-// Out: receives put<0>(ham), put<1..3>(mom0..2).
+//     Gt, gt, At; second derivatives of chi and gt are the ones read) plus
+//     crd<0..2>(), the point's coordinates (read by psi4 only).
+// Out: receives put<0>(ham), put<1..3>(mom0..2), put<4>(psi4 real),
+//     put<5>(psi4 imaginary). psi4 is singular on the z axis; the caller
+//     replaces it there (physcon.cpp:82-85).
 #pragma once
 
 #ifndef DGR_HD

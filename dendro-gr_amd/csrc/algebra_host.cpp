@@ -1,15 +1,19 @@
-// Host build of the generated pointwise BSSN equations and Hamiltonian
-// constraint, for checking the
+// Host build of the generated pointwise BSSN equations and constraints
+// (Hamiltonian, momentum, psi4), for checking the
 // generated code itself on the CPU against the reference's golden vectors.
 // Not part of the device path. Input layout per point: 163 doubles
 // ([0,24) fields, [24,96) first derivatives 3*v+d, [96,162) second
 // derivatives 6*m+c of alpha, chi, beta0..2, gt0..5, [162] eta).
+#include <cmath>
 #include <cstdint>
 
 #include "bssn_rhs_generated.h"
 #include "bssn_ham_generated.h"
 
 namespace {
+// Coordinates for the callers without any: the psi4 expressions are still
+// evaluated (sqrt keeps them alive on the host) and must read valid memory.
+constexpr double kNoCoords[3] = {1.0, 2.0, 3.0};
 template <typename T>
 struct In163 {
     const double *p;
@@ -19,6 +23,9 @@ struct In163 {
     T d1() const { return (T)p[24 + 3 * V + D]; }
     template <int M, int C>
     T d2() const { return (T)p[96 + 6 * M + C]; }
+    const double *xyz = kNoCoords;  // coordinates, read by psi4 only
+    template <int D>
+    T crd() const { return (T)xyz[D]; }
 };
 template <typename T>
 struct Out24 {
@@ -45,7 +52,7 @@ struct Out1 {  // keeps the Hamiltonian constraint (output 0) only
     }
 };
 template <typename T>
-struct Out4 {  // ham, mom0, mom1, mom2
+struct Out6 {  // ham, mom0, mom1, mom2, psi4 real, psi4 imaginary
     double *o;
     template <int V>
     void put(T x) { o[V] = (double)x; }
@@ -60,10 +67,12 @@ extern "C" void dgr_host_ham_points(const double *in, int64_t n, double *out) {
     }
 }
 
-extern "C" void dgr_host_constraint_points(const double *in, int64_t n, double *out) {
+// Six outputs per point (ham, mom0..2, psi4 real and imaginary) at the
+// coordinates xyz (three per point).
+extern "C" void dgr_host_constraint_points(const double *in, const double *xyz, int64_t n, double *out) {
     for (int64_t q = 0; q < n; q++) {
-        In163<double> a{in + 163 * q};
-        Out4<double> b{out + 4 * q};
+        In163<double> a{in + 163 * q, xyz + 3 * q};
+        Out6<double> b{out + 6 * q};
         dgr::bssn_ham_point<double>(a, b);
     }
 }

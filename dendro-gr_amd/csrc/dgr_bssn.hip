@@ -224,6 +224,9 @@ struct PointIn {
     __device__ __forceinline__ double d1() const { return W[(int64_t)(3 * V + D) * P + q]; }
     template <int M, int C>
     __device__ __forceinline__ double d2() const { return W[(int64_t)(SLOT_D2 + 6 * M + C) * P + q]; }
+    double x[3];  // coordinates (read by the psi4 expressions only)
+    template <int D>
+    __device__ __forceinline__ double crd() const { return x[D]; }
 };
 
 enum { MODE_RHS = 0, MODE_RK = 1 };
@@ -1076,22 +1079,27 @@ __global__ __launch_bounds__(256) void k_pad_copy(VarPtrs dst, VarPtrs src, cons
 // than a fused kernel of its own.
 // -------------------------------------------------------------------------
 // physical_constraints (physcon.cpp:12-89): the generated constraint header
-// produces ham (output 0) and mom0..2 (outputs 1-3); with MOM false only ham
-// is kept and the compiler drops the momentum expressions.
-template <bool MOM>
+// produces ham (output 0), mom0..2 (outputs 1-3) and psi4 real/imaginary
+// (outputs 4-5). MASK selects the groups kept (CONS_HAM, CONS_MOM, CONS_PSI4);
+// the compiler drops the expressions of the others.
+enum { CONS_HAM = 1, CONS_MOM = 2, CONS_PSI4 = 4 };
+
+template <int MASK>
 struct ConsOut {
-    double v[4];
+    double v[6];
     template <int V>
     __device__ __forceinline__ void put(double x) {
-        if (V == 0 || MOM) v[V] = x;
+        if ((V == 0 && (MASK & CONS_HAM)) || (V >= 1 && V <= 3 && (MASK & CONS_MOM)) ||
+            (V >= 4 && (MASK & CONS_PSI4)))
+            v[V] = x;
     }
 };
 
 struct ConsPtrs {
-    double *c[4];  // ham, mom0, mom1, mom2 (any may be null)
+    double *c[6];  // ham, mom0, mom1, mom2, psi4 real, psi4 imaginary (any may be null)
 };
 
-template <bool MOM>
+template <int MASK>
 __global__ __launch_bounds__(NTHREADS) void k_constraints(VarPtrs u, ConsPtrs o,
                                                           const DBlock *__restrict__ blocks,
                                                           const Tile *__restrict__ tiles,
@@ -1104,14 +1112,26 @@ __global__ __launch_bounds__(NTHREADS) void k_constraints(VarPtrs u, ConsPtrs o,
     const Tile T = tiles[t];
     const DBlock B = blocks[T.block];
     const PointGeom g = locate(T, B, (int)(q - T.q0));
-    PointIn in{&u, g.idx, W, P, q};
-    ConsOut<MOM> out;
+    // x = pmin + i h as physcon.cpp:75-77 computes it
+    PointIn in{&u, g.idx, W, P, q,
+               {__dadd_rn(B.pmin[0], __dmul_rn((double)g.i, B.h[0])),
+                __dadd_rn(B.pmin[1], __dmul_rn((double)g.j, B.h[1])),
+                __dadd_rn(B.pmin[2], __dmul_rn((double)g.k, B.h[2]))}};
+    ConsOut<MASK> out;
     dgr::bssn_ham_point<double>(in, out);
-    if (o.c[0]) o.c[0][g.idx] = out.v[0];
-    if (MOM) {
+    if (MASK & CONS_HAM)
+        if (o.c[0]) o.c[0][g.idx] = out.v[0];
+    if (MASK & CONS_MOM) {
 #pragma unroll
         for (int i = 1; i < 4; i++)
             if (o.c[i]) o.c[i][g.idx] = out.v[i];
+    }
+    if (MASK & CONS_PSI4) {
+        // psi4 is set to zero within 1e-7 of the z axis (physcon.cpp:82-85)
+        const bool axis = fabs(in.x[0]) <= 1e-7 && fabs(in.x[1]) <= 1e-7;
+#pragma unroll
+        for (int i = 4; i < 6; i++)
+            if (o.c[i]) o.c[i][g.idx] = axis ? 0.0 : out.v[i];
     }
 }
 
@@ -2224,17 +2244,19 @@ int dgr_enforce_constraints(dgr_plan *plan, double *const *d_u, double chi_floor
     return DGR_OK;
 }
 
-int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, const double *const *d_u, void *stream) {
+int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, double *const *d_psi4,
+                    const double *const *d_u, void *stream) {
     g_err.clear();
     if (!plan) return fail(DGR_E_INVALID, "null plan");
-    ConsPtrs o{{d_ham, nullptr, nullptr, nullptr}};
-    bool mom = false;
+    ConsPtrs o{{d_ham, nullptr, nullptr, nullptr, nullptr, nullptr}};
+    int mask = d_ham ? CONS_HAM : 0;
     if (d_mom)
-        for (int i = 0; i < 3; i++) {
-            o.c[1 + i] = d_mom[i];
-            mom = mom || d_mom[i];
-        }
-    if (!d_ham && !mom) return fail(DGR_E_INVALID, "no output: d_ham and every d_mom[i] are null");
+        for (int i = 0; i < 3; i++)
+            if ((o.c[1 + i] = d_mom[i])) mask |= CONS_MOM;
+    if (d_psi4)
+        for (int i = 0; i < 2; i++)
+            if ((o.c[4 + i] = d_psi4[i])) mask |= CONS_PSI4;
+    if (!mask) return fail(DGR_E_INVALID, "no output: d_ham, every d_mom[i] and every d_psi4[i] are null");
     VarPtrs u;
     int rc;
     if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
@@ -2243,19 +2265,18 @@ int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, const d
         hipError_t e = hipMalloc(&plan->d_W, sizeof(double) * NSLOT * (size_t)plan->max_chunk_points);
         if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "derivative workspace");
     }
+    // the smallest instantiated superset of the requested groups
+    void (*kern)(VarPtrs, ConsPtrs, const DBlock *, const Tile *, const int *, int, const double *, int64_t) =
+        (mask & CONS_PSI4) ? k_constraints<CONS_HAM | CONS_MOM | CONS_PSI4>
+        : (mask & CONS_MOM) ? k_constraints<CONS_HAM | CONS_MOM>
+                            : k_constraints<CONS_HAM>;
     hipStream_t st = (hipStream_t)stream;
     for (const Chunk &c : plan->chunks) {
         const int ntile = c.tile_end - c.tile_begin;
         hipLaunchKernelGGL(k_derivs, dim3(ntile, NV), dim3(NTHREADS), 0, st, u, plan->d_blocks, plan->d_tiles,
                            c.tile_begin, plan->d_W, (int64_t)c.npoints);
-        if (mom)
-            hipLaunchKernelGGL(k_constraints<true>, dim3(c.nwg), dim3(NTHREADS), 0, st, u, o, plan->d_blocks,
-                               plan->d_tiles, plan->d_wg_tile + c.wg_begin, c.tile_end, plan->d_W,
-                               (int64_t)c.npoints);
-        else
-            hipLaunchKernelGGL(k_constraints<false>, dim3(c.nwg), dim3(NTHREADS), 0, st, u, o, plan->d_blocks,
-                               plan->d_tiles, plan->d_wg_tile + c.wg_begin, c.tile_end, plan->d_W,
-                               (int64_t)c.npoints);
+        hipLaunchKernelGGL(kern, dim3(c.nwg), dim3(NTHREADS), 0, st, u, o, plan->d_blocks, plan->d_tiles,
+                           plan->d_wg_tile + c.wg_begin, c.tile_end, plan->d_W, (int64_t)c.npoints);
     }
     HIP_OK(hipGetLastError());
     return DGR_OK;
@@ -2266,7 +2287,7 @@ int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, 
         g_err.clear();
         return fail(DGR_E_INVALID, "null plan or d_ham");
     }
-    return dgr_constraints(plan, d_ham, nullptr, d_u, stream);
+    return dgr_constraints(plan, d_ham, nullptr, nullptr, d_u, stream);
 }
 
 int dgr_constraint_l2_norm(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2,

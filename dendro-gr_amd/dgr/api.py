@@ -77,7 +77,7 @@ def load():
         L.dgr_plan_set_precision.argtypes = [vp, C.c_int]
         L.dgr_rk4_steps.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), C.c_int, C.c_int, vp]
         L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
-        L.dgr_constraints.argtypes = [vp, vp, vp, vp, vp]
+        L.dgr_constraints.argtypes = [vp, vp, vp, vp, vp, vp]
         L.dgr_plan_autotune.argtypes = [vp, vp, vp, C.POINTER(BssnParams), vp]
         L.dgr_plan_tiling.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]
         L.dgr_plan_set_tiling.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
@@ -165,19 +165,22 @@ class Plan:
         _check(load().dgr_ham_constraint(self._h, C.c_void_p(out.data_ptr()), var_pointers(state),
                                          _stream_ptr(stream)), "dgr_ham_constraint")
 
-    def constraints(self, ham, mom, state, stream=None):
-        """Hamiltonian constraint into `ham` and the three momentum constraints
+    def constraints(self, ham, mom, state, psi4=None, stream=None):
+        """Hamiltonian constraint into `ham`, the three momentum constraints
         into `mom` (a (3, szpdof) tensor or a sequence of three 1-D tensors,
-        entries may be None); either argument may be None."""
+        entries may be None) and psi4 (real, imaginary) into `psi4` (likewise,
+        two entries); each argument may be None."""
         def ptr(t):
             if t is None:
                 return None
             assert t.dim() == 1 and t.is_cuda and str(t.dtype) == "torch.float64"
             return t.data_ptr()
-        mp = None
+        mp = pp = None
         if mom is not None:
             mp = (C.c_void_p * 3)(*[ptr(mom[i]) for i in range(3)])
-        _check(load().dgr_constraints(self._h, C.c_void_p(ptr(ham)), mp, var_pointers(state),
+        if psi4 is not None:
+            pp = (C.c_void_p * 2)(*[ptr(psi4[i]) for i in range(2)])
+        _check(load().dgr_constraints(self._h, C.c_void_p(ptr(ham)), mp, pp, var_pointers(state),
                                       _stream_ptr(stream)), "dgr_constraints")
 
     def constraint_norm(self, c, bh=((3.0, 0.0, 0.0), (-3.0, 0.0, 0.0)), r1: float = 5.0,

@@ -208,17 +208,23 @@ int dgr_enforce_constraints(dgr_plan *plan, double *const *d_u, double chi_floor
  * `stream`; uses the plan's derivative workspace (allocated on first use). */
 int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, void *stream);
 
-/* The Hamiltonian and momentum constraints at the interior points of every
- * block: d_ham as dgr_ham_constraint, d_mom[i] (i = 0,1,2) the momentum
- * constraint igt^jk (D_k At_ij - C2^m_ki At_jm) - Gt^j At_ij
- * - 3/2 igt^jk At_ki d_j chi / chi - 2/3 d_i K, each laid out like a variable
- * (padding not written). Replaces the C_HAM and C_MOM outputs of
- * physical_constraints (BSSN_GR/src/physcon.cpp:12-89; equations from
- * CodeGen/constraints.py:256-282, generated into physconeqs.cpp). d_mom is a
- * host array of 3 device pointers or NULL; d_ham and any d_mom[i] may be NULL
- * (that output is skipped), but not all four. Asynchronous on `stream`; uses
- * the plan's derivative workspace (allocated on first use). */
-int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, const double *const *d_u, void *stream);
+/* physical_constraints (BSSN_GR/src/physcon.cpp:12-89) at the interior points
+ * of every block, each output laid out like a variable (padding not written):
+ *   d_ham      the Hamiltonian constraint, as dgr_ham_constraint;
+ *   d_mom[i]   (i = 0,1,2) the momentum constraint igt^jk (D_k At_ij
+ *              - C2^m_ki At_jm) - Gt^j At_ij - 3/2 igt^jk At_ki d_j chi / chi
+ *              - 2/3 d_i K (C_MOM);
+ *   d_psi4[0], d_psi4[1]  real and imaginary part of the Weyl scalar psi4 in
+ *              the Gram-Schmidt tetrad of (x,y,z) at x = pmin + i*h
+ *              (C_PSI4_REAL, C_PSI4_IMG), zero within 1e-7 of the z axis as
+ *              physcon.cpp:82-85.
+ * Equations from CodeGen/constraints.py:42-282 (generated into physconeqs.cpp).
+ * d_mom and d_psi4 are host arrays of 3 and 2 device pointers, or NULL; d_ham
+ * and any entry may be NULL (that output is skipped), but not all six.
+ * Asynchronous on `stream`; uses the plan's derivative workspace (allocated on
+ * first use). */
+int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, double *const *d_psi4,
+                    const double *const *d_u, void *stream);
 
 /* computeConstraintL2Norm (BSSN_GR/include/grUtils.tcc:37-134): sqrt of the mean
  * of c^2 over the interior points whose distance to (bh[0],bh[1],bh[2]) is >= r1

@@ -215,9 +215,11 @@ static int d2slot(int v) {
 }
 
 /* The pointwise algebra shared by the right-hand side (rhs != NULL, needs p),
- * the Hamiltonian constraint (ham != NULL) and the momentum constraints
- * (mom != NULL, three values). */
-static void point_algebra(const double *in, const orc_params *p, double *rhs, double *ham, double *mom) {
+ * the Hamiltonian constraint (ham != NULL), the momentum constraints
+ * (mom != NULL, three values) and psi4 (psi4 != NULL: real and imaginary
+ * part, at the coordinates xyz). */
+static void point_algebra(const double *in, const orc_params *p, double *rhs, double *ham, double *mom,
+                          const double *xyz, double *psi4) {
     const double *f = in;
 #define D1(v, d) in[24 + 3 * (v) + (d)]
 #define D2(v, a, b) in[96 + 6 * d2slot(v) + SYM6[a][b]]
@@ -397,6 +399,92 @@ static void point_algebra(const double *in, const orc_params *p, double *rhs, do
             mom[i] = s1 - s2 - 1.5 * s3 - 2.0 / 3.0 * dK[i];
         }
     }
+    /* psi4 (CodeGen/constraints.py:42-244): the tetrad from Gram-Schmidt on
+     * r = (x,y,z), theta = (xz, yz, -(x^2+y^2)), phi = (-y, x, 0) in the
+     * physical metric gt/chi; m = (theta + i phi)/sqrt(2); the real part
+     * carries the LazEv sign convention of constraints.py:238-244. */
+    if (psi4) {
+        const double x = xyz[0], y = xyz[1], z = xyz[2];
+        double gd[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) gd[i][j] = g[i][j] / chi;
+#define GDOT(u, v) (gd[0][0] * (u)[0] * (v)[0] + gd[0][1] * (u)[0] * (v)[1] + gd[0][2] * (u)[0] * (v)[2] + \
+                    gd[1][0] * (u)[1] * (v)[0] + gd[1][1] * (u)[1] * (v)[1] + gd[1][2] * (u)[1] * (v)[2] + \
+                    gd[2][0] * (u)[2] * (v)[0] + gd[2][1] * (u)[2] * (v)[1] + gd[2][2] * (u)[2] * (v)[2])
+        double rv[3] = {x, y, z};
+        double th[3] = {x * z, y * z, -(x * x + y * y)};
+        double ph[3] = {-y, x, 0.0};
+        const double nr = sqrt(GDOT(rv, rv));
+        for (int i = 0; i < 3; i++) rv[i] /= nr;
+        {
+            const double i1 = GDOT(th, th), i2 = GDOT(rv, th);
+            for (int i = 0; i < 3; i++) th[i] -= i2 * rv[i];
+            const double n = sqrt(i1 - i2 * i2);
+            for (int i = 0; i < 3; i++) th[i] /= n;
+        }
+        {
+            const double i1 = GDOT(ph, ph), i2 = GDOT(rv, ph), i3 = GDOT(th, ph);
+            for (int i = 0; i < 3; i++) ph[i] -= i2 * rv[i] + i3 * th[i];
+            const double n = sqrt(i1 - i2 * i2 - i3 * i3);
+            for (int i = 0; i < 3; i++) ph[i] /= n;
+        }
+#undef GDOT
+        const double is2 = 1.0 / sqrt(2.0);
+        double mr[3], mi[3];
+        for (int i = 0; i < 3; i++) {
+            mr[i] = th[i] * is2;
+            mi[i] = ph[i] * is2;
+        }
+        double MM[3][3], NN[3][3], MR[3][3], NR[3][3], Uu[3][3], Vv[3][3], Av[3];
+        for (int i = 0; i < 3; i++) {
+            Av[i] = 0.0;
+            for (int j = 0; j < 3; j++) Av[i] += A[i][j] * rv[j];
+            for (int j = 0; j < 3; j++) {
+                MM[i][j] = mr[i] * mr[j] - mi[i] * mi[j];
+                NN[i][j] = mr[i] * mi[j] + mr[j] * mi[i];
+                MR[i][j] = mr[i] * rv[j] - mr[j] * rv[i];
+                NR[i][j] = mi[i] * rv[j] - mi[j] * rv[i];
+                /* U_ij = m_k (d_j At_ki + C2^m_ki At_mj) */
+                double u = 0.0, v = 0.0;
+                for (int k = 0; k < 3; k++) {
+                    double t = dA[j][k][i];
+                    for (int m = 0; m < 3; m++) t += C2[m][k][i] * A[m][j];
+                    u += mr[k] * t;
+                    v += mi[k] * t;
+                }
+                Uu[i][j] = u;
+                Vv[i][j] = v;
+            }
+        }
+        double rA = 0.0, rdchi = 0.0, mrd = 0.0, mid = 0.0, mrA = 0.0, miA = 0.0;
+        for (int i = 0; i < 3; i++) {
+            rA += Av[i] * rv[i];
+            rdchi += rv[i] * dchi[i];
+            mrd += mr[i] * dchi[i];
+            mid += mi[i] * dchi[i];
+            mrA += mr[i] * Av[i];
+            miA += mi[i] * Av[i];
+        }
+        const double Atmp = (rA + K * chi / 3.0 + 0.5 * rdchi) / (chi * chi);
+        double p1r = 0.0, p1i = 0.0, p2r = 0.0, p2i = 0.0, p3r = 0.0, p3i = 0.0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                p1r += R[i][j] * MM[i][j];
+                p1i += R[i][j] * NN[i][j];
+                p2r += A[i][j] * MM[i][j];
+                p2i += A[i][j] * NN[i][j];
+                p3r += MR[i][j] * Uu[i][j] - NR[i][j] * Vv[i][j];
+                p3i += NR[i][j] * Uu[i][j] + MR[i][j] * Vv[i][j];
+            }
+        p2r *= Atmp;
+        p2i *= Atmp;
+        p3r /= chi;
+        p3i /= chi;
+        const double p4r = (mrA * (mrA + 0.5 * mrd) - miA * (miA + 0.5 * mid)) / (chi * chi);
+        const double p4i = (mrA * (miA + 0.5 * mid) + miA * (mrA + 0.5 * mrd)) / (chi * chi);
+        psi4[0] = -(p1r + p2r - p3r - p4r);
+        psi4[1] = p1i + p2i - p3i - p4i;
+    }
     if (!rhs) return;
 
     double divb = db[0][0] + db[1][1] + db[2][2];
@@ -469,11 +557,23 @@ static void point_algebra(const double *in, const orc_params *p, double *rhs, do
     }
 }
 
-void orc_bssn_point(const double *in, const orc_params *p, double *rhs) { point_algebra(in, p, rhs, NULL, NULL); }
+void orc_bssn_point(const double *in, const orc_params *p, double *rhs) { point_algebra(in, p, rhs, NULL, NULL, NULL, NULL); }
 
-void orc_ham_point(const double *in, double *ham) { point_algebra(in, NULL, NULL, ham, NULL); }
+void orc_ham_point(const double *in, double *ham) { point_algebra(in, NULL, NULL, ham, NULL, NULL, NULL); }
 
-void orc_constraint_point(const double *in, double *ham, double *mom) { point_algebra(in, NULL, NULL, ham, mom); }
+void orc_constraint_point(const double *in, double *ham, double *mom) {
+    point_algebra(in, NULL, NULL, ham, mom, NULL, NULL);
+}
+
+void orc_constraint_point_xyz(const double *in, const double *xyz, double *ham, double *mom, double *psi4) {
+    point_algebra(in, NULL, NULL, ham, mom, xyz, psi4);
+}
+
+void orc_constraint_points_xyz(const double *in, const double *xyz, int64_t npts, double *ham, double *mom,
+                               double *psi4) {
+    for (int64_t q = 0; q < npts; q++)
+        orc_constraint_point_xyz(in + q * ORC_POINT_NIN, xyz + 3 * q, ham + q, mom + 3 * q, psi4 + 2 * q);
+}
 
 void orc_constraint_points(const double *in, int64_t npts, double *ham, double *mom) {
     for (int64_t q = 0; q < npts; q++) orc_constraint_point(in + q * ORC_POINT_NIN, ham + q, mom + 3 * q);
@@ -688,16 +788,19 @@ int orc_bssnrhs_batch(double *const *rhs, const double *const *u, const orc_bloc
 /* Hamiltonian constraint per block: physical_constraints, physcon.cpp:12-89  */
 /* (C_HAM only; interior points, padding of `ham` untouched)                  */
 /* ======================================================================== */
-static int constraint_block(double *ham_all, double *const *mom_all, const double *const *u_all,
-                            const orc_block *blk);
+static int constraint_block(double *ham_all, double *const *mom_all, double *const *psi4_all,
+                            const double *const *u_all, const orc_block *blk);
 
 int orc_ham_block(double *ham_all, const double *const *u_all, const orc_block *blk) {
-    return constraint_block(ham_all, NULL, u_all, blk);
+    return constraint_block(ham_all, NULL, NULL, u_all, blk);
 }
 
-/* C_HAM and (mom_all != NULL: three arrays laid out like a variable) C_MOM */
-static int constraint_block(double *ham_all, double *const *mom_all, const double *const *u_all,
-                            const orc_block *blk) {
+/* physical_constraints (physcon.cpp:12-89): C_HAM, with mom_all != NULL
+ * (three arrays laid out like a variable) C_MOM, with psi4_all != NULL (two
+ * arrays) psi4 real and imaginary part at x = pmin + i*h (physcon.cpp:75-77),
+ * zero within 1e-7 of the z axis (physcon.cpp:82-85). */
+static int constraint_block(double *ham_all, double *const *mom_all, double *const *psi4_all,
+                            const double *const *u_all, const orc_block *blk) {
     const uint32_t *sz = blk->sz;
     const int nx = (int)sz[0], ny = (int)sz[1], nz = (int)sz[2];
     if (nx < 13 || ny < 13 || nz < 13) return -1;
@@ -725,22 +828,27 @@ static int constraint_block(double *ham_all, double *const *mom_all, const doubl
                 for (int m = 0; m < 11; m++)
                     for (int c = 0; c < 6; c++) in[96 + 6 * m + c] = d2[m][c][pp];
                 in[162] = 0.0;
-                double m[3];
-                orc_constraint_point(in, &ham[pp], m);
+                double m[3], ps[2];
+                const double xyz[3] = {blk->pmin[0] + i * hx, blk->pmin[1] + j * hy, blk->pmin[2] + k * hz};
+                orc_constraint_point_xyz(in, xyz, &ham[pp], m, psi4_all ? ps : NULL);
                 if (mom_all)
                     for (int c = 0; c < 3; c++) mom_all[c][blk->offset + pp] = m[c];
+                if (psi4_all) {
+                    if (fabs(xyz[0]) <= 1e-7 && fabs(xyz[1]) <= 1e-7) ps[0] = ps[1] = 0.0;
+                    for (int c = 0; c < 2; c++) psi4_all[c][blk->offset + pp] = ps[c];
+                }
             }
     return 0;
 }
 
-int orc_constraints_batch(double *ham, double *const *mom, const double *const *u, const orc_block *blks,
-                          int64_t nblocks, int nthreads) {
+int orc_constraints_batch(double *ham, double *const *mom, double *const *psi4, const double *const *u,
+                          const orc_block *blks, int64_t nblocks, int nthreads) {
     int err = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
 #endif
-    for (int64_t b = 0; b < nblocks; b++) err |= (constraint_block(ham, mom, u, &blks[b]) != 0);
+    for (int64_t b = 0; b < nblocks; b++) err |= (constraint_block(ham, mom, psi4, u, &blks[b]) != 0);
     (void)nthreads;
     return err ? -1 : 0;
 }

@@ -25,6 +25,12 @@
  *                                       Minkowski => 0; Brill-Lindquist data
  *                                       (an exact vacuum solution) => 0 up to
  *                                       truncation, converging at 4th order
+ *   orc_constraint_point_xyz (mom)      golden vectors (same build); Minkowski
+ *                                       => 0; closed forms on linear data
+ *   orc_constraint_point_xyz (psi4)     golden vectors (same build, per-point
+ *                                       coordinates); Minkowski => 0; a pure
+ *                                       At_xx / R_xx probe against the tetrad
+ *                                       closed form on the flat metric
  *   orc_constraint_l2_norm              brute-force mask/mean in the tests
  *   orc_enforce_constraints             det(gt)=1, tr(At)=0, idempotence
  *   orc_rk4_step                        exact on polynomial-in-t RHS (closed
@@ -108,14 +114,20 @@ void orc_ham_points(const double *in, int64_t npts, double *ham);
  * of one point / npts points (mom: three per point). */
 void orc_constraint_point(const double *in, double *ham, double *mom);
 void orc_constraint_points(const double *in, int64_t npts, double *ham, double *mom);
+/* The same plus psi4 (CodeGen/constraints.py:42-244; psi4: real, imaginary)
+ * at the coordinates xyz (three per point); mom and psi4 may be NULL. */
+void orc_constraint_point_xyz(const double *in, const double *xyz, double *ham, double *mom, double *psi4);
+void orc_constraint_points_xyz(const double *in, const double *xyz, int64_t npts, double *ham, double *mom,
+                               double *psi4);
 /* C_HAM of physical_constraints (physcon.cpp:12-89) at the interior points of
  * every block; the padding of `ham` is not written. */
 int orc_ham_block(double *ham, const double *const *u, const orc_block *blk);
 int orc_ham_batch(double *ham, const double *const *u, const orc_block *blks, int64_t nblocks, int nthreads);
-/* orc_ham_batch plus the momentum constraints into mom[0..2] (laid out like a
- * variable; padding not written). */
-int orc_constraints_batch(double *ham, double *const *mom, const double *const *u, const orc_block *blks,
-                          int64_t nblocks, int nthreads);
+/* orc_ham_batch plus the momentum constraints into mom[0..2] and (psi4 !=
+ * NULL) psi4 real/imaginary into psi4[0..1] (laid out like a variable; padding
+ * not written; psi4 = 0 within 1e-7 of the z axis as physcon.cpp:82-85). */
+int orc_constraints_batch(double *ham, double *const *mom, double *const *psi4, const double *const *u,
+                          const orc_block *blks, int64_t nblocks, int nthreads);
 /* computeConstraintL2Norm (grUtils.tcc:37-134) over interior points outside
  * r1 of BH 1 and r2 of BH 2 (bh = x1,y1,z1,x2,y2,z2); -1 if none qualifies. */
 double orc_constraint_l2_norm(const double *c, const orc_block *blks, int64_t nblocks, const double *bh,

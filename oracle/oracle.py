@@ -83,7 +83,9 @@ def lib():
         L.orc_ham_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
         L.orc_ham_batch.restype = C.c_int
         L.orc_constraint_points.argtypes = [dp, C.c_int64, dp, dp]
-        L.orc_constraints_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+        L.orc_constraint_points_xyz.argtypes = [dp, dp, C.c_int64, dp, dp, dp]
+        L.orc_constraints_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                            C.c_int]
         L.orc_constraints_batch.restype = C.c_int
         L.orc_constraint_l2_norm.argtypes = [dp, C.c_void_p, C.c_int64, dp, C.c_double, C.c_double]
         L.orc_constraint_l2_norm.restype = C.c_double
@@ -114,6 +116,7 @@ def ref_lib():
         R.ref_bssn_points.argtypes = [dp, C.c_int64, dp, dp, dp]
         R.ref_ham_points.argtypes = [dp, C.c_int64, dp]
         R.ref_constraint_points.argtypes = [dp, C.c_int64, dp, dp]
+        R.ref_constraints_xyz.argtypes = [dp, dp, C.c_int64, dp, dp, dp]
         _ref = R
     return _ref
 
@@ -192,6 +195,29 @@ def ref_constraint_points(inp: np.ndarray):
     mom = np.zeros((inp.shape[0], 3))
     R.ref_constraint_points(_dp(inp), inp.shape[0], _dp(ham), _dp(mom))
     return ham, mom
+
+
+def constraint_points_xyz(inp: np.ndarray, xyz: np.ndarray):
+    """(ham, mom, psi4) per point at coordinates xyz (n, 3): mom (n, 3),
+    psi4 (n, 2) real and imaginary part."""
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    assert xyz.shape == (inp.shape[0], 3)
+    ham, mom, psi4 = np.zeros(inp.shape[0]), np.zeros((inp.shape[0], 3)), np.zeros((inp.shape[0], 2))
+    lib().orc_constraint_points_xyz(_dp(inp), _dp(xyz), inp.shape[0], _dp(ham), _dp(mom), _dp(psi4))
+    return ham, mom, psi4
+
+
+def ref_constraints_xyz(inp: np.ndarray, xyz: np.ndarray):
+    """The reference's physconeqs.cpp at coordinates xyz: (ham, mom, psi4)."""
+    R = ref_lib()
+    assert R is not None, "oracle/_ref not built (needs the reference checkout)"
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    assert xyz.shape == (inp.shape[0], 3)
+    ham, mom, psi4 = np.zeros(inp.shape[0]), np.zeros((inp.shape[0], 3)), np.zeros((inp.shape[0], 2))
+    R.ref_constraints_xyz(_dp(inp), _dp(xyz), inp.shape[0], _dp(ham), _dp(mom), _dp(psi4))
+    return ham, mom, psi4
 
 
 def stencil(name: str, u: np.ndarray, h: float, sz, bflag: int) -> np.ndarray:
@@ -282,18 +308,21 @@ def ham_batch(state: np.ndarray, blocks: np.ndarray, nthreads: int = 0) -> np.nd
     return ham
 
 
-def constraints_batch(state: np.ndarray, blocks: np.ndarray, nthreads: int = 0):
+def constraints_batch(state: np.ndarray, blocks: np.ndarray, nthreads: int = 0, psi4: bool = False):
     """(ham, mom) at the interior points of every block: C_HAM (szpdof) and the
-    momentum constraints (3, szpdof); padding left 0."""
+    momentum constraints (3, szpdof); with psi4=True also psi4 (2, szpdof:
+    real, imaginary) as a third element; padding left 0."""
     ham = np.zeros(state.shape[1])
     mom = np.zeros((3, state.shape[1]))
+    ps = np.zeros((2, state.shape[1])) if psi4 else None
     blocks = np.ascontiguousarray(blocks)
     mp = (C.c_void_p * 3)(*[mom[c].ctypes.data for c in range(3)])
-    err = lib().orc_constraints_batch(_dp(ham), mp, _ptrs(state_views(state)), blocks.ctypes.data,
+    pp = (C.c_void_p * 2)(*[ps[c].ctypes.data for c in range(2)]) if psi4 else None
+    err = lib().orc_constraints_batch(_dp(ham), mp, pp, _ptrs(state_views(state)), blocks.ctypes.data,
                                       len(blocks), nthreads)
     if err != 0:
         raise RuntimeError("orc_constraints_batch failed")
-    return ham, mom
+    return (ham, mom, ps) if psi4 else (ham, mom)
 
 
 def constraint_l2_norm(c: np.ndarray, blocks: np.ndarray, bh=((3.0, 0, 0), (-3.0, 0, 0)),

@@ -82,9 +82,11 @@ extern "C" void ref_bssn_point_orc(const double *in, const void *params, double 
 
 // The reference's generated constraint fragment (BSSN_GR/src/physconeqs.cpp,
 // included in place as physcon.cpp:78-86 does) over the same point layout:
-// ham[pp] (and, with mom_out, mom0..2[pp], three per point) are returned;
-// psi4 is discarded. x, y, z (read by the psi4 part) are set to (1, 2, 3).
-extern "C" int ref_constraint_points(const double *in, int64_t npts, double *ham_out, double *mom_out) {
+// ham[pp], and with mom_out mom0..2[pp] (three per point), with psi4_out
+// psi4_real, psi4_img[pp] (two per point). xyz (three per point) gives the
+// coordinates the psi4 part reads; NULL sets them to (1, 2, 3).
+extern "C" int ref_constraints_xyz(const double *in, const double *xyz, int64_t npts, double *ham_out,
+                                   double *mom_out, double *psi4_out) {
     std::vector<double> soa((size_t)NIN * (size_t)npts);
     std::vector<double> res((size_t)6 * (size_t)npts);
     for (int64_t q = 0; q < npts; q++)
@@ -100,8 +102,10 @@ extern "C" int ref_constraint_points(const double *in, int64_t npts, double *ham
     GRAD2(alpha, 0) GRAD2(chi, 1) GRAD2(beta0, 2) GRAD2(beta1, 3) GRAD2(beta2, 4)
     GRAD2(gt0, 5) GRAD2(gt1, 6) GRAD2(gt2, 7) GRAD2(gt3, 8) GRAD2(gt4, 9) GRAD2(gt5, 10)
     OUT(ham, 0) OUT(mom0, 1) OUT(mom1, 2) OUT(mom2, 3) OUT(psi4_real, 4) OUT(psi4_img, 5)
-    const double x = 1.0, y = 2.0, z = 3.0;
     for (int64_t pp = 0; pp < npts; pp++) {
+        const double x = xyz ? xyz[3 * pp] : 1.0;
+        const double y = xyz ? xyz[3 * pp + 1] : 2.0;
+        const double z = xyz ? xyz[3 * pp + 2] : 3.0;
 #include "physconeqs.cpp"
     }
     for (int64_t q = 0; q < npts; q++) ham_out[q] = ham[q];
@@ -111,7 +115,16 @@ extern "C" int ref_constraint_points(const double *in, int64_t npts, double *ham
             mom_out[3 * q + 1] = mom1[q];
             mom_out[3 * q + 2] = mom2[q];
         }
+    if (psi4_out)
+        for (int64_t q = 0; q < npts; q++) {
+            psi4_out[2 * q] = psi4_real[q];
+            psi4_out[2 * q + 1] = psi4_img[q];
+        }
     return 0;
+}
+
+extern "C" int ref_constraint_points(const double *in, int64_t npts, double *ham_out, double *mom_out) {
+    return ref_constraints_xyz(in, nullptr, npts, ham_out, mom_out, nullptr);
 }
 
 extern "C" int ref_ham_points(const double *in, int64_t npts, double *ham_out) {

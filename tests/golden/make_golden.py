@@ -21,7 +21,9 @@ Fixtures:
                       the Hamiltonian constraint (ham) and the momentum
                       constraints (mom, 200 x 3) of the reference's generated
                       constraint equations (BSSN_GR/src/physconeqs.cpp,
-                      compiled in place)
+                      compiled in place), and psi4 (200 x 2: real, imaginary)
+                      of the same equations at per-point coordinates xyz
+                      (200 x 3, uniform in [-8, 8]^3, seed 11)
 
 `python make_golden.py ham` writes ham_points.npz only.
 """
@@ -88,8 +90,9 @@ def block_params():
 
 def write_ham():
     inp = algebra_inputs()
-    ham, mom = O.ref_constraint_points(inp)
-    np.savez_compressed(os.path.join(HERE, "ham_points.npz"), ham=ham, mom=mom,
+    xyz = np.random.default_rng(11).uniform(-8.0, 8.0, (len(inp), 3))
+    ham, mom, psi4 = O.ref_constraints_xyz(inp, xyz)
+    np.savez_compressed(os.path.join(HERE, "ham_points.npz"), ham=ham, mom=mom, xyz=xyz, psi4=psi4,
                         inputs_sum=np.array([inp.sum(), np.abs(inp).sum()]))
 
 

@@ -89,15 +89,19 @@ def test_generated_ham_matches_reference_constraint_vectors():
     assert np.abs(out - ref).max() / np.abs(ref).max() < 1e-13
 
 
-def test_generated_momentum_matches_reference_constraint_vectors():
-    """The generated constraint header's four outputs (ham, mom0..2) against
-    the reference's physconeqs.cpp vectors (CodeGen/constraints.py:256-282)."""
+def test_generated_constraints_match_reference_constraint_vectors():
+    """The generated constraint header's six outputs (ham, mom0..2, psi4 real
+    and imaginary) against the reference's physconeqs.cpp vectors
+    (CodeGen/constraints.py:42-282) at the fixture's coordinates."""
     g = np.load(os.path.join(GOLD, "algebra_points.npz"))
     ref = np.load(os.path.join(GOLD, "ham_points.npz"))
     inp = np.ascontiguousarray(g["inputs"])
-    out = np.zeros((len(inp), 4))
+    xyz = np.ascontiguousarray(ref["xyz"])
+    out = np.zeros((len(inp), 6))
     dp = C.POINTER(C.c_double)
-    _host_lib().dgr_host_constraint_points(inp.ctypes.data_as(dp), len(inp), out.ctypes.data_as(dp))
+    _host_lib().dgr_host_constraint_points(inp.ctypes.data_as(dp), xyz.ctypes.data_as(dp), len(inp),
+                                           out.ctypes.data_as(dp))
     assert np.abs(out[:, 0] - ref["ham"]).max() / np.abs(ref["ham"]).max() < 1e-13
-    err = np.abs(out[:, 1:] - ref["mom"]).max(0) / np.abs(ref["mom"]).max(0)
-    assert err.max() < 1e-13, err
+    for name, cols in (("mom", slice(1, 4)), ("psi4", slice(4, 6))):
+        err = np.abs(out[:, cols] - ref[name]).max(0) / np.abs(ref[name]).max(0)
+        assert err.max() < 1e-13, (name, err)

@@ -1,4 +1,4 @@
-"""Hamiltonian and momentum constraints and the masked norm on the GPU
+"""Hamiltonian and momentum constraints, psi4 and the masked norm on the GPU
 (through the C-ABI) against the oracle: per point within 1e-12 normwise
 relative, the norm within 1e-12 relative (the reference sums in another
 order)."""
@@ -114,21 +114,23 @@ def test_ham_norm_unchanged_over_an_rk4_step():
     plan.close()
 
 
-def gpu_constraints(t, st, which=(True, True, True, True)):
+def gpu_constraints(t, st, which=(True,) * 6):
+    """dgr_constraints into six NaN-filled arrays: ham, mom0..2, psi4 real and
+    imaginary (an output left out of `which` is passed as NULL)."""
     import torch
     plan = dgr.Plan(t, st.shape[1])
-    out = torch.full((4, st.shape[1]), float("nan"), dtype=torch.float64, device="cuda")
+    out = torch.full((6, st.shape[1]), float("nan"), dtype=torch.float64, device="cuda")
     sel = [out[i] if w else None for i, w in enumerate(which)]
-    plan.constraints(sel[0], sel[1:], to_dev(st))
+    plan.constraints(sel[0], sel[1:4], to_dev(st), psi4=sel[4:6])
     torch.cuda.synchronize()
     plan.close()
     return out.cpu().numpy()
 
 
 def check_constraints(got, t, st, scale_floor=0.0):
-    ham, mom = O.constraints_batch(st, t, nthreads=8)
+    ham, mom, psi4 = O.constraints_batch(st, t, nthreads=8, psi4=True)
     m = ~padding_mask(t, st.shape[1])
-    for i, ref in enumerate([ham] + list(mom)):
+    for i, ref in enumerate([ham] + list(mom) + list(psi4)):
         err = np.abs(got[i][m] - ref[m]).max()
         scale = max(np.abs(ref[m]).max(), scale_floor)
         log(f"  output {i}: max err {err:.3e} scale {scale:.3e}")
@@ -156,11 +158,13 @@ def test_momentum_uneven_puncture_blocks_match_oracle():
     t["bflag"][0] = 63
     st = host_state(t, init="puncture", seed=6, noise=1e-3)
     got = gpu_constraints(t, st)
-    ham, mom = O.constraints_batch(st, t, nthreads=8)
+    ham, mom, psi4 = O.constraints_batch(st, t, nthreads=8, psi4=True)
     m = ~padding_mask(t, st.shape[1])
     assert np.abs(got[0][m] - ham[m]).max() <= TOL * max(np.abs(ham[m]).max(), 1.0)
     for i in range(3):
         assert np.abs(got[1 + i][m] - mom[i][m]).max() <= TOL * np.abs(mom[i][m]).max(), i
+    for i in range(2):
+        assert np.abs(got[4 + i][m] - psi4[i][m]).max() <= TOL * np.abs(psi4[i][m]).max(), i
 
 
 def test_constraints_skip_null_outputs():
@@ -168,8 +172,8 @@ def test_constraints_skip_null_outputs():
     log("constraints null outputs")
     t = W.single_block_19(21)
     st = host_state(t, seed=3, amp=1e-2)
-    got = gpu_constraints(t, st, which=(False, True, False, True))
-    assert np.isnan(got[0]).all() and np.isnan(got[2]).all()
+    got = gpu_constraints(t, st, which=(False, True, False, True, False, False))
+    assert np.isnan(got[0]).all() and np.isnan(got[2]).all() and np.isnan(got[4:]).all()
     full = gpu_constraints(t, st)
     m = ~padding_mask(t, st.shape[1])
     for i in (1, 3):
@@ -180,5 +184,27 @@ def test_constraints_skip_null_outputs():
     with pytest.raises(RuntimeError):
         plan.constraints(None, None, to_dev(st))
     with pytest.raises(RuntimeError):
-        plan.constraints(None, [None, None, None], to_dev(st))
+        plan.constraints(None, [None, None, None], to_dev(st), psi4=[None, None])
     plan.close()
+    only_psi4 = gpu_constraints(t, st, which=(False, False, False, False, False, True))
+    assert np.isnan(only_psi4[:5]).all()
+    assert np.abs(only_psi4[5][m] - full[5][m]).max() <= TOL * np.abs(full[5][m]).max()
+
+
+def test_psi4_on_a_block_through_the_z_axis():
+    """A 13^3 block whose grid contains x = y = 0: psi4 is exactly zero on the
+    axis (physcon.cpp:82-85) and matches the oracle everywhere else, where the
+    tetrad degenerates as the axis is approached."""
+    log("psi4 block through the z axis")
+    t = W._table([13], [(-0.3, -0.3, 0.5)], [0.05], [0])
+    st = host_state(t, seed=4, amp=1e-2)
+    got = gpu_constraints(t, st)
+    check_constraints(got, t, st)
+    b = t[0]
+    h = (b["pmax"] - b["pmin"]) / (b["sz"] - 1)
+    x = b["pmin"][0] + np.arange(13) * h[0]
+    i0 = int(np.argmin(np.abs(x)))
+    assert abs(x[i0]) <= 1e-7
+    sl = got[4:, b["offset"]:b["offset"] + 13 ** 3].reshape(2, 13, 13, 13)  # (c, k, j, i)
+    assert (sl[:, 3:10, i0, i0] == 0.0).all()
+    assert (sl[:, 3:10, i0, i0 + 1] != 0.0).all()

@@ -192,3 +192,90 @@ def test_mom_divergence_term_contracts_the_right_index():
     assert np.abs(interior_flat(mom[0], t) - a).max() < 1e-12
     assert np.abs(interior_flat(mom[1], t)).max() < 1e-12
     assert np.abs(interior_flat(mom[2], t)).max() < 1e-12
+
+
+# --- psi4 (CodeGen/constraints.py:42-244) -----------------------------------
+
+def test_psi4_matches_reference_constraint_equations():
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    h = np.load(os.path.join(GOLD, "ham_points.npz"))
+    ham, mom, psi4 = O.constraint_points_xyz(g["inputs"], h["xyz"])
+    ref = h["psi4"]
+    assert ref.shape == (g["inputs"].shape[0], 2)
+    assert (np.abs(psi4 - ref).max(axis=0) / np.abs(ref).max(axis=0)).max() < 1e-13
+    # ham and mom do not depend on the coordinates
+    assert np.array_equal(ham, O.constraint_points(g["inputs"])[0])
+    assert np.array_equal(mom, O.constraint_points(g["inputs"])[1])
+
+
+def test_psi4_fixture_still_matches_reference_build():
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    inp = np.load(os.path.join(GOLD, "algebra_points.npz"))["inputs"]
+    h = np.load(os.path.join(GOLD, "ham_points.npz"))
+    assert np.array_equal(O.ref_constraints_xyz(inp, h["xyz"])[2], h["psi4"])
+
+
+def _flat_point():
+    """Point input of flat data: alpha = chi = 1, gt = delta, nothing else."""
+    inp = np.zeros((1, 163))
+    inp[0, 0] = inp[0, 1] = 1.0
+    inp[0, 12] = inp[0, 15] = inp[0, 17] = 1.0
+    return inp
+
+
+ON_X = np.array([[1.0, 0.0, 0.0]])  # r = e_x, theta = -e_z, phi = e_y: m = (-e_z + i e_y)/sqrt(2)
+
+
+def test_psi4_of_curvature_in_the_transverse_plane():
+    """Flat metric with only chi's second derivatives set: R_ij = 1/2 chi_ij +
+    1/2 delta_ij lap(chi), so at (x,y,z) = (1,0,0) (and anywhere on the x axis,
+    where the tetrad is the same) psi4 = -1/4 (chi_zz - chi_yy) - i/2 chi_yz.
+    A wrong tetrad orientation or a dropped factor 1/2 of m fails."""
+    a, b, c = 0.3, -0.7, 0.45
+    inp = _flat_point()
+    inp[0, 96 + 6 + 0] = 0.2  # chi_xx (no psi4 contribution)
+    inp[0, 96 + 6 + 3], inp[0, 96 + 6 + 4], inp[0, 96 + 6 + 5] = a, c, b
+    for x in (1.0, 2.5):
+        _, _, p = O.constraint_points_xyz(inp, ON_X * x)
+        assert p[0] == pytest.approx([-0.25 * (b - a), -0.5 * c], rel=1e-14)
+
+
+def test_psi4_of_constant_At():
+    """Flat metric, constant At, K = 0, at (1,0,0): the A_temp and quadratic-A
+    terms give psi4 = -1/2 At_xx (At_zz - At_yy) + 1/2 (At_xz^2 - At_xy^2)
+    + i (At_xz At_xy - At_xx At_yz) (signs of psi4_2 and psi4_4)."""
+    A = dict(xx=0.3, xy=-0.2, xz=0.5, yy=0.7, yz=-0.4, zz=0.15)
+    inp = _flat_point()
+    inp[0, 18:24] = [A["xx"], A["xy"], A["xz"], A["yy"], A["yz"], A["zz"]]
+    _, _, p = O.constraint_points_xyz(inp, ON_X)
+    want = [-0.5 * A["xx"] * (A["zz"] - A["yy"]) + 0.5 * (A["xz"] ** 2 - A["xy"] ** 2),
+            A["xz"] * A["xy"] - A["xx"] * A["yz"]]
+    assert p[0] == pytest.approx(want, rel=1e-13)
+
+
+def test_psi4_of_At_gradients():
+    """Flat metric, At = 0 with d_x At_zz = e and d_x At_yz = f at (1,0,0): only
+    the magnetic term psi4_3 contributes, psi4 = e/2 + i f (which index of
+    d_j At_ki is differentiated and the sign of psi4_3)."""
+    e, f = 0.8, -0.35
+    inp = _flat_point()
+    inp[0, 24 + 3 * 23 + 0] = e  # d_x At_zz
+    inp[0, 24 + 3 * 22 + 0] = f  # d_x At_yz
+    _, _, p = O.constraint_points_xyz(inp, ON_X)
+    assert p[0] == pytest.approx([e / 2, f], rel=1e-14)
+
+
+def test_psi4_of_minkowski_is_zero_and_axis_is_zeroed():
+    t, st = _flat_block_state(lambda st, x, y, z: None)
+    _, _, psi4 = O.constraints_batch(st, t, nthreads=1, psi4=True)
+    assert np.count_nonzero(psi4) == 0
+    # a block through the z axis with data that does not vanish: psi4 is set
+    # to zero where |x|, |y| <= 1e-7 (physcon.cpp:82-85) and nowhere else
+    t = W._table([13], [(-0.3, -0.3, 0.5)], [0.05], [0])
+    st = np.zeros((24, W.total_padded(t)))
+    O.init_perturbed_minkowski(st, t, seed=4, amp=1e-2)
+    _, _, psi4 = O.constraints_batch(st, t, nthreads=1, psi4=True)
+    p = psi4[:, :13 ** 3].reshape(2, 13, 13, 13)[:, 3:10, 3:10, 3:10]
+    assert (p[:, :, 3, 3] == 0.0).all()
+    assert np.count_nonzero(p) == p.size - 2 * 7
