@@ -262,6 +262,39 @@ def roofline(wl, kt, stages, precision="fp64"):
             "per_kernel": per_kernel}
 
 
+def constraint_rates(wl, plan, state, stream, calls=5):
+    """The constraint diagnostic (dgr_constraints) on the state after the timed
+    steps: interior points per second with all six outputs (ham, mom0..2,
+    psi4 real/imaginary) and with ham alone, plus the algorithmic HBM rate of
+    the all-outputs call (padded state read once, six outputs written once).
+    Skipped when HBM cannot hold the six output arrays."""
+    import torch
+    free, _ = torch.cuda.mem_get_info()
+    need = 6 * wl.n_pad * 8
+    if free < need * 1.1 + 2e9:
+        log("constraint rates skipped: no room for the six output arrays")
+        return None
+    out = torch.empty((6, wl.n_pad), dtype=torch.float64, device=state.device)
+    res = {}
+    for name, args in (("all_outputs", (out[0], out[1:4], out[4:6])), ("ham_only", (out[0], None, None))):
+        plan.constraints(args[0], args[1], state, psi4=args[2], stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            plan.constraints(args[0], args[1], state, psi4=args[2], stream=stream)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / calls
+        res[name + "_mpoints_s"] = round(wl.npts / dt / 1e6, 2)
+        if name == "all_outputs":
+            res["all_outputs_ms_per_call"] = round(dt * 1e3, 4)
+            byts = 24 * 8 * wl.n_pad + 6 * 8 * wl.npts
+            res["all_outputs_algorithmic_gb_s"] = round(byts / dt / 1e9, 1)
+    res["note"] = ("derivatives staged through the split path's 162-slot workspace, then the "
+                   "generated constraint expressions per point; FP64")
+    del out
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -365,6 +398,7 @@ def main():
     else:
         log("right-hand-side-only rate skipped: no room for a second padded state")
     roof = roofline(wl, kt, 4 * ksteps, args.precision)
+    constraints = constraint_rates(wl, plan, state, stream)
     plan.close()
 
     mixed = None
@@ -392,7 +426,7 @@ def main():
                        "precision": args.precision, "rhs_evals_per_step": 4, "rk4_step": "hipGraph replay",
                        "tiling": tiling, "autotuned": not args.no_autotune,
                        "dt": wl.dt, **wl.params},
-            "rhs_only_mpoints_s": rhs_only,
+            "rhs_only_mpoints_s": rhs_only, "constraints": constraints,
             "roofline": roof, "cpu_baseline": cpu, "mixed_precision": mixed}
     print(json.dumps(line), flush=True)
 
