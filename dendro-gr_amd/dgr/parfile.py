@@ -1,0 +1,100 @@
+"""The reference's parameter files (.par.json / .par.toml) read into the
+right-hand side's parameters, so a run configured for the reference can be
+replayed here.
+
+readParamFile (BSSN_GR/src/grUtils.cpp:37-52) picks TOML for a ".toml" suffix
+(readParamTOMLFile, BSSN_GR/src/parameters.cpp:183-560) and JSON otherwise
+(readParamJSONFile, grUtils.cpp:54-468). Only the keys that reach the
+right-hand side, the constraint diagnostic and the time step are read; the
+rest of the file (IO, AMR, initial data, solver control) belongs to parts of
+the reference that are not on this path and is returned untouched in `raw`.
+
+Keys read, with the reference's behaviour:
+  BSSN_LAMBDA       four integers (JSON: BSSN_LAMBDA_1..4, cast to unsigned;
+                    TOML: an array)                                 required
+  BSSN_LAMBDA_F     two floats (JSON: BSSN_LAMBDA_F0/F1; TOML: array) required
+  CHI_FLOOR, KO_DISS_SIGMA                                          required
+  RIT_ETA_CENTRAL, RIT_ETA_OUTER, RIT_ETA_WIDTH                     optional
+  BSSN_KO_SIGMA_SCALE_BY_CONFORMAL (enables CAKO) and
+  BSSN_KO_SIGMA_SCALE_BY_CONFORMAL_POST_MERGER_ONLY (if true, CAKO stays off
+  before the merger: grUtils.cpp:186-191)                           optional
+  BSSN_EPSILON_CAKO_GAUGE, BSSN_EPSILON_CAKO_OTHER                  optional
+  BSSN_BH1_CONSTRAINT_R, BSSN_BH2_CONSTRAINT_R (default 5.0)        optional
+  BSSN_BH1, BSSN_BH2 tables (X, Y, Z: the norm's excision centres)  optional here
+  BSSN_CFL_FACTOR, BSSN_RK_TIME_BEGIN, BSSN_RK_TIME_END             optional here
+A required key that is missing raises KeyError, where the reference's JSON
+reader throws on the null it would convert.
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import os
+
+from .api import BssnParams, default_params
+
+
+@dataclasses.dataclass
+class ParamFile:
+    params: BssnParams                      # for dgr_bssn_rhs / dgr_rk4_step(s)
+    bh: tuple | None                        # ((x1, y1, z1), (x2, y2, z2)) or None
+    constraint_r: tuple                     # (BSSN_BH1_CONSTRAINT_R, BSSN_BH2_CONSTRAINT_R)
+    cfl: float | None                       # BSSN_CFL_FACTOR (dt = cfl * h_min in the reference)
+    time: tuple | None                      # (BSSN_RK_TIME_BEGIN, BSSN_RK_TIME_END)
+    raw: dict                               # the whole file as parsed
+
+
+def _load(path: str) -> tuple[dict, bool]:
+    if path.endswith(".toml"):
+        try:
+            import tomllib as toml  # Python >= 3.11
+        except ImportError:
+            import tomli as toml
+        with open(path, "rb") as fh:
+            return toml.load(fh), True
+    with open(path) as fh:
+        return json.load(fh), False
+
+
+def _required(d: dict, key: str):
+    if key not in d:
+        raise KeyError(f"parameter file lacks the required key {key}")
+    return d[key]
+
+
+def read_param_file(path: str | os.PathLike) -> ParamFile:
+    path = os.fspath(path)
+    d, is_toml = _load(path)
+    p = default_params()
+    lam = _required(d, "BSSN_LAMBDA")
+    lamf = _required(d, "BSSN_LAMBDA_F")
+    if is_toml:
+        lam4 = [int(lam[i]) for i in range(4)]
+        lamf2 = [float(lamf[0]), float(lamf[1])]
+    else:
+        lam4 = [int(_required(lam, f"BSSN_LAMBDA_{i + 1}")) for i in range(4)]
+        lamf2 = [float(_required(lamf, f"BSSN_LAMBDA_F{i}")) for i in range(2)]
+    for i in range(4):
+        p.lambda_[i] = float(lam4[i])
+    p.lambda_f[0], p.lambda_f[1] = lamf2
+    p.chi_floor = float(_required(d, "CHI_FLOOR"))
+    p.ko_sigma = float(_required(d, "KO_DISS_SIGMA"))
+    for key, field in (("RIT_ETA_CENTRAL", "eta_central"), ("RIT_ETA_OUTER", "eta_outer"),
+                       ("RIT_ETA_WIDTH", "eta_width"), ("BSSN_EPSILON_CAKO_GAUGE", "eps_cako_gauge"),
+                       ("BSSN_EPSILON_CAKO_OTHER", "eps_cako_other")):
+        if key in d:
+            setattr(p, field, float(d[key]))
+    scale = bool(d.get("BSSN_KO_SIGMA_SCALE_BY_CONFORMAL", False))
+    if bool(d.get("BSSN_KO_SIGMA_SCALE_BY_CONFORMAL_POST_MERGER_ONLY", False)):
+        scale = False
+    p.cako_enabled = 1 if scale else 0
+
+    bh = None
+    if "BSSN_BH1" in d and "BSSN_BH2" in d:
+        bh = tuple(tuple(float(d[b][c]) for c in ("X", "Y", "Z")) for b in ("BSSN_BH1", "BSSN_BH2"))
+    r = (float(d.get("BSSN_BH1_CONSTRAINT_R", 5.0)), float(d.get("BSSN_BH2_CONSTRAINT_R", 5.0)))
+    cfl = float(d["BSSN_CFL_FACTOR"]) if "BSSN_CFL_FACTOR" in d else None
+    time = None
+    if "BSSN_RK_TIME_BEGIN" in d and "BSSN_RK_TIME_END" in d:
+        time = (float(d["BSSN_RK_TIME_BEGIN"]), float(d["BSSN_RK_TIME_END"]))
+    return ParamFile(params=p, bh=bh, constraint_r=r, cfl=cfl, time=time, raw=d)
