@@ -1071,6 +1071,34 @@ __global__ __launch_bounds__(256) void k_pad_copy(VarPtrs dst, VarPtrs src, cons
     }
 }
 
+// Padding refresh from same-level neighbours, the copy part of the
+// reference's unzip (bssnCtx.cpp:75): a padding point of block b lying in the
+// interior of the neighbour nb in direction (di,dj,dk) takes nb's value at the
+// same coordinates. Neighbours share their boundary nodes, so nb's node index
+// is b's minus d*(n-2*PW-1) along each axis; every source is an interior point
+// and every destination a padding point, so blocks can be filled in place in
+// any order. nbr: 27 entries per block, -1 for none (that padding is kept).
+// One workgroup per (block, variable).
+__global__ __launch_bounds__(256) void k_halo_fill(VarPtrs u, const DBlock *__restrict__ blocks,
+                                                   const int *__restrict__ nbr) {
+    const DBlock &B = blocks[blockIdx.x];
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int n = nx * ny * nz;
+    const int *__restrict__ tab = nbr + 27 * (size_t)blockIdx.x;
+    double *__restrict__ f = u.p[blockIdx.y];
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const int i = p % nx, j = (p / nx) % ny, k = p / (nx * ny);
+        const int di = i < PW ? -1 : (i >= nx - PW ? 1 : 0);
+        const int dj = j < PW ? -1 : (j >= ny - PW ? 1 : 0);
+        const int dk = k < PW ? -1 : (k >= nz - PW ? 1 : 0);
+        if (!(di | dj | dk)) continue;
+        const int nb = tab[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)];
+        if (nb < 0) continue;
+        const int si = i - di * (nx - 2 * PW - 1), sj = j - dj * (ny - 2 * PW - 1), sk = k - dk * (nz - 2 * PW - 1);
+        f[B.offset + p] = f[blocks[nb].offset + si + (size_t)nx * (sj + (size_t)ny * sk)];
+    }
+}
+
 // -------------------------------------------------------------------------
 // Hamiltonian constraint (C_HAM of physical_constraints, physcon.cpp:12-89):
 // the generated constraint at every interior point of a chunk, from the
@@ -1300,6 +1328,10 @@ struct dgr_plan {
     dgr::AxisConsts *d_bconst = nullptr;  // k_fused: x, y, z stencil constants per block
     uint16_t *d_tabs = nullptr;           // k_fused: lane tables per tile shape, slot tables per config
     int dtab_wide = 0, dtab_thin = 0, dtab_full = 0;  // offsets of the configurations' slot tables
+    std::vector<int> hnbr;      // same-level neighbour table, 27 per block (built on first use)
+    int *d_nbr = nullptr;       // its device copy
+    int64_t nlinks = -1;        // links in the table (-1: not built yet)
+    bool halo = false;          // dgr_rk4_step(s): refresh padding from neighbours before each stage
     double *d_istage = nullptr; // k_fused RK4: 2 * 24 * interior (u0 copy, accumulator)
     double *d_wstate = nullptr; // k_fused RK4: 24 * szpdof, the padded work state W
     int *d_err = nullptr;
@@ -1311,6 +1343,7 @@ struct dgr_plan {
         double dt;
         dgr_bssn_params prm;
         int enforce;
+        bool halo;
     } rk4_key{};
     int nwg_total = 0;                    // k_rhs-style workgroups over all chunks
     double *d_norm = nullptr;             // norm partials: nwg_total sums, nwg_total counts, 2 results
@@ -1738,6 +1771,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     (void)hipFree(plan->d_tabs);
     (void)hipFree(plan->d_istage);
     (void)hipFree(plan->d_wstate);
+    (void)hipFree(plan->d_nbr);
     (void)hipFree(plan->d_err);
     (void)hipFree(plan->d_norm);
     if (plan->rk4_exec) (void)hipGraphExecDestroy(plan->rk4_exec);
@@ -2104,6 +2138,72 @@ int launch_pad_copy(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
     return DGR_OK;
 }
 
+// Same-level neighbour table from the block geometry: nb is b's neighbour in
+// direction d when both have the same sizes and spacing (within 1e-9
+// relative) and pmin_nb = pmin_b + d (n-2*PW-1) h within 1e-6 h. Blocks are
+// grouped by (sizes, spacing) and indexed by their lattice position
+// relative to the group's first block.
+int link_neighbors(dgr_plan *plan) {
+    const size_t nb = plan->hblocks.size();
+    plan->hnbr.assign(27 * nb, -1);
+    std::vector<int> group(nb, -1);
+    std::vector<int> rep;  // first block of each group
+    for (size_t b = 0; b < nb; b++) {
+        const DBlock &B = plan->hblocks[b];
+        for (size_t g = 0; g < rep.size() && group[b] < 0; g++) {
+            const DBlock &R = plan->hblocks[rep[g]];
+            bool same = true;
+            for (int a = 0; a < 3; a++)
+                same = same && B.n[a] == R.n[a] && std::fabs(B.h[a] - R.h[a]) <= 1e-9 * R.h[a];
+            if (same) group[b] = (int)g;
+        }
+        if (group[b] < 0) {
+            group[b] = (int)rep.size();
+            rep.push_back((int)b);
+        }
+    }
+    std::map<std::array<int64_t, 4>, int> at;
+    std::vector<std::array<int64_t, 3>> cell(nb);
+    for (size_t b = 0; b < nb; b++) {
+        const DBlock &B = plan->hblocks[b], &R = plan->hblocks[rep[group[b]]];
+        for (int a = 0; a < 3; a++) cell[b][a] = std::llround((B.pmin[a] - R.pmin[a]) / ((B.n[a] - 2 * PW - 1) * R.h[a]));
+        at[{group[b], cell[b][0], cell[b][1], cell[b][2]}] = (int)b;
+    }
+    int64_t links = 0;
+    for (size_t b = 0; b < nb; b++) {
+        const DBlock &B = plan->hblocks[b];
+        for (int d = 0; d < 27; d++) {
+            const int di = d % 3 - 1, dj = (d / 3) % 3 - 1, dk = d / 9 - 1;
+            if (!(di | dj | dk)) continue;
+            auto it = at.find({group[b], cell[b][0] + di, cell[b][1] + dj, cell[b][2] + dk});
+            if (it == at.end()) continue;
+            const DBlock &N = plan->hblocks[it->second];
+            const int dd[3] = {di, dj, dk};
+            bool ok = true;
+            for (int a = 0; a < 3; a++)
+                ok = ok && std::fabs(N.pmin[a] - (B.pmin[a] + dd[a] * (B.n[a] - 2 * PW - 1) * B.h[a])) <= 1e-6 * B.h[a];
+            if (!ok) continue;
+            plan->hnbr[27 * b + d] = it->second;
+            links++;
+        }
+    }
+    if (!plan->d_nbr && nb) {
+        hipError_t e = hipMalloc(&plan->d_nbr, sizeof(int) * 27 * nb);
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "neighbour table");
+    }
+    if (nb) HIP_OK(hipMemcpy(plan->d_nbr, plan->hnbr.data(), sizeof(int) * 27 * nb, hipMemcpyHostToDevice));
+    plan->nlinks = links;
+    return DGR_OK;
+}
+
+int launch_halo_fill(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
+    const uint32_t nb = (uint32_t)plan->hblocks.size();
+    if (!nb || plan->nlinks == 0) return DGR_OK;
+    hipLaunchKernelGGL(k_halo_fill, dim3(nb, NV), dim3(256), 0, st, u, plan->d_blocks, plan->d_nbr);
+    HIP_OK(hipGetLastError());
+    return DGR_OK;
+}
+
 // The four stage launches of one RK4 step on the fused path (rkBSSN.cpp:764-860):
 // stages read u, W, u, W and write the next state into W, u, W, u (see
 // StageIO). Needs ensure_istage() first.
@@ -2127,9 +2227,11 @@ int rk4_fused_launches(dgr_plan *plan, const VarPtrs &u, double dt, const RhsPar
         io.stage = s;
         io.enforce_stage = enforce_per_stage ? 1 : 0;
         io.err = plan->d_err;
-        int rc = run_fused(plan, io, prm, st);
-        if (rc) return rc;
+        int rc;
+        if (plan->halo && (rc = launch_halo_fill(plan, io.src, st))) return rc;
+        if ((rc = run_fused(plan, io, prm, st))) return rc;
     }
+    if (plan->halo) return launch_halo_fill(plan, u, st);
     return DGR_OK;
 }
 
@@ -2150,6 +2252,7 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
         if ((rc = ensure_istage(plan)) || (rc = launch_pad_copy(plan, u, st))) return rc;
         return rk4_fused_launches(plan, u, dt, to_params(params), enforce_per_stage, st);
     }
+    if (plan->halo) return fail(DGR_E_INVALID, "the padding refresh needs the fused path (DGR_KERNEL=split is set)");
     const size_t n = plan->szpdof;
     if (!plan->d_stage && n) {
         hipError_t e = hipMalloc(&plan->d_stage, sizeof(double) * 2 * NV * n);
@@ -2195,7 +2298,7 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
     if ((rc = ensure_istage(plan))) return rc;
     auto &key = plan->rk4_key;
     const int enf = enforce_per_stage ? 1 : 0;
-    bool same = plan->rk4_exec && key.dt == dt && key.enforce == enf &&
+    bool same = plan->rk4_exec && key.dt == dt && key.enforce == enf && key.halo == plan->halo &&
                 std::memcmp(&key.prm, params, sizeof(*params)) == 0;
     for (int v = 0; same && v < NV; v++) same = key.u[v] == u.p[v];
     if (!same) {
@@ -2223,9 +2326,43 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         key.dt = dt;
         key.prm = *params;
         key.enforce = enf;
+        key.halo = plan->halo;
     }
     if (nsteps > 0 && (rc = launch_pad_copy(plan, u, (hipStream_t)stream))) return rc;
     for (int i = 0; i < nsteps; i++) HIP_OK(hipGraphLaunch(plan->rk4_exec, (hipStream_t)stream));
+    return DGR_OK;
+}
+
+int dgr_plan_link_neighbors(dgr_plan *plan, int32_t *nbr, int64_t *nlinks) {
+    g_err.clear();
+    if (!plan) return fail(DGR_E_INVALID, "null plan");
+    HIP_OK(hipSetDevice(plan->device));
+    int rc;
+    if ((rc = link_neighbors(plan))) return rc;
+    if (nbr) std::memcpy(nbr, plan->hnbr.data(), sizeof(int32_t) * plan->hnbr.size());
+    if (nlinks) *nlinks = plan->nlinks;
+    return DGR_OK;
+}
+
+int dgr_halo_fill(dgr_plan *plan, double *const *d_u, void *stream) {
+    g_err.clear();
+    if (!plan) return fail(DGR_E_INVALID, "null plan");
+    VarPtrs u;
+    int rc;
+    if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
+    HIP_OK(hipSetDevice(plan->device));
+    if (plan->nlinks < 0 && (rc = link_neighbors(plan))) return rc;
+    return launch_halo_fill(plan, u, (hipStream_t)stream);
+}
+
+int dgr_plan_set_halo_refresh(dgr_plan *plan, int on) {
+    g_err.clear();
+    if (!plan) return fail(DGR_E_INVALID, "null plan");
+    if (on && !plan->fused) return fail(DGR_E_INVALID, "the padding refresh needs the fused path (DGR_KERNEL=split is set)");
+    HIP_OK(hipSetDevice(plan->device));
+    int rc;
+    if (on && plan->nlinks < 0 && (rc = link_neighbors(plan))) return rc;
+    plan->halo = on != 0;
     return DGR_OK;
 }
 

@@ -23,6 +23,7 @@ SYMBOLS = [
     "dgr_plan_status", "dgr_plan_set_timing", "dgr_plan_kernel_times",
     "dgr_ham_constraint", "dgr_constraint_l2_norm", "dgr_rk4_steps", "dgr_plan_set_precision",
     "dgr_plan_autotune", "dgr_plan_tiling", "dgr_plan_set_tiling", "dgr_constraints",
+    "dgr_plan_link_neighbors", "dgr_halo_fill", "dgr_plan_set_halo_refresh",
 ]
 PRECISION = {"fp64": 0, "mixed": 1}
 TILINGS = ("wt", "w", "full")  # DGR_TILING_WT, _W, _FULL
@@ -78,6 +79,9 @@ def load():
         L.dgr_rk4_steps.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), C.c_int, C.c_int, vp]
         L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
         L.dgr_constraints.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.dgr_plan_link_neighbors.argtypes = [vp, vp, C.POINTER(C.c_int64)]
+        L.dgr_halo_fill.argtypes = [vp, vp, vp]
+        L.dgr_plan_set_halo_refresh.argtypes = [vp, C.c_int]
         L.dgr_plan_autotune.argtypes = [vp, vp, vp, C.POINTER(BssnParams), vp]
         L.dgr_plan_tiling.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]
         L.dgr_plan_set_tiling.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
@@ -182,6 +186,26 @@ class Plan:
             pp = (C.c_void_p * 2)(*[ptr(psi4[i]) for i in range(2)])
         _check(load().dgr_constraints(self._h, C.c_void_p(ptr(ham)), mp, pp, var_pointers(state),
                                       _stream_ptr(stream)), "dgr_constraints")
+
+    def link_neighbors(self):
+        """Same-level neighbour table (nblocks x 27 int32, -1 for none) as the
+        library found it from the block geometry."""
+        import numpy as np
+        n = len(self._blocks)
+        out = np.full((n, 27), -1, dtype=np.int32)
+        links = C.c_int64()
+        _check(load().dgr_plan_link_neighbors(self._h, C.c_void_p(out.ctypes.data), C.byref(links)),
+               "dgr_plan_link_neighbors")
+        assert int(links.value) == int((out >= 0).sum())
+        return out
+
+    def halo_fill(self, state, stream=None):
+        """Padding of `state` refreshed in place from same-level neighbours."""
+        _check(load().dgr_halo_fill(self._h, var_pointers(state), _stream_ptr(stream)), "dgr_halo_fill")
+
+    def set_halo_refresh(self, on: bool):
+        """RK4 entry points refresh the padding from neighbours before every stage."""
+        _check(load().dgr_plan_set_halo_refresh(self._h, int(bool(on))), "dgr_plan_set_halo_refresh")
 
     def constraint_norm(self, c, bh=((3.0, 0.0, 0.0), (-3.0, 0.0, 0.0)), r1: float = 5.0,
                         r2: float = 5.0, stream=None) -> float:

@@ -39,6 +39,49 @@ def _table(sizes, pmins, hs, bflags=None, align=1):
     return t
 
 
+def lattice(shape=(3, 3, 2), n: int = 13, h: float = 0.05, origin=(-0.47, -0.31, -0.22), holes=(),
+            extra_level: bool = False) -> np.ndarray:
+    """Blocks of n^3 points on a regular lattice of shape (nbx, nby, nbz):
+    neighbours share their boundary nodes, so block (a,b,c) starts at
+    origin + (a,b,c) (n-7) h. `holes` lists lattice cells left out; with
+    extra_level, one block of spacing h/2 is appended (it links to nothing).
+    Blocks are stored in a shuffled order so the table order is not the
+    lattice order."""
+    cells = [(a, b, c) for c in range(shape[2]) for b in range(shape[1]) for a in range(shape[0])
+             if (a, b, c) not in set(holes)]
+    order = np.random.default_rng(1).permutation(len(cells))
+    cells = [cells[i] for i in order]
+    step = (n - 7) * h
+    pmins = [tuple(origin[d] + cells[q][d] * step for d in range(3)) for q in range(len(cells))]
+    hs = [h] * len(cells)
+    if extra_level:
+        pmins.append(tuple(origin[d] + shape[d] * step + 1.0 for d in range(3)))
+        hs.append(h / 2)
+    return _table([n] * len(pmins), pmins, hs)
+
+
+def neighbors(t: np.ndarray, tol: float = 1e-6) -> np.ndarray:
+    """Same-level neighbour table (nblocks x 27 int32, index
+    (di+1) + 3(dj+1) + 9(dk+1), -1 for none, the centre entry -1): block nb
+    is b's neighbour in direction d when both have the same sizes and spacing
+    and pmin_nb = pmin_b + d (n-7) h within tol h. Brute force over pairs."""
+    nb = len(t)
+    out = np.full((nb, 27), -1, dtype=np.int32)
+    sz = t["sz"].astype(np.int64)
+    h = (t["pmax"] - t["pmin"]) / (sz - 1)
+    for b in range(nb):
+        for c in range(nb):
+            if c == b or not np.array_equal(sz[b], sz[c]) or np.abs(h[b] - h[c]).max() > tol * h[b].min():
+                continue
+            d = (t["pmin"][c] - t["pmin"][b]) / ((sz[b] - 7) * h[b])
+            r = np.rint(d)
+            if np.abs(d - r).max() * (sz[b] - 7).max() > tol or np.abs(r).max() > 1 or not r.any():
+                continue
+            di, dj, dk = (int(x) for x in r)
+            out[b, (di + 1) + 3 * (dj + 1) + 9 * (dk + 1)] = c
+    return out
+
+
 def total_padded(t: np.ndarray) -> int:
     """Length each variable array needs (max offset + block size)."""
     if len(t) == 0:

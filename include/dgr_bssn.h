@@ -195,6 +195,28 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
 int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
                   int enforce_per_stage, int nsteps, void *stream);
 
+/* Padding refresh from same-level neighbours: the part of the reference's
+ * unzip (BSSN_GR/src/bssnCtx.cpp:75, octree code in dendrolib) that needs no
+ * interpolation. Block nb is block b's neighbour in direction
+ * (di,dj,dk) in {-1,0,1}^3 when both have the same sizes and spacing and
+ * pmin_nb = pmin_b + d (n - 7) h (the two share their boundary nodes); the
+ * table is found from the block geometry. Padding points of b inside such a
+ * neighbour take its value at the same coordinates; padding across a level
+ * jump or the domain boundary (no same-level neighbour) is left as it is.
+ *
+ * dgr_plan_link_neighbors builds (again) the table; nbr (27 int32 per block,
+ * index (di+1)+3(dj+1)+9(dk+1), -1 for none; may be NULL) and nlinks (may be
+ * NULL) receive it. dgr_halo_fill refreshes the padding of d_u in place
+ * (asynchronous on `stream`; builds the table on first use).
+ * dgr_plan_set_halo_refresh(plan, 1) makes dgr_rk4_step(s) refresh the
+ * padding of each stage's state before its right-hand side and the caller's
+ * padding after every step, as the reference's unzip between stages does
+ * (rkBSSN.cpp:764-860); 0 (the default) keeps every block's padding at its
+ * value at the start of the call. Needs the fused path. */
+int dgr_plan_link_neighbors(dgr_plan *plan, int32_t *nbr, int64_t *nlinks);
+int dgr_halo_fill(dgr_plan *plan, double *const *d_u, void *stream);
+int dgr_plan_set_halo_refresh(dgr_plan *plan, int on);
+
 /* enforce_bssn_constraints (BSSN_GR/include/bssn_constraints.h:15-172) on the
  * interior points of every block: det(gt) = 1, tr(At) = 0, chi and alpha
  * floored at chi_floor. */

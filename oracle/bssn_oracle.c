@@ -1026,6 +1026,107 @@ int orc_rk4_step(double *const *u, const orc_block *blks, int64_t nblocks, doubl
 }
 
 /* ======================================================================== */
+/* Padding refresh from same-level neighbours and RK4 with it                 */
+/* ======================================================================== */
+/* The part of unzip (bssnCtx.cpp:75, dendrolib) that needs no interpolation:
+ * a padding point of block b that lies in the interior of the neighbouring
+ * block nb in direction (di,dj,dk) (same size and spacing, the two blocks
+ * sharing their boundary nodes, so nb starts n-2*PW-1 nodes further along)
+ * takes nb's value at the same coordinates. nbr holds 27 entries per block,
+ * index (di+1) + 3(dj+1) + 9(dk+1), -1 where there is no such neighbour;
+ * those padding points keep their values. */
+void orc_halo_fill(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr) {
+    const int PW = 3;
+    for (int64_t b = 0; b < nblocks; b++) {
+        const orc_block *blk = &blks[b];
+        const int nx = (int)blk->sz[0], ny = (int)blk->sz[1], nz = (int)blk->sz[2];
+        const int st[3] = {nx - 2 * PW - 1, ny - 2 * PW - 1, nz - 2 * PW - 1};
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    const int di = i < PW ? -1 : (i >= nx - PW ? 1 : 0);
+                    const int dj = j < PW ? -1 : (j >= ny - PW ? 1 : 0);
+                    const int dk = k < PW ? -1 : (k >= nz - PW ? 1 : 0);
+                    if (!di && !dj && !dk) continue;
+                    const int32_t nb = nbr[27 * b + (di + 1) + 3 * (dj + 1) + 9 * (dk + 1)];
+                    if (nb < 0) continue;
+                    const int si = i - di * st[0], sj = j - dj * st[1], sk = k - dk * st[2];
+                    const int64_t src = (int64_t)blks[nb].offset + si + (int64_t)nx * (sj + (int64_t)ny * sk);
+                    const int64_t dst = (int64_t)blk->offset + i + (int64_t)nx * (j + (int64_t)ny * k);
+                    for (int v = 0; v < 24; v++) u[v][dst] = u[v][src];
+                }
+    }
+}
+
+/* RK4 (rkBSSN.cpp:764-860) with the padding of every stage state refreshed
+ * from the neighbours before its right-hand side, as the reference's unzip
+ * does between stages: u's padding before stage 0, each later stage's state
+ * before its stage, and u's padding again after the step. Padding without a
+ * neighbour is frozen at its value at the start of the step. */
+int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, double dt,
+                      const orc_params *p, int enforce_per_stage, int nthreads) {
+    const int PW = 3;
+    int64_t n = 0;
+    for (int64_t b = 0; b < nblocks; b++) {
+        const int64_t e = (int64_t)blks[b].offset + (int64_t)blks[b].sz[0] * blks[b].sz[1] * blks[b].sz[2];
+        if (e > n) n = e;
+    }
+    double *buf = (double *)malloc(sizeof(double) * (size_t)n * 24 * 3);
+    if (!buf) return -2;
+    double *u0[24], *us[24], *k[24];
+    orc_halo_fill(u, blks, nblocks, nbr);
+    for (int v = 0; v < 24; v++) {
+        u0[v] = buf + (size_t)n * v;
+        us[v] = buf + (size_t)n * (24 + v);
+        k[v] = buf + (size_t)n * (48 + v);
+        memcpy(u0[v], u[v], sizeof(double) * (size_t)n);
+        memcpy(us[v], u[v], sizeof(double) * (size_t)n);
+        memset(k[v], 0, sizeof(double) * (size_t)n);
+    }
+    int err = 0;
+    for (int s = 0; s < 4 && !err; s++) {
+        if (s > 0) orc_halo_fill(us, blks, nblocks, nbr);
+#ifdef _OPENMP
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+#endif
+        for (int64_t b = 0; b < nblocks; b++) {
+            const orc_block *blk = &blks[b];
+            int e = orc_bssnrhs_block(k, (const double *const *)us, blk, p);
+            const int nx = (int)blk->sz[0], ny = (int)blk->sz[1], nz = (int)blk->sz[2];
+            for (int kk = PW; kk < nz - PW && !e; kk++)
+                for (int j = PW; j < ny - PW; j++)
+                    for (int i = PW; i < nx - PW; i++) {
+                        const int64_t pp = (int64_t)blk->offset + i + (int64_t)nx * (j + (int64_t)ny * kk);
+                        for (int v = 0; v < 24; v++) u[v][pp] += (RK4_C[s] * dt * k[v][pp]);
+                        if (s < 3) {
+                            for (int v = 0; v < 24; v++) us[v][pp] = u0[v][pp] + (RK4_U[s + 1] * dt * k[v][pp]);
+                            if (enforce_per_stage) {
+                                const int e2 = orc_enforce_constraints_point(us, (uint64_t)pp, p->chi_floor);
+                                if (e2) e = e2;
+                            }
+                        }
+                    }
+            err |= (e != 0);
+        }
+    }
+    (void)nthreads;
+    for (int64_t b = 0; b < nblocks && !err; b++) {
+        const orc_block *blk = &blks[b];
+        const int nx = (int)blk->sz[0], ny = (int)blk->sz[1], nz = (int)blk->sz[2];
+        for (int kk = PW; kk < nz - PW; kk++)
+            for (int j = PW; j < ny - PW; j++)
+                for (int i = PW; i < nx - PW; i++) {
+                    const int64_t pp = (int64_t)blk->offset + i + (int64_t)nx * (j + (int64_t)ny * kk);
+                    if (orc_enforce_constraints_point(u, (uint64_t)pp, p->chi_floor)) err = 1;
+                }
+    }
+    if (!err) orc_halo_fill(u, blks, nblocks, nbr);
+    free(buf);
+    return err ? -1 : 0;
+}
+
+/* ======================================================================== */
 /* Synthetic inputs                                                           */
 /* ======================================================================== */
 

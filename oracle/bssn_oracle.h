@@ -128,6 +128,13 @@ int orc_ham_batch(double *ham, const double *const *u, const orc_block *blks, in
  * not written; psi4 = 0 within 1e-7 of the z axis as physcon.cpp:82-85). */
 int orc_constraints_batch(double *ham, double *const *mom, double *const *psi4, const double *const *u,
                           const orc_block *blks, int64_t nblocks, int nthreads);
+/* Padding refresh from same-level neighbours (the copy part of unzip): nbr
+ * has 27 entries per block, index (di+1)+3(dj+1)+9(dk+1), -1 for none. */
+void orc_halo_fill(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr);
+/* One RK4 step with every stage state's padding refreshed by orc_halo_fill
+ * before its right-hand side, and u's padding after the step. */
+int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, double dt,
+                      const orc_params *p, int enforce_per_stage, int nthreads);
 /* computeConstraintL2Norm (grUtils.tcc:37-134) over interior points outside
  * r1 of BH 1 and r2 of BH 2 (bh = x1,y1,z1,x2,y2,z2); -1 if none qualifies. */
 double orc_constraint_l2_norm(const double *c, const orc_block *blks, int64_t nblocks, const double *bh,

@@ -79,6 +79,10 @@ def lib():
         L.orc_rk4_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
                                    C.POINTER(Params), C.c_int, C.c_int]
         L.orc_rk4_step.restype = C.c_int
+        L.orc_halo_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_rk4_step_halo.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_double,
+                                        C.POINTER(Params), C.c_int, C.c_int]
+        L.orc_rk4_step_halo.restype = C.c_int
         L.orc_ham_points.argtypes = [dp, C.c_int64, dp]
         L.orc_ham_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
         L.orc_ham_batch.restype = C.c_int
@@ -295,6 +299,28 @@ def rk4_step(state: np.ndarray, blocks: np.ndarray, dt: float, p: Params,
                              C.byref(p), int(enforce_per_stage), nthreads)
     if err != 0:
         raise RuntimeError("orc_rk4_step failed (constraint violation)")
+
+
+def halo_fill(state: np.ndarray, blocks: np.ndarray, nbr: np.ndarray) -> None:
+    """In place: padding points refreshed from the same-level neighbours in
+    nbr (nblocks x 27 int32, -1 for none)."""
+    blocks = np.ascontiguousarray(blocks)
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    assert nbr.shape == (len(blocks), 27)
+    lib().orc_halo_fill(_ptrs(state_views(state)), blocks.ctypes.data, len(blocks), nbr.ctypes.data)
+
+
+def rk4_step_halo(state: np.ndarray, blocks: np.ndarray, nbr: np.ndarray, dt: float, p: Params,
+                  enforce_per_stage: bool = True, nthreads: int = 0) -> None:
+    """In place: one RK4 step with the padding refreshed from the neighbours
+    before every stage and after the step."""
+    blocks = np.ascontiguousarray(blocks)
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    assert nbr.shape == (len(blocks), 27)
+    err = lib().orc_rk4_step_halo(_ptrs(state_views(state)), blocks.ctypes.data, len(blocks), nbr.ctypes.data,
+                                  dt, C.byref(p), int(enforce_per_stage), nthreads)
+    if err != 0:
+        raise RuntimeError("orc_rk4_step_halo failed (constraint violation)")
 
 
 def ham_batch(state: np.ndarray, blocks: np.ndarray, nthreads: int = 0) -> np.ndarray:
