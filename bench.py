@@ -73,6 +73,15 @@ class Workload:
             self.desc = f"uniform_13^3 x {nblocks} blocks, one RK4 step per step"
             self.dt = 0.25 * 0.1
             self.params = dict(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
+        elif name == "lattice13":
+            # 16^3 lattice of 13^3 blocks sharing boundary nodes: RK4 with the
+            # padding refreshed from the neighbours before every stage
+            self.blocks = W.lattice(shape=(16, 16, 16), n=13, h=0.05, origin=(-4.8, -4.8, -4.8))
+            self.key = "lattice13x4096"
+            self.desc = ("lattice of 16^3 blocks of 13^3 sharing boundary nodes, padding refreshed "
+                         "from same-level neighbours before every RK4 stage, one RK4 step per step")
+            self.dt = 0.25 * 0.05
+            self.params = dict(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
         elif name == "puncture":
             self.blocks = W.puncture_octree(scale)
             self.key = f"puncture{scale:g}"
@@ -89,7 +98,7 @@ class Workload:
 
     def device_state(self, dev):
         from dgr import synth
-        if self.name == "uniform13":
+        if self.name in ("uniform13", "lattice13"):
             return synth.perturbed_minkowski(self.blocks, seed=1, amp=1e-3, device=dev)
         return synth.brill_lindquist(self.blocks, seed=1, noise=1e-6, device=dev)
 
@@ -262,6 +271,33 @@ def roofline(wl, kt, stages, precision="fp64"):
             "per_kernel": per_kernel}
 
 
+def halo_rate(wl, plan, state, stream, calls=20):
+    """dgr_halo_fill alone: average launch time from HIP events on the launch
+    stream, and its algorithmic HBM rate (every linked padding point read
+    from the neighbour's interior and written, 24 variables x 16 B)."""
+    import torch
+    nbr = plan.link_neighbors()
+    n = 13
+    ext = {-1: 3, 0: n - 6, 1: 3}
+    pts = 0
+    for d in range(27):
+        di, dj, dk = d % 3 - 1, (d // 3) % 3 - 1, d // 9 - 1
+        if di or dj or dk:
+            pts += int((nbr[:, d] >= 0).sum()) * ext[di] * ext[dj] * ext[dk]
+    byts = pts * 24 * 16
+    plan.halo_fill(state, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(calls):
+        plan.halo_fill(state, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / calls
+    return {"links": int((nbr >= 0).sum()), "padding_points_filled": pts, "bytes_per_launch": byts,
+            "ms_per_launch": round(ms, 4), "achieved_GBs": round(byts / ms / 1e6, 1),
+            "launches_per_rk4_step": 5}
+
+
 def constraint_rates(wl, plan, state, stream, calls=5):
     """The constraint diagnostic (dgr_constraints) on the state after the timed
     steps: interior points per second with all six outputs (ham, mom0..2,
@@ -300,7 +336,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["uniform13", "puncture"], default="uniform13")
+    ap.add_argument("--workload", choices=["uniform13", "puncture", "lattice13"], default="uniform13")
     ap.add_argument("--nblocks", type=int, default=4096, help="uniform13: number of blocks")
     ap.add_argument("--scale", type=float, default=0.42, help="puncture: block-count scale")
     ap.add_argument("--cpu-blocks", type=int, default=1024, help="uniform13: CPU baseline sample")
@@ -338,9 +374,14 @@ def main():
     wl = Workload(args.workload, args.nblocks, args.scale)
     log(f"workload: {wl.desc}; {wl.npts} interior points, state {24 * wl.n_pad * 8 / 1e9:.2f} GB")
     state = wl.device_state(dev)
+    # the mixed comparison and the CPU baseline are for the frozen-padding step
+    if wl.name == "lattice13":
+        args.no_mixed = args.no_cpu_baseline = True
     state0 = state.clone() if (args.precision == "fp64" and not args.no_mixed) else None
     plan = dgr.Plan(wl.blocks, wl.n_pad)
     plan.set_precision(args.precision)
+    if wl.name == "lattice13":
+        plan.set_halo_refresh(True)
     params = dgr.default_params(**wl.params)
     stream = torch.cuda.current_stream()
     if not args.no_autotune:
@@ -399,6 +440,7 @@ def main():
         log("right-hand-side-only rate skipped: no room for a second padded state")
     roof = roofline(wl, kt, 4 * ksteps, args.precision)
     constraints = constraint_rates(wl, plan, state, stream)
+    halo = halo_rate(wl, plan, state, stream) if wl.name == "lattice13" else None
     plan.close()
 
     mixed = None
@@ -426,7 +468,7 @@ def main():
                        "precision": args.precision, "rhs_evals_per_step": 4, "rk4_step": "hipGraph replay",
                        "tiling": tiling, "autotuned": not args.no_autotune,
                        "dt": wl.dt, **wl.params},
-            "rhs_only_mpoints_s": rhs_only, "constraints": constraints,
+            "rhs_only_mpoints_s": rhs_only, "constraints": constraints, "halo_fill": halo,
             "roofline": roof, "cpu_baseline": cpu, "mixed_precision": mixed}
     print(json.dumps(line), flush=True)
 

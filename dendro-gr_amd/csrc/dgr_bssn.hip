@@ -1078,24 +1078,37 @@ __global__ __launch_bounds__(256) void k_pad_copy(VarPtrs dst, VarPtrs src, cons
 // is b's minus d*(n-2*PW-1) along each axis; every source is an interior point
 // and every destination a padding point, so blocks can be filled in place in
 // any order. nbr: 27 entries per block, -1 for none (that padding is kept).
-// One workgroup per (block, variable).
+// One thread per point, workgroups over (block, 256-point chunk): the
+// neighbours' offsets are staged in LDS, and each padding point's index
+// arithmetic is done once for all 24 variables, whose loads are issued
+// together before the stores.
 __global__ __launch_bounds__(256) void k_halo_fill(VarPtrs u, const DBlock *__restrict__ blocks,
                                                    const int *__restrict__ nbr) {
+    __shared__ int64_t s_off[27];
     const DBlock &B = blocks[blockIdx.x];
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const int n = nx * ny * nz;
-    const int *__restrict__ tab = nbr + 27 * (size_t)blockIdx.x;
-    double *__restrict__ f = u.p[blockIdx.y];
-    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    if (threadIdx.x < 27) {
+        const int nb = nbr[27 * (size_t)blockIdx.x + threadIdx.x];
+        s_off[threadIdx.x] = (nb >= 0 && threadIdx.x != 13) ? (int64_t)blocks[nb].offset : -1;
+    }
+    __syncthreads();
+    const int p = blockIdx.y * blockDim.x + threadIdx.x;
+    if (p < n) {
         const int i = p % nx, j = (p / nx) % ny, k = p / (nx * ny);
         const int di = i < PW ? -1 : (i >= nx - PW ? 1 : 0);
         const int dj = j < PW ? -1 : (j >= ny - PW ? 1 : 0);
         const int dk = k < PW ? -1 : (k >= nz - PW ? 1 : 0);
-        if (!(di | dj | dk)) continue;
-        const int nb = tab[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)];
-        if (nb < 0) continue;
+        const int64_t off = s_off[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)];  // -1 for interior points too
+        if (off < 0) return;
         const int si = i - di * (nx - 2 * PW - 1), sj = j - dj * (ny - 2 * PW - 1), sk = k - dk * (nz - 2 * PW - 1);
-        f[B.offset + p] = f[blocks[nb].offset + si + (size_t)nx * (sj + (size_t)ny * sk)];
+        const size_t src = (size_t)off + si + (size_t)nx * (sj + (size_t)ny * sk);
+        const size_t dst = B.offset + p;
+        double t[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) t[v] = u.p[v][src];
+#pragma unroll
+        for (int v = 0; v < NV; v++) u.p[v][dst] = t[v];
     }
 }
 
@@ -2199,7 +2212,10 @@ int link_neighbors(dgr_plan *plan) {
 int launch_halo_fill(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
     const uint32_t nb = (uint32_t)plan->hblocks.size();
     if (!nb || plan->nlinks == 0) return DGR_OK;
-    hipLaunchKernelGGL(k_halo_fill, dim3(nb, NV), dim3(256), 0, st, u, plan->d_blocks, plan->d_nbr);
+    int64_t maxn = 0;
+    for (const DBlock &d : plan->hblocks) maxn = std::max<int64_t>(maxn, (int64_t)d.n[0] * d.n[1] * d.n[2]);
+    hipLaunchKernelGGL(k_halo_fill, dim3(nb, (uint32_t)((maxn + 255) / 256)), dim3(256), 0, st, u, plan->d_blocks,
+                       plan->d_nbr);
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }
