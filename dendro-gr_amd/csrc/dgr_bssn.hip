@@ -1078,22 +1078,26 @@ __global__ __launch_bounds__(256) void k_pad_copy(VarPtrs dst, VarPtrs src, cons
 // is b's minus d*(n-2*PW-1) along each axis; every source is an interior point
 // and every destination a padding point, so blocks can be filled in place in
 // any order. nbr: 27 entries per block, -1 for none (that padding is kept).
-// One thread per point, workgroups over (block, 256-point chunk): the
+// One thread per point, one workgroup per 256-point chunk of a block
+// (blockIdx.x = chunk * nblocks + block: consecutive workgroups take the same
+// chunk of consecutive blocks, which measured faster than a block's chunks
+// in a row): the
 // neighbours' offsets are staged in LDS, and each padding point's index
 // arithmetic is done once for all 24 variables, whose loads are issued
 // together before the stores.
 __global__ __launch_bounds__(256) void k_halo_fill(VarPtrs u, const DBlock *__restrict__ blocks,
-                                                   const int *__restrict__ nbr) {
+                                                   const int *__restrict__ nbr, int nblocks) {
     __shared__ int64_t s_off[27];
-    const DBlock &B = blocks[blockIdx.x];
+    const int b = blockIdx.x % nblocks, chunk = blockIdx.x / nblocks;
+    const DBlock &B = blocks[b];
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const int n = nx * ny * nz;
     if (threadIdx.x < 27) {
-        const int nb = nbr[27 * (size_t)blockIdx.x + threadIdx.x];
+        const int nb = nbr[27 * (size_t)b + threadIdx.x];
         s_off[threadIdx.x] = (nb >= 0 && threadIdx.x != 13) ? (int64_t)blocks[nb].offset : -1;
     }
     __syncthreads();
-    const int p = blockIdx.y * blockDim.x + threadIdx.x;
+    const int p = chunk * blockDim.x + threadIdx.x;
     if (p < n) {
         const int i = p % nx, j = (p / nx) % ny, k = p / (nx * ny);
         const int di = i < PW ? -1 : (i >= nx - PW ? 1 : 0);
@@ -2214,8 +2218,10 @@ int launch_halo_fill(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
     if (!nb || plan->nlinks == 0) return DGR_OK;
     int64_t maxn = 0;
     for (const DBlock &d : plan->hblocks) maxn = std::max<int64_t>(maxn, (int64_t)d.n[0] * d.n[1] * d.n[2]);
-    hipLaunchKernelGGL(k_halo_fill, dim3(nb, (uint32_t)((maxn + 255) / 256)), dim3(256), 0, st, u, plan->d_blocks,
-                       plan->d_nbr);
+    const int64_t nch = (maxn + 255) / 256;
+    if ((int64_t)nb * nch > INT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many blocks x points");
+    hipLaunchKernelGGL(k_halo_fill, dim3((uint32_t)(nb * nch)), dim3(256), 0, st, u, plan->d_blocks, plan->d_nbr,
+                       (int)nb);
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }
@@ -2244,7 +2250,9 @@ int rk4_fused_launches(dgr_plan *plan, const VarPtrs &u, double dt, const RhsPar
         io.enforce_stage = enforce_per_stage ? 1 : 0;
         io.err = plan->d_err;
         int rc;
-        if (plan->halo && (rc = launch_halo_fill(plan, io.src, st))) return rc;
+        // stage 0 reads u, whose padding the entry point (before the first
+        // step) or the previous step (after it) has refreshed
+        if (plan->halo && s > 0 && (rc = launch_halo_fill(plan, io.src, st))) return rc;
         if ((rc = run_fused(plan, io, prm, st))) return rc;
     }
     if (plan->halo) return launch_halo_fill(plan, u, st);
@@ -2265,7 +2273,9 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
     HIP_OK(hipSetDevice(plan->device));
     hipStream_t st = (hipStream_t)stream;
     if (plan->fused) {
-        if ((rc = ensure_istage(plan)) || (rc = launch_pad_copy(plan, u, st))) return rc;
+        if ((rc = ensure_istage(plan)) || (plan->halo && (rc = launch_halo_fill(plan, u, st))) ||
+            (rc = launch_pad_copy(plan, u, st)))
+            return rc;
         return rk4_fused_launches(plan, u, dt, to_params(params), enforce_per_stage, st);
     }
     if (plan->halo) return fail(DGR_E_INVALID, "the padding refresh needs the fused path (DGR_KERNEL=split is set)");
@@ -2344,7 +2354,9 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         key.enforce = enf;
         key.halo = plan->halo;
     }
-    if (nsteps > 0 && (rc = launch_pad_copy(plan, u, (hipStream_t)stream))) return rc;
+    if (nsteps > 0 && ((plan->halo && (rc = launch_halo_fill(plan, u, (hipStream_t)stream))) ||
+                       (rc = launch_pad_copy(plan, u, (hipStream_t)stream))))
+        return rc;
     for (int i = 0; i < nsteps; i++) HIP_OK(hipGraphLaunch(plan->rk4_exec, (hipStream_t)stream));
     return DGR_OK;
 }
