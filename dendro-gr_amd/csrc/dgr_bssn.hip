@@ -590,6 +590,11 @@ struct StageIO {
     int stage;     // -1 in RHS mode
     int enforce_stage;
     int *err;
+    // 27 per block: the offset of the same-level neighbour in each direction
+    // (-1 for none), read only by the HALO instantiation of k_fused, which
+    // stages region points inside a linked neighbour from that neighbour's
+    // interior instead of the block's own padding
+    const int64_t *nbo;
 };
 
 // FACE: the tile's block has flagged faces (closures and boundary values are
@@ -939,7 +944,7 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const T *s_ko,
     }
 }
 
-template <bool FACE, class C, typename T>
+template <bool FACE, class C, typename T, bool HALO = false>
 __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &tl, const DBlock &B,
                                            const dgr::AxisConsts *K, const uint16_t *__restrict__ tabs,
                                            int dtab, double *s_tile, double *s_dx, double *s_dy, T *s_ko) {
@@ -989,14 +994,40 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     // region elements: e = tid + c*FT -> (ii, jj, kk) in the 13 x 13 x (ez+6) box;
     // elements outside the region (x or y beyond a short tile, z beyond ez+6)
     // load the block's first point and are never read
+    if constexpr (HALO) {
+        // offsets are indices into the whole variable (below 2^32: the plan
+        // refuses larger variables); a padding point inside a linked
+        // same-level neighbour reads that neighbour's interior point at the
+        // same coordinates (node index minus d (n - 2 PW - 1) per axis)
+        ctx.boff = 0;
+        const int64_t *__restrict__ nbo = io.nbo + 27 * (size_t)tl.block;
 #pragma unroll
-    for (int c = 0; c < PREF; c++) {
-        const int e = tid + c * FT;
-        const int kk = e / FRXY, r = e - kk * FRXY;
-        const int jj = r / FRX, ii = r - jj * FRX;
-        const bool in_region = ii < ex + 2 * PW && jj < ey + 2 * PW && kk < ez + 2 * PW;
-        const int gi = ctx.ox - PW + ii, gj = ctx.oy - PW + jj, gk = ctx.oz - PW + kk;
-        ctx.goff[c] = in_region ? (uint32_t)(gi + nx * (gj + ny * gk)) : 0u;
+        for (int c = 0; c < PREF; c++) {
+            const int e = tid + c * FT;
+            const int kk = e / FRXY, r = e - kk * FRXY;
+            const int jj = r / FRX, ii = r - jj * FRX;
+            const bool in_region = ii < ex + 2 * PW && jj < ey + 2 * PW && kk < ez + 2 * PW;
+            const int gi = ctx.ox - PW + ii, gj = ctx.oy - PW + jj, gk = ctx.oz - PW + kk;
+            const int di = gi < PW ? -1 : (gi >= nx - PW ? 1 : 0);
+            const int dj = gj < PW ? -1 : (gj >= ny - PW ? 1 : 0);
+            const int dk = gk < PW ? -1 : (gk >= nz - PW ? 1 : 0);
+            const int64_t o = (in_region && (di | dj | dk)) ? nbo[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)] : -1;
+            const uint64_t a =
+                o >= 0 ? (uint64_t)o + (uint64_t)((gi - di * (nx - 2 * PW - 1)) +
+                                                   nx * ((gj - dj * (ny - 2 * PW - 1)) + ny * (gk - dk * (nz - 2 * PW - 1))))
+                       : B.offset + (uint64_t)(in_region ? gi + nx * (gj + ny * gk) : 0);
+            ctx.goff[c] = (uint32_t)a;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < PREF; c++) {
+            const int e = tid + c * FT;
+            const int kk = e / FRXY, r = e - kk * FRXY;
+            const int jj = r / FRX, ii = r - jj * FRX;
+            const bool in_region = ii < ex + 2 * PW && jj < ey + 2 * PW && kk < ez + 2 * PW;
+            const int gi = ctx.ox - PW + ii, gj = ctx.oy - PW + jj, gk = ctx.oz - PW + kk;
+            ctx.goff[c] = in_region ? (uint32_t)(gi + nx * (gj + ny * gk)) : 0u;
+        }
     }
     // where this thread's D_x / D_y evaluations go (the config's slot table)
 #pragma unroll
@@ -1037,7 +1068,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
 #define DGR_FP32_WAVES 2  // waves per SIMD the FP32 kernels are built for
 #endif
 
-template <bool FACE, class C, typename T>
+template <bool FACE, class C, typename T, bool HALO = false>
 __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES : 1) void k_fused(
     const StageIO io, const RhsParams prm, const DBlock *__restrict__ blocks,
     const dgr::AxisConsts *__restrict__ bconst, const Tile *__restrict__ tiles,
@@ -1052,7 +1083,8 @@ __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES 
 
     const Tile tl = tiles[blockIdx.x];
     const DBlock B = blocks[tl.block];
-    fused_tile<FACE, C, T>(io, prm, tl, B, bconst + 3 * (size_t)tl.block, tabs, dtab, s_tile, s_dx, s_dy, s_ko);
+    fused_tile<FACE, C, T, HALO>(io, prm, tl, B, bconst + 3 * (size_t)tl.block, tabs, dtab, s_tile, s_dx, s_dy,
+                                 s_ko);
 }
 
 // Copies the padding points of every block of src into dst (one workgroup per
@@ -1347,6 +1379,7 @@ struct dgr_plan {
     int dtab_wide = 0, dtab_thin = 0, dtab_full = 0;  // offsets of the configurations' slot tables
     std::vector<int> hnbr;      // same-level neighbour table, 27 per block (built on first use)
     int *d_nbr = nullptr;       // its device copy
+    int64_t *d_nbo = nullptr;   // the same as neighbour block offsets (-1 for none), for k_fused
     int64_t nlinks = -1;        // links in the table (-1: not built yet)
     bool halo = false;          // dgr_rk4_step(s): refresh padding from neighbours before each stage
     double *d_istage = nullptr; // k_fused RK4: 2 * 24 * interior (u0 copy, accumulator)
@@ -1616,6 +1649,8 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
     g_err.clear();
     if (!out || (!blocks && nblocks)) return fail(DGR_E_INVALID, "null argument");
     *out = nullptr;
+    // the fused kernel's in-kernel padding refresh indexes a variable with 32 bits
+    if (szpdof >= (1ull << 32)) return fail(DGR_E_INVALID, "szpdof must be below 2^32 points per variable");
     std::vector<DBlock> hb(nblocks);
     for (uint32_t b = 0; b < nblocks; b++) {
         const dgr_block &s = blocks[b];
@@ -1789,6 +1824,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     (void)hipFree(plan->d_istage);
     (void)hipFree(plan->d_wstate);
     (void)hipFree(plan->d_nbr);
+    (void)hipFree(plan->d_nbo);
     (void)hipFree(plan->d_err);
     (void)hipFree(plan->d_norm);
     if (plan->rk4_exec) (void)hipGraphExecDestroy(plan->rk4_exec);
@@ -1918,11 +1954,12 @@ int run_chunks(dgr_plan *plan, const VarPtrs &state, const VarPtrs &rhs, const R
 
 // Address of a fused kernel: the FP64 ones live in this unit, the FP32 ones
 // in the unit compiled with DGR_F32_KERNELS (see there).
-template <bool FACE, class C, typename T>
+template <bool FACE, class C, typename T, bool HALO>
 const void *fused_kernel() {
     if constexpr (sizeof(T) == sizeof(double)) {
-        return reinterpret_cast<const void *>(&k_fused<FACE, C, T>);
+        return reinterpret_cast<const void *>(&k_fused<FACE, C, T, HALO>);
     } else {
+        static_assert(!HALO, "the in-kernel padding refresh is FP64 only");
         static const void *const *tab = [] {
             static const void *k[4];
             dgr_internal_fused_f32_kernels(k);
@@ -1935,7 +1972,7 @@ const void *fused_kernel() {
     }
 }
 
-template <bool FACE, class C, typename T>
+template <bool FACE, class C, typename T, bool HALO = false>
 void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, int dtab, const StageIO &io,
                   const RhsParams &prm, hipStream_t st) {
     const int n = fg[group + 1] - fg[group];
@@ -1950,7 +1987,7 @@ void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, i
     int a_dtab = dtab;
     void *args[] = {&a_io, &a_prm, &a_blocks, &a_bconst, &a_tiles, &a_tabs, &a_dtab};
     timed(plan, 3 + group, st, [&] {
-        (void)hipLaunchKernel(fused_kernel<FACE, C, T>(), dim3(n), dim3(C::FT), args, 0, st);
+        (void)hipLaunchKernel(fused_kernel<FACE, C, T, HALO>(), dim3(n), dim3(C::FT), args, 0, st);
     });
 }
 
@@ -1958,6 +1995,14 @@ void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, i
 template <typename T>
 void launch_fused_all(dgr_plan *plan, const Tile *tiles, const int *fg, const StageIO &io, const RhsParams &prm,
                       hipStream_t st) {
+    if constexpr (sizeof(T) == sizeof(double)) {
+        if (io.nbo) {  // the padding refresh inside the stage kernel
+            launch_fused<false, CfgWide, T, true>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
+            launch_fused<false, CfgThin, T, true>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
+            launch_fused<true, CfgWide, T, true>(plan, tiles, fg, FG_FACE, plan->dtab_wide, io, prm, st);
+            return;
+        }
+    }
     launch_fused<false, typename CfgOf<T>::Wide, T>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
     launch_fused<false, typename CfgOf<T>::Thin, T>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
     if constexpr (sizeof(T) < sizeof(double))
@@ -2209,6 +2254,14 @@ int link_neighbors(dgr_plan *plan) {
         if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "neighbour table");
     }
     if (nb) HIP_OK(hipMemcpy(plan->d_nbr, plan->hnbr.data(), sizeof(int) * 27 * nb, hipMemcpyHostToDevice));
+    std::vector<int64_t> off(27 * nb);
+    for (size_t i = 0; i < off.size(); i++)
+        off[i] = plan->hnbr[i] >= 0 ? (int64_t)plan->hblocks[plan->hnbr[i]].offset : -1;
+    if (!plan->d_nbo && nb) {
+        hipError_t e = hipMalloc(&plan->d_nbo, sizeof(int64_t) * 27 * nb);
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "neighbour table");
+    }
+    if (nb) HIP_OK(hipMemcpy(plan->d_nbo, off.data(), sizeof(int64_t) * 27 * nb, hipMemcpyHostToDevice));
     plan->nlinks = links;
     return DGR_OK;
 }
@@ -2225,6 +2278,11 @@ int launch_halo_fill(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }
+
+// With the padding refresh on, FP64 stages read linked padding straight from
+// the neighbours (k_fused<..., HALO>); the mixed-precision kernels have no such
+// variant and get the padding copied in by k_halo_fill before each stage.
+bool halo_in_kernel(const dgr_plan *plan) { return plan->halo && plan->precision == DGR_PRECISION_FP64; }
 
 // The four stage launches of one RK4 step on the fused path (rkBSSN.cpp:764-860):
 // stages read u, W, u, W and write the next state into W, u, W, u (see
@@ -2250,12 +2308,16 @@ int rk4_fused_launches(dgr_plan *plan, const VarPtrs &u, double dt, const RhsPar
         io.enforce_stage = enforce_per_stage ? 1 : 0;
         io.err = plan->d_err;
         int rc;
-        // stage 0 reads u, whose padding the entry point (before the first
-        // step) or the previous step (after it) has refreshed
-        if (plan->halo && s > 0 && (rc = launch_halo_fill(plan, io.src, st))) return rc;
+        if (halo_in_kernel(plan)) {
+            io.nbo = plan->d_nbo;  // FP64: the stage kernel reads the neighbours itself
+        } else if (plan->halo && s > 0 && (rc = launch_halo_fill(plan, io.src, st))) {
+            // mixed precision: copy first; stage 0 reads u, whose padding the
+            // entry point (before the first step) or the previous step has refreshed
+            return rc;
+        }
         if ((rc = run_fused(plan, io, prm, st))) return rc;
     }
-    if (plan->halo) return launch_halo_fill(plan, u, st);
+    if (plan->halo && !halo_in_kernel(plan)) return launch_halo_fill(plan, u, st);
     return DGR_OK;
 }
 
@@ -2273,10 +2335,12 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
     HIP_OK(hipSetDevice(plan->device));
     hipStream_t st = (hipStream_t)stream;
     if (plan->fused) {
-        if ((rc = ensure_istage(plan)) || (plan->halo && (rc = launch_halo_fill(plan, u, st))) ||
-            (rc = launch_pad_copy(plan, u, st)))
+        if ((rc = ensure_istage(plan)) || (plan->halo && !halo_in_kernel(plan) && (rc = launch_halo_fill(plan, u, st))) ||
+            (rc = launch_pad_copy(plan, u, st)) ||
+            (rc = rk4_fused_launches(plan, u, dt, to_params(params), enforce_per_stage, st)))
             return rc;
-        return rk4_fused_launches(plan, u, dt, to_params(params), enforce_per_stage, st);
+        // in-kernel refresh: the caller's padding, refreshed from the new interiors
+        return halo_in_kernel(plan) ? launch_halo_fill(plan, u, st) : DGR_OK;
     }
     if (plan->halo) return fail(DGR_E_INVALID, "the padding refresh needs the fused path (DGR_KERNEL=split is set)");
     const size_t n = plan->szpdof;
@@ -2354,10 +2418,12 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         key.enforce = enf;
         key.halo = plan->halo;
     }
-    if (nsteps > 0 && ((plan->halo && (rc = launch_halo_fill(plan, u, (hipStream_t)stream))) ||
+    if (nsteps > 0 && ((plan->halo && !halo_in_kernel(plan) && (rc = launch_halo_fill(plan, u, (hipStream_t)stream))) ||
                        (rc = launch_pad_copy(plan, u, (hipStream_t)stream))))
         return rc;
     for (int i = 0; i < nsteps; i++) HIP_OK(hipGraphLaunch(plan->rk4_exec, (hipStream_t)stream));
+    // in-kernel refresh: the caller's padding, refreshed from the new interiors
+    if (nsteps > 0 && halo_in_kernel(plan)) return launch_halo_fill(plan, u, (hipStream_t)stream);
     return DGR_OK;
 }
 
