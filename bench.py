@@ -295,7 +295,8 @@ def halo_rate(wl, plan, state, stream, calls=20):
     ms = e0.elapsed_time(e1) / calls
     return {"links": int((nbr >= 0).sum()), "padding_points_filled": pts, "bytes_per_launch": byts,
             "ms_per_launch": round(ms, 4), "achieved_GBs": round(byts / ms / 1e6, 1),
-            "launches_per_rk4_step": 4}
+            "use": ("FP64 RK4 stages read linked padding inside k_fused (HALO variant); this fill "
+                    "runs once per RK4 call, leaving the caller's padding refreshed")}
 
 
 def constraint_rates(wl, plan, state, stream, calls=5):
