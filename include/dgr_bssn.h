@@ -208,11 +208,14 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
  * index (di+1)+3(dj+1)+9(dk+1), -1 for none; may be NULL) and nlinks (may be
  * NULL) receive it. dgr_halo_fill refreshes the padding of d_u in place
  * (asynchronous on `stream`; builds the table on first use).
- * dgr_plan_set_halo_refresh(plan, 1) makes dgr_rk4_step(s) refresh the
- * padding of each stage's state before its right-hand side and the caller's
- * padding after every step, as the reference's unzip between stages does
- * (rkBSSN.cpp:764-860); 0 (the default) keeps every block's padding at its
- * value at the start of the call. Needs the fused path. */
+ * dgr_plan_set_halo_refresh(plan, 1) makes every stage of dgr_rk4_step(s)
+ * see its input state's padding refreshed from the neighbours, as the
+ * reference's unzip between stages does (rkBSSN.cpp:764-860): FP64 stages
+ * read linked padding from the neighbours inside the stage kernel, mixed
+ * precision copies it in before each stage. When the call returns, the
+ * caller's padding is refreshed from the new interiors. 0 (the default) keeps
+ * every block's padding at its value at the start of the call. Needs the
+ * fused path. */
 int dgr_plan_link_neighbors(dgr_plan *plan, int32_t *nbr, int64_t *nlinks);
 int dgr_halo_fill(dgr_plan *plan, double *const *d_u, void *stream);
 int dgr_plan_set_halo_refresh(dgr_plan *plan, int on);
