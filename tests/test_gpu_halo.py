@@ -110,3 +110,31 @@ def test_refresh_off_keeps_padding_frozen():
     assert np.array_equal(got[:, ~inner], st[:, ~inner])
     assert normwise(interior_values(got - st, t), interior_values(ref - st, t)).max() <= 1e-11
     plan.close()
+
+
+def test_mixed_precision_rk4_with_refresh_stays_within_error():
+    """Mixed precision has no in-kernel refresh: the padding is copied in
+    before each stage. Two steps against the oracle's FP64 stage-by-stage
+    step: the increments within 1e-4 normwise (FP32 derivatives and
+    equations; the FP64 path meets 1e-11)."""
+    import torch
+    log("halo: mixed rk4")
+    t = lattice()
+    st = smooth_state(t, amp=1e-2)
+    nbr = W.neighbors(t)
+    p = dgr.default_params(ko_sigma=0.4)
+    plan = dgr.Plan(t, st.shape[1])
+    plan.set_precision("mixed")
+    plan.set_halo_refresh(True)
+    u = to_dev(st)
+    plan.rk4_steps(u, 0.02, p, nsteps=2)
+    torch.cuda.synchronize()
+    assert plan.status() == 0
+    ref = st.copy()
+    for _ in range(2):
+        O.rk4_step_halo(ref, t, nbr, 0.02, oracle_params(p), enforce_per_stage=True, nthreads=8)
+    got = u.cpu().numpy()
+    e = normwise(interior_values(got - st, t), interior_values(ref - st, t)).max()
+    log(f"  mixed interior increments normwise {e:.3e}")
+    assert e <= 1e-4
+    plan.close()
