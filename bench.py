@@ -76,10 +76,11 @@ class Workload:
         elif name == "lattice13":
             # 16^3 lattice of 13^3 blocks sharing boundary nodes: RK4 with the
             # padding refreshed from the neighbours before every stage
-            self.blocks = W.lattice(shape=(16, 16, 16), n=13, h=0.05, origin=(-4.8, -4.8, -4.8))
+            self.blocks = W.lattice(shape=(16, 16, 16), n=13, h=0.05, origin=(-4.8, -4.8, -4.8), order="morton")
             self.key = "lattice13x4096"
-            self.desc = ("lattice of 16^3 blocks of 13^3 sharing boundary nodes, padding refreshed "
-                         "from same-level neighbours before every RK4 stage, one RK4 step per step")
+            self.desc = ("lattice of 16^3 blocks of 13^3 sharing boundary nodes, stored in Morton order, "
+                         "padding refreshed from same-level neighbours before every RK4 stage, "
+                         "one RK4 step per step")
             self.dt = 0.25 * 0.05
             self.params = dict(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
         elif name == "puncture":

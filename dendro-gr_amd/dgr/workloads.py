@@ -39,18 +39,32 @@ def _table(sizes, pmins, hs, bflags=None, align=1):
     return t
 
 
+def _morton(a: int, b: int, c: int) -> int:
+    key = 0
+    for bit in range(21):
+        key |= ((a >> bit) & 1) << (3 * bit) | ((b >> bit) & 1) << (3 * bit + 1) | ((c >> bit) & 1) << (3 * bit + 2)
+    return key
+
+
 def lattice(shape=(3, 3, 2), n: int = 13, h: float = 0.05, origin=(-0.47, -0.31, -0.22), holes=(),
-            extra_level: bool = False) -> np.ndarray:
+            extra_level: bool = False, order: str = "shuffled") -> np.ndarray:
     """Blocks of n^3 points on a regular lattice of shape (nbx, nby, nbz):
     neighbours share their boundary nodes, so block (a,b,c) starts at
     origin + (a,b,c) (n-7) h. `holes` lists lattice cells left out; with
     extra_level, one block of spacing h/2 is appended (it links to nothing).
-    Blocks are stored in a shuffled order so the table order is not the
-    lattice order."""
+    order: "shuffled" (the table order is not the lattice order) or "morton"
+    (Z-order, as Dendro stores an octree's blocks along its space-filling
+    curve)."""
+    hole = set(holes)
     cells = [(a, b, c) for c in range(shape[2]) for b in range(shape[1]) for a in range(shape[0])
-             if (a, b, c) not in set(holes)]
-    order = np.random.default_rng(1).permutation(len(cells))
-    cells = [cells[i] for i in order]
+             if (a, b, c) not in hole]
+    if order == "morton":
+        cells.sort(key=lambda x: _morton(*x))
+    elif order == "shuffled":
+        perm = np.random.default_rng(1).permutation(len(cells))
+        cells = [cells[i] for i in perm]
+    else:
+        raise ValueError(order)
     step = (n - 7) * h
     pmins = [tuple(origin[d] + cells[q][d] * step for d in range(3)) for q in range(len(cells))]
     hs = [h] * len(cells)
