@@ -365,7 +365,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
-              f"by blocks with no collective, see DESIGN.md", file=sys.stderr)
+              f"by blocks, each GPU running its own plan, with no collective on the hot path", file=sys.stderr)
         sys.exit(2)
 
     import torch
