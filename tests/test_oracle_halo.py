@@ -126,3 +126,25 @@ def test_refresh_keeps_shared_boundary_nodes_consistent(refresh):
         assert worst < 1e-13
     else:
         assert worst > 1e-9
+
+
+def test_morton_order_links_the_same_lattice():
+    """The bench's Morton-ordered lattice is the same set of blocks: the same
+    number of links, each block keeping its neighbours by position."""
+    a = W.lattice(shape=(4, 4, 2), order="morton")
+    b = W.lattice(shape=(4, 4, 2), order="shuffled")
+    na, nb = W.neighbors(a), W.neighbors(b)
+    assert (na >= 0).sum() == (nb >= 0).sum()
+    pos_a = {tuple(np.round(p, 9)): i for i, p in enumerate(a["pmin"])}
+    pos_b = {tuple(np.round(p, 9)): i for i, p in enumerate(b["pmin"])}
+    assert pos_a.keys() == pos_b.keys()
+    for key, i in pos_a.items():
+        j = pos_b[key]
+        for d in range(27):
+            ka = tuple(np.round(a["pmin"][na[i, d]], 9)) if na[i, d] >= 0 else None
+            kb = tuple(np.round(b["pmin"][nb[j, d]], 9)) if nb[j, d] >= 0 else None
+            assert ka == kb
+    # Z-order: the first 8 blocks form the first 2x2x2 cube
+    h = (a["pmax"][0] - a["pmin"][0]) / 12
+    cells = np.rint((a["pmin"][:8] - a["pmin"][0]) / (6 * h)).astype(int)
+    assert {tuple(c) for c in cells} == {(i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)}
