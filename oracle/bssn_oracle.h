@@ -18,7 +18,17 @@
  *                                       (oracle/_ref, tests/golden/)
  *   orc_bssnrhs_block                   golden block outputs (reference algebra
  *                                       + these stencils), Minkowski => 0,
- *                                       axis-permutation covariance
+ *                                       axis-permutation covariance; its glue
+ *                                       by closed forms (tests/
+ *                                       test_oracle_block_glue.py): the
+ *                                       radiative BC on f_inf + c r^-p for p =
+ *                                       falloff and falloff +- 1 on every face
+ *                                       plane and nowhere else, KO added after
+ *                                       it; eta(r) from B_rhs = -eta B on flat
+ *                                       data; the CAKO sqrt(chi) eps_gauge /
+ *                                       eps_other split from the KO of a
+ *                                       product of sines; each check fails on
+ *                                       a copy with a planted mistake
  *   orc_ham_point                       golden vectors from the reference's
  *                                       generated physconeqs.cpp compiled in
  *                                       place (oracle/_ref, tests/golden/);

@@ -7,6 +7,7 @@ this module; the product never does.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -63,49 +64,67 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def _configure(L):
+    """Argument and return types of the oracle's C entry points on a loaded library."""
+    dp = C.POINTER(C.c_double)
+    L.orc_default_params.argtypes = [C.POINTER(Params)]
+    L.orc_bssn_points.argtypes = [dp, C.c_int64, C.POINTER(Params), dp]
+    L.orc_bssnrhs_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                    C.POINTER(Params), C.c_int]
+    L.orc_bssnrhs_batch.restype = C.c_int
+    L.orc_rk4_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
+                               C.POINTER(Params), C.c_int, C.c_int]
+    L.orc_rk4_step.restype = C.c_int
+    L.orc_halo_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    L.orc_rk4_step_halo.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_double,
+                                    C.POINTER(Params), C.c_int, C.c_int]
+    L.orc_rk4_step_halo.restype = C.c_int
+    L.orc_ham_points.argtypes = [dp, C.c_int64, dp]
+    L.orc_ham_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+    L.orc_ham_batch.restype = C.c_int
+    L.orc_constraint_points.argtypes = [dp, C.c_int64, dp, dp]
+    L.orc_constraint_points_xyz.argtypes = [dp, dp, C.c_int64, dp, dp, dp]
+    L.orc_constraints_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                        C.c_int]
+    L.orc_constraints_batch.restype = C.c_int
+    L.orc_constraint_l2_norm.argtypes = [dp, C.c_void_p, C.c_int64, dp, C.c_double, C.c_double]
+    L.orc_constraint_l2_norm.restype = C.c_double
+    L.orc_enforce_constraints_point.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
+    L.orc_enforce_constraints_point.restype = C.c_int
+    for nm in ("orc_deriv644_x", "orc_deriv644_y", "orc_deriv644_z", "orc_deriv644_xx",
+               "orc_deriv644_yy", "orc_deriv644_zz", "orc_ko_deriv42_x", "orc_ko_deriv42_y",
+               "orc_ko_deriv42_z"):
+        f = getattr(L, nm)
+        f.argtypes = [dp, dp, C.c_double, C.POINTER(C.c_uint32), C.c_uint32]
+    L.orc_init_perturbed_minkowski.argtypes = [C.c_void_p, C.c_void_p, C.c_int64,
+                                               C.c_uint64, C.c_double]
+    L.orc_init_brill_lindquist.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
+                                           C.c_double, C.c_double]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         path = os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
-        dp = C.POINTER(C.c_double)
-        L.orc_default_params.argtypes = [C.POINTER(Params)]
-        L.orc_bssn_points.argtypes = [dp, C.c_int64, C.POINTER(Params), dp]
-        L.orc_bssnrhs_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
-                                        C.POINTER(Params), C.c_int]
-        L.orc_bssnrhs_batch.restype = C.c_int
-        L.orc_rk4_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
-                                   C.POINTER(Params), C.c_int, C.c_int]
-        L.orc_rk4_step.restype = C.c_int
-        L.orc_halo_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
-        L.orc_rk4_step_halo.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_double,
-                                        C.POINTER(Params), C.c_int, C.c_int]
-        L.orc_rk4_step_halo.restype = C.c_int
-        L.orc_ham_points.argtypes = [dp, C.c_int64, dp]
-        L.orc_ham_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
-        L.orc_ham_batch.restype = C.c_int
-        L.orc_constraint_points.argtypes = [dp, C.c_int64, dp, dp]
-        L.orc_constraint_points_xyz.argtypes = [dp, dp, C.c_int64, dp, dp, dp]
-        L.orc_constraints_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
-                                            C.c_int]
-        L.orc_constraints_batch.restype = C.c_int
-        L.orc_constraint_l2_norm.argtypes = [dp, C.c_void_p, C.c_int64, dp, C.c_double, C.c_double]
-        L.orc_constraint_l2_norm.restype = C.c_double
-        L.orc_enforce_constraints_point.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
-        L.orc_enforce_constraints_point.restype = C.c_int
-        for nm in ("orc_deriv644_x", "orc_deriv644_y", "orc_deriv644_z", "orc_deriv644_xx",
-                   "orc_deriv644_yy", "orc_deriv644_zz", "orc_ko_deriv42_x", "orc_ko_deriv42_y",
-                   "orc_ko_deriv42_z"):
-            f = getattr(L, nm)
-            f.argtypes = [dp, dp, C.c_double, C.POINTER(C.c_uint32), C.c_uint32]
-        L.orc_init_perturbed_minkowski.argtypes = [C.c_void_p, C.c_void_p, C.c_int64,
-                                                   C.c_uint64, C.c_double]
-        L.orc_init_brill_lindquist.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
-                                               C.c_double, C.c_double]
-        _lib = L
+        _lib = _configure(C.CDLL(path))
     return _lib
+
+
+@contextlib.contextmanager
+def using_library(path: str):
+    """Runs the block in this module against another build of the oracle
+    (tests that plant a mistake in a copy of the source use it to show that
+    the pins catch it); the default build is restored afterwards."""
+    global _lib
+    saved = _lib
+    _lib = _configure(C.CDLL(os.path.abspath(path)))
+    try:
+        yield _lib
+    finally:
+        _lib = saved
 
 
 def ref_lib():
