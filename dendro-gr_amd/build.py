@@ -3,6 +3,8 @@
   lib/libdgr_bssn.so          HIP kernels + C-ABI + C++ shims, gfx950
   lib/libdgr_algebra_host.so  host build of the generated equations, used by
                               the CPU tests to check the generated code itself
+  ../tests/cpp/shim_driver    test program calling the reference-signature
+                              entry points (bssnRHS, bssnrhs) of libdgr_bssn.so
 
 Usage: python build.py [--regen]   (--regen reruns the equation generator)
 """
@@ -44,8 +46,18 @@ def build_bssn(src: str, out: str, verbose: bool = False, extra=()) -> None:
     """libdgr_bssn.so from the HIP source `src` (two units) and the C++ shim."""
     inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc")]
     objs = [out + ".main.o", out + ".f32.o"]
-    _run([HIPCC, *HIP_FLAGS, *MAIN_FLAGS, *extra, "-c", *inc, src, "-o", objs[0]], verbose)
-    _run([HIPCC, *HIP_FLAGS, *extra, "-DDGR_F32_KERNELS", "-c", *inc, src, "-o", objs[1]], verbose)
+    # the two units compile in parallel
+    cmds = [[HIPCC, *HIP_FLAGS, *MAIN_FLAGS, *extra, "-c", *inc, src, "-o", objs[0]],
+            [HIPCC, *HIP_FLAGS, *extra, "-DDGR_F32_KERNELS", "-c", *inc, src, "-o", objs[1]]]
+    procs = []
+    for c in cmds:
+        if verbose:
+            print(" ".join(c), flush=True)
+        procs.append(subprocess.Popen(c))
+    rcs = [p.wait() for p in procs]
+    for c, rc in zip(cmds, rcs):
+        if rc:
+            raise subprocess.CalledProcessError(rc, c)
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", *inc, *objs, os.path.join(HERE, "host", "bssn_shim.cpp"), "-o", tmp],
          verbose)
@@ -62,6 +74,16 @@ def build(verbose: bool = False) -> None:
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", *inc,
           os.path.join(HERE, "csrc", "algebra_host.cpp"), "-o", host + ".tmp"], verbose)
     os.replace(host + ".tmp", host)
+    build_shim_driver(verbose)
+
+
+def build_shim_driver(verbose: bool = False) -> None:
+    """tests/cpp/shim_driver: host program linked against lib/libdgr_bssn.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "shim_driver.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "shim_driver")
+    _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-L", LIB, "-ldgr_bssn",
+          "-Wl,-rpath,$ORIGIN/../../dendro-gr_amd/lib", "-o", out + ".tmp"], verbose)
+    os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

@@ -1612,6 +1612,8 @@ hipError_t build_ftiles(dgr_plan *plan, int p) {
     return hipSuccess;
 }
 
+int link_neighbors(dgr_plan *plan);
+
 }  // namespace
 
 extern "C" {
@@ -1834,6 +1836,36 @@ int dgr_plan_destroy(dgr_plan *plan) {
 }
 
 int64_t dgr_plan_interior_points(const dgr_plan *plan) { return plan ? plan->interior : -1; }
+
+int dgr_plan_move_block(dgr_plan *plan, uint32_t b, uint64_t offset, const double *pmin, const double *pmax) {
+    g_err.clear();
+    if (!plan || !pmin || !pmax) return fail(DGR_E_INVALID, "null argument");
+    if (b >= plan->hblocks.size()) return fail(DGR_E_INVALID, "block index out of range");
+    DBlock d = plan->hblocks[b];
+    const uint64_t n3 = (uint64_t)d.n[0] * d.n[1] * d.n[2];
+    if (offset + n3 > plan->szpdof || offset + n3 < offset)
+        return fail(DGR_E_INVALID, "block " + std::to_string(b) + " extends past szpdof");
+    d.offset = offset;
+    for (int a = 0; a < 3; a++) {
+        d.pmin[a] = pmin[a];
+        d.h[a] = (pmax[a] - pmin[a]) / (d.n[a] - 1);  // rhs.cpp:144-146
+        if (!(d.h[a] > 0.0) || !std::isfinite(d.h[a]))
+            return fail(DGR_E_INVALID, "block " + std::to_string(b) + ": pmax must exceed pmin");
+    }
+    dgr::AxisConsts k[3];
+    for (int a = 0; a < 3; a++) k[a] = dgr::axis_consts(d.h[a]);
+    HIP_OK(hipSetDevice(plan->device));
+    // earlier launches may still read the tables
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(plan->d_blocks + b, &d, sizeof(DBlock), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(plan->d_bconst + 3 * (size_t)b, k, sizeof(k), hipMemcpyHostToDevice));
+    plan->hblocks[b] = d;
+    if (plan->nlinks >= 0) {  // the neighbour table follows the geometry
+        int rc = link_neighbors(plan);
+        if (rc) return rc;
+    }
+    return DGR_OK;
+}
 
 int dgr_plan_set_precision(dgr_plan *plan, int precision) {
     if (!plan) return fail(DGR_E_INVALID, "null plan");
@@ -2189,12 +2221,20 @@ int ensure_istage(dgr_plan *plan) {
     return DGR_OK;
 }
 
+// The RK4 work buffers of the fused path: the padded work state W (24 x
+// szpdof doubles) and the interior-only u0 and accumulator (2 x 24 x interior).
+struct RkBuffers {
+    double *wstate;
+    double *istage;
+};
+RkBuffers plan_rk_buffers(const dgr_plan *plan) { return {plan->d_wstate, plan->d_istage}; }
+
 // W's padding <- u's (before the steps of one RK4 entry-point call)
-int launch_pad_copy(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
+int launch_pad_copy(dgr_plan *plan, const RkBuffers &buf, const VarPtrs &u, hipStream_t st) {
     const uint32_t nb = (uint32_t)plan->hblocks.size();
     if (!nb) return DGR_OK;
     VarPtrs w;
-    for (int v = 0; v < NV; v++) w.p[v] = plan->d_wstate + (size_t)v * plan->szpdof;
+    for (int v = 0; v < NV; v++) w.p[v] = buf.wstate + (size_t)v * plan->szpdof;
     hipLaunchKernelGGL(k_pad_copy, dim3(nb, NV), dim3(256), 0, st, w, u, plan->d_blocks);
     HIP_OK(hipGetLastError());
     return DGR_OK;
@@ -2287,14 +2327,14 @@ bool halo_in_kernel(const dgr_plan *plan) { return plan->halo && plan->precision
 // The four stage launches of one RK4 step on the fused path (rkBSSN.cpp:764-860):
 // stages read u, W, u, W and write the next state into W, u, W, u (see
 // StageIO). Needs ensure_istage() first.
-int rk4_fused_launches(dgr_plan *plan, const VarPtrs &u, double dt, const RhsParams &prm, int enforce_per_stage,
-                       hipStream_t st) {
+int rk4_fused_launches(dgr_plan *plan, const RkBuffers &buf, const VarPtrs &u, double dt, const RhsParams &prm,
+                       int enforce_per_stage, hipStream_t st) {
     const size_t ni = (size_t)plan->interior;
     VarPtrs w, u0, acc;
     for (int v = 0; v < NV; v++) {
-        w.p[v] = plan->d_wstate + (size_t)v * plan->szpdof;
-        u0.p[v] = plan->d_istage + (size_t)v * ni;
-        acc.p[v] = plan->d_istage + (size_t)(NV + v) * ni;
+        w.p[v] = buf.wstate + (size_t)v * plan->szpdof;
+        u0.p[v] = buf.istage + (size_t)v * ni;
+        acc.p[v] = buf.istage + (size_t)(NV + v) * ni;
     }
     for (int s = 0; s < 4; s++) {
         StageIO io{};
@@ -2336,8 +2376,8 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
     hipStream_t st = (hipStream_t)stream;
     if (plan->fused) {
         if ((rc = ensure_istage(plan)) || (plan->halo && !halo_in_kernel(plan) && (rc = launch_halo_fill(plan, u, st))) ||
-            (rc = launch_pad_copy(plan, u, st)) ||
-            (rc = rk4_fused_launches(plan, u, dt, to_params(params), enforce_per_stage, st)))
+            (rc = launch_pad_copy(plan, plan_rk_buffers(plan), u, st)) ||
+            (rc = rk4_fused_launches(plan, plan_rk_buffers(plan), u, dt, to_params(params), enforce_per_stage, st)))
             return rc;
         // in-kernel refresh: the caller's padding, refreshed from the new interiors
         return halo_in_kernel(plan) ? launch_halo_fill(plan, u, st) : DGR_OK;
@@ -2398,7 +2438,7 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         }
         if (!plan->cap_stream) HIP_OK(hipStreamCreateWithFlags(&plan->cap_stream, hipStreamNonBlocking));
         HIP_OK(hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal));
-        rc = rk4_fused_launches(plan, u, dt, to_params(params), enf, plan->cap_stream);
+        rc = rk4_fused_launches(plan, plan_rk_buffers(plan), u, dt, to_params(params), enf, plan->cap_stream);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(plan->cap_stream, &g);
         if (rc) {
@@ -2419,7 +2459,7 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         key.halo = plan->halo;
     }
     if (nsteps > 0 && ((plan->halo && !halo_in_kernel(plan) && (rc = launch_halo_fill(plan, u, (hipStream_t)stream))) ||
-                       (rc = launch_pad_copy(plan, u, (hipStream_t)stream))))
+                       (rc = launch_pad_copy(plan, plan_rk_buffers(plan), u, (hipStream_t)stream))))
         return rc;
     for (int i = 0; i < nsteps; i++) HIP_OK(hipGraphLaunch(plan->rk4_exec, (hipStream_t)stream));
     // in-kernel refresh: the caller's padding, refreshed from the new interiors

@@ -147,6 +147,15 @@ int dgr_plan_tiling(const dgr_plan *plan, int precision, int max_sizes, uint32_t
 /* Interior points (sum over blocks of (nx-6)(ny-6)(nz-6)) the plan covers. */
 int64_t dgr_plan_interior_points(const dgr_plan *plan);
 
+/* Moves block b of the plan to another place in the unzipped arrays and in
+ * space (offset, pmin, pmax as in dgr_block), keeping its sizes and face
+ * flags, so one plan serves every block of one shape: the per-block form of
+ * the reference's loop (BSSN_GR/src/rhs.cpp:49-71 calls bssnrhs once per
+ * block with that block's offset, corner coordinates and spacing). Waits for
+ * the device (earlier launches may still read the block table); rebuilds the
+ * neighbour table when one was built. */
+int dgr_plan_move_block(dgr_plan *plan, uint32_t b, uint64_t offset, const double *pmin, const double *pmax);
+
 /* Kernel timing (no counterpart in the reference, whose profiler_t timers wrap
  * the phases of bssnrhs, include/profile_params.h:15-53): when enabled, every
  * launch is bracketed by HIP events on the caller's stream.
