@@ -1,20 +1,22 @@
 #!/usr/bin/env python
 """Benchmark of the BSSN right-hand side on one MI355X.
 
-Workloads (interior points are the non-padding points of all blocks):
-  uniform13 (default)  4096 uniform 13^3 padded blocks (7^3 interior each,
-                       1,404,928 points), seeded perturbed-Minkowski data,
-                       KO sigma 0.4, eta 2, dt 0.025
-  puncture             the uneven binary-puncture octree: 13^3/19^3/31^3/55^3
-                       blocks with counts 60000/8000/1000/150 x --scale
-                       (0.42 -> ~30M points), Brill-Lindquist data plus 1e-6
-                       noise, KO sigma 0.4, RIT eta, dt 0.25 dx_min
-One timed step = one full RK4 step (4 right-hand side evaluations: derivatives,
-equations, KO dissipation, stage update, constraint enforcement per stage),
-FP64, inputs resident in HBM, replayed from a captured HIP graph.
-value = interior points x 4 / step time.
-
-Prints one JSON line. Also reports:
+Default line (--workload octree): the uneven binary-puncture octree,
+13^3/19^3/31^3/55^3 blocks with counts 60000/8000/1000/150 x 0.42 (29043
+blocks, 30.0 M interior points), Brill-Lindquist data plus 1e-6 noise, KO
+sigma 0.4, RIT eta, dt 0.25 dx_min. One timed step = one full RK4 step (4
+right-hand side evaluations: derivatives, equations, KO dissipation, stage
+update, constraint enforcement per stage), FP64, inputs resident in HBM,
+replayed from a captured HIP graph; value = interior points x 4 / step time.
+Beside it, in the same line:
+  mixed_precision  the batched mixed-precision path on the same octree: its
+                rate, speedup, per-variable max and L2 errors against FP64
+                from identical inputs, and the Hamiltonian-constraint norm
+                after 10 RK4 steps in both precisions
+  uniform13     4096 uniform 13^3 blocks (1.40 M points, perturbed Minkowski,
+                eta 2, dt 0.025): rate, roofline, mixed-precision errors
+  large_octree  the same octree with 4x the block counts (120 M points),
+                --large-steps RK4 steps in FP64 and in mixed precision
   roofline      the dominant kernel (the fused kernel on the regular tiles of
                 blocks without flagged faces) against HBM: algorithmic bytes
                 per launch (per block 192 n^3 for reading the padded state once,
@@ -22,13 +24,14 @@ Prints one JSON line. Also reports:
                 update, split over the launches by the points each covers) over
                 its average launch time, measured with HIP events on the launch
                 stream; `traffic` is the HBM bytes per launch measured with
-                rocprofv3 counters (profiles/), when a summary for this
-                workload is committed
-  cpu_baseline  the oracle's OpenMP C restatement on the host cores, one RK4
-                step of a bounded sample of the same workload
+                rocprofv3 counters (profiles/), when a summary measured on the
+                current kernel sources is committed
+  cpu_baseline  the CPU path (the reference's generated equations inside the
+                oracle's block loop, OpenMP over blocks on `nproc` threads), one
+                RK4 step of a bounded sample of the workload, with the phase
+                split; and one 19^3 block on one thread
   rhs_only      right-hand side alone (no RK update), Mpoints/s
-  mixed_precision  (FP64 runs) the batched mixed-precision path on the same
-                workload: its RK4 Mpoints/s, speedup, and errors against FP64
+--workload uniform13 / puncture / lattice13 runs one workload alone.
 """
 import argparse
 import json
@@ -103,15 +106,17 @@ class Workload:
             return synth.perturbed_minkowski(self.blocks, seed=1, amp=1e-3, device=dev)
         return synth.brill_lindquist(self.blocks, seed=1, noise=1e-6, device=dev)
 
-    def cpu_sample(self, nblocks_uniform):
-        """A bounded sample of the same workload for the CPU baseline."""
+    def cpu_sample(self):
+        """A bounded sample of the same workload for the CPU baseline (1.4 M
+        interior points of uniform blocks, 2.5 M of the octree: one or two
+        seconds per RK4 step on 16 host threads)."""
         from dgr import workloads as W
         if self.name == "uniform13":
-            return (W.uniform_13(nblocks_uniform),
-                    f"the first {nblocks_uniform} blocks of the same uniform 13^3 batch")
+            n = min(len(self.blocks), 4096)
+            return W.uniform_13(n), f"the first {n} blocks of the same uniform 13^3 batch"
         # every k-th block of the octree, keeping the size mix (and enough
         # large blocks that OpenMP over blocks is not bound by one of them)
-        k = max(1, len(self.blocks) // 1400)
+        k = max(1, len(self.blocks) // 2000)
         sub = self.blocks[::k].copy()
         off = 0
         for b in range(len(sub)):
@@ -120,46 +125,141 @@ class Workload:
         return sub, f"every {k}th block of the same octree ({len(sub)} blocks)"
 
 
+# the sources the kernels are built from: a traffic summary in profiles/ is
+# used only when it was measured on a build of these exact files
+KERNEL_SOURCES = ("dendro-gr_amd/csrc/dgr_bssn.hip", "dendro-gr_amd/csrc/stencils.h",
+                  "dendro-gr_amd/csrc/bssn_rhs_phased.h", "dendro-gr_amd/csrc/bssn_rhs_generated.h",
+                  "dendro-gr_amd/csrc/bssn_ham_generated.h", "dendro-gr_amd/build.py")
+
+
+def source_sha():
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kind, key):
     """HBM bytes per launch of `kind` from the committed rocprofv3 counter
-    summaries (profiles/*traffic*.txt lines 'traffic <kind> <workload> <bytes>'),
-    else None."""
+    summaries (profiles/*traffic*.txt: a line 'source_sha <sha>' and lines
+    'traffic <kind> <workload> <bytes>'), only from a summary whose sha is
+    that of the current kernel sources. Returns (bytes or None, note)."""
     import glob
+    sha = source_sha()
+    stale = None
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.txt"))):
+        fsha, val = None, None
         with open(f) as fh:
             for line in fh:
                 p = line.split()
+                if len(p) == 2 and p[0] == "source_sha":
+                    fsha = p[1]
                 if len(p) == 4 and p[0] == "traffic" and p[1] == kind and p[2] == key:
-                    return float(p[3])
-    return None
+                    val = float(p[3])
+        if val is None:
+            continue
+        if fsha == sha:
+            return val, f"profiles/{os.path.basename(f)} (source_sha {sha})"
+        stale = f"profiles/{os.path.basename(f)} was measured on other kernel sources ({fsha} != {sha})"
+    return None, stale or f"no counter summary for {kind} on {key}"
 
 
-def cpu_baseline(wl, nblocks_uniform, threads, reps=3):
+def nproc():
+    try:
+        return int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def host_info():
+    info = {"nproc": nproc(), "os_cpu_count": os.cpu_count(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        info["nproc_all"] = int(subprocess.run(["nproc", "--all"], capture_output=True, text=True).stdout.strip())
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return info
+
+
+def time_cpu_rk4(O, st, blocks, dt, p, threads, reps, algebra):
+    """Median seconds of one oracle RK4 step over `reps` runs after a warm-up,
+    and the phase split (fractions of the summed thread time) of those runs."""
+    import contextlib
+    ctx = O.reference_block_algebra() if algebra == "reference" else contextlib.nullcontext()
+    times = []
+    with ctx:
+        for r in range(reps + 1):
+            s = st.copy()
+            if r == 1:
+                O.phase_times(reset=True)
+            t0 = time.perf_counter()
+            O.rk4_step(s, blocks, dt, p, enforce_per_stage=True, nthreads=threads)
+            if r > 0:
+                times.append(time.perf_counter() - t0)
+    ph = O.phase_times(reset=True)
+    tot = sum(ph.values()) or 1.0
+    return statistics.median(times), {k: round(v / tot, 4) for k, v in ph.items()}
+
+
+def cpu_baseline(wl, reps=5):
+    """The CPU path on the host cores: an RK4 step (4 right-hand sides) with the
+    reference's own generated equations (compiled in place, oracle/_ref) over
+    each block, inside the oracle's restatement of rhs.cpp's block loop and
+    derivs.cpp's stencils, OpenMP over blocks on `nproc` threads; median of
+    `reps` after a warm-up, on a bounded sample of the workload. Also the same
+    with the oracle's own tensor-loop algebra ('port'), the single-thread
+    19^3 block of the CPU configuration, and the phase split at the
+    reference's timer boundaries (deriv, rhs, bc, ko)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from dgr import workloads as W
 
-    t, what = wl.cpu_sample(nblocks_uniform)
+    info = host_info()
+    threads = info["nproc"]
+    algebra = "reference" if O.ref_lib() is not None else "port"
+    t, what = wl.cpu_sample()
     st = np.zeros((24, W.total_padded(t)))
-    if wl.name == "uniform13":
+    if wl.name in ("uniform13", "lattice13"):
         O.init_perturbed_minkowski(st, t, seed=1, amp=1e-3)
     else:
         O.init_brill_lindquist(st, t, seed=1, noise=1e-6)
     p = O.default_params(**wl.params)
     pts = W.interior_points(t)
-    times = []
-    for r in range(reps + 1):
-        s = st.copy()
-        t0 = time.perf_counter()
-        O.rk4_step(s, t, wl.dt, p, enforce_per_stage=True, nthreads=threads)
-        dt = time.perf_counter() - t0
-        if r > 0:
-            times.append(dt)
-    med = statistics.median(times)
-    return {"value": round(pts * 4 / med / 1e6, 3), "unit": "Mpoints/s", "cores": threads,
-            "kind": "port",
-            "sample": f"one RK4 step (4 RHS evaluations) of {what} ({pts} interior points), "
-                      f"OpenMP over blocks, median of {reps} after 1 warm-up"}
+    log(f"cpu baseline: {what}, {threads} threads, {algebra} algebra")
+    med, split = time_cpu_rk4(O, st, t, wl.dt, p, threads, reps, algebra)
+    res = {"value": round(pts * 4 / med / 1e6, 3), "unit": "Mpoints/s", "cores": threads, "kind": algebra,
+           "sample": (f"one RK4 step (4 right-hand sides, stage updates, constraint enforcement) of {what} "
+                      f"({pts} interior points), OpenMP over blocks on {threads} threads, median of {reps} "
+                      f"after 1 warm-up; " +
+                      ("the reference's generated equations (BSSN_GR/src/bssneqs_eta_const_standard_gauge.cpp, "
+                       "compiled in place) over each block, with the oracle's restatement of the derivative "
+                       "stencils, boundary pass, KO sum and RK4 around them" if algebra == "reference" else
+                       "the oracle's restatement (tensor-loop algebra)")),
+           "seconds_per_step": round(med, 4), "phase_split": split, **info}
+    if algebra == "reference":
+        med2, split2 = time_cpu_rk4(O, st, t, wl.dt, p, threads, reps, "port")
+        res["port"] = {"value": round(pts * 4 / med2 / 1e6, 3), "phase_split": split2,
+                       "algebra": "the oracle's tensor loops (no common subexpressions)"}
+    # the CPU configuration: one 19^3 block, one thread
+    t1 = W.single_block_19()
+    st1 = np.zeros((24, W.total_padded(t1)))
+    O.init_perturbed_minkowski(st1, t1, seed=1, amp=1e-3)
+    p1 = O.default_params(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
+    med1, split1 = time_cpu_rk4(O, st1, t1, 0.025, p1, 1, reps, algebra)
+    res["single_block_19_one_thread"] = {"value": round(W.interior_points(t1) * 4 / med1 / 1e6, 4),
+                                         "unit": "Mpoints/s", "seconds_per_step": round(med1, 5),
+                                         "phase_split": split1, "kind": algebra}
+    return res
 
 
 def mixed_comparison(wl, state0, params, steps, fp64_value, stream, autotune=True):
@@ -259,8 +359,9 @@ def roofline(wl, kt, stages, precision="fp64"):
     # algorithmic FLOP rate of the dominant kernel (halo recomputation not counted)
     tflops = kind_pts.get(dom, wl.npts) * FLOP_PER_POINT_STAGE / (dom_ms * 1e-3) / 1e12
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(dom, wl.key + precision_tag)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, wl.key + precision_tag),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
             "kernel": KERNEL_NAMES.get(dom, dom), "launch_ms": round(dom_ms, 5),
             "algorithmic_bytes_per_launch": dom_bytes,
             "algorithmic_bytes_per_point": round(W.stage_bytes(wl.blocks) / wl.npts, 1),
@@ -333,15 +434,152 @@ def constraint_rates(wl, plan, state, stream, calls=5):
     return res
 
 
+def run_fp64(wl, args, stream, dev, precision="fp64", keep_state0=True):
+    """The timed RK4 steps of one workload (graph replay), the per-launch kernel
+    times, the right-hand-side-only rate, the roofline of the dominant launch,
+    and the constraint diagnostic. Returns (result dict, initial state or None)."""
+    import torch
+    import dgr
+    log(f"workload: {wl.desc}; {wl.npts} interior points, state {24 * wl.n_pad * 8 / 1e9:.2f} GB")
+    state = wl.device_state(dev)
+    state0 = state.clone() if keep_state0 else None
+    plan = dgr.Plan(wl.blocks, wl.n_pad)
+    plan.set_precision(precision)
+    if wl.name == "lattice13":
+        plan.set_halo_refresh(True)
+    params = dgr.default_params(**wl.params)
+    if not args.no_autotune:
+        # tile shapes per block size, timed on this workload's own data
+        scratch = torch.empty_like(state)
+        plan.autotune(scratch, state, params, stream=stream)
+        torch.cuda.synchronize()
+        del scratch
+    # the plan allocates its RK4 buffers with hipMalloc: hand torch's cached
+    # blocks back first (the 120 M-point octree needs ~200 GB of them)
+    torch.cuda.empty_cache()
+    tiling = {"x".join(str(n) for n in k): v for k, v in plan.tiling(precision).items()}
+    log(f"tiling per block size ({'autotuned' if not args.no_autotune else 'default'}): {tiling}")
+
+    # the RK4 step is captured once as a HIP graph and replayed (dgr_rk4_steps)
+    plan.rk4_steps(state, wl.dt, params, nsteps=max(args.warmup, 1), stream=stream)
+    torch.cuda.synchronize()
+    if plan.status(stream) != 0:
+        raise RuntimeError("constraint violation during warm-up")
+    t0 = time.perf_counter()
+    plan.rk4_steps(state, wl.dt, params, nsteps=args.steps, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    if plan.status(stream) != 0:
+        raise RuntimeError("constraint violation during the timed steps")
+    elapsed = t1 - t0
+    res = {"value": round(wl.npts * 4 * args.steps / elapsed / 1e6, 2),
+           "ms_per_step": round(elapsed / args.steps * 1e3, 4), "tiling": tiling}
+    log(f"{wl.key}: {res['value']} Mpoints/s ({res['ms_per_step']} ms per RK4 step)")
+
+    # per-kernel times: the same steps launched eagerly with HIP events around each launch
+    ksteps = min(args.steps, 5)
+    plan.set_timing(True)
+    plan.kernel_times()
+    for _ in range(ksteps):
+        plan.rk4_step(state, wl.dt, params, enforce_per_stage=True, stream=stream)
+    kt = plan.kernel_times()
+    plan.set_timing(False)
+
+    # right-hand side alone (needs room for a padded output next to the state
+    # and the plan's RK4 buffers; skipped when HBM cannot hold it)
+    res["rhs_only_mpoints_s"] = None
+    free, _ = torch.cuda.mem_get_info()
+    if free > state.numel() * state.element_size() * 1.05:
+        out = torch.empty_like(state)
+        for _ in range(2):
+            plan.rhs(out, state, params, stream=stream)
+        torch.cuda.synchronize()
+        r0 = time.perf_counter()
+        nrhs = 4 * args.steps
+        for _ in range(nrhs):
+            plan.rhs(out, state, params, stream=stream)
+        torch.cuda.synchronize()
+        res["rhs_only_mpoints_s"] = round(wl.npts * nrhs / (time.perf_counter() - r0) / 1e6, 2)
+        del out
+    else:
+        log("right-hand-side-only rate skipped: no room for a second padded state")
+    res["roofline"] = roofline(wl, kt, 4 * ksteps, precision)
+    res["constraints"] = constraint_rates(wl, plan, state, stream)
+    res["halo_fill"] = halo_rate(wl, plan, state, stream) if wl.name == "lattice13" else None
+    plan.close()
+    del state
+    torch.cuda.empty_cache()
+    return res, state0
+
+
+def large_octree(args, stream, dev):
+    """The 120 M-point octree (the puncture distribution with 4x the block
+    counts): `--large-steps` graph-replayed RK4 steps in FP64, then in mixed
+    precision from the same initial data; sustained Mpoints/s, the algorithmic
+    HBM rate and FP64/FP32 vector-ALU rate of the whole stage, every value
+    finite, and the Hamiltonian-constraint norm after the steps in both."""
+    import torch
+    import dgr
+    wl = Workload("puncture", 0, args.large_scale)
+    log(f"large octree: {wl.desc}; {wl.npts} interior points, state {24 * wl.n_pad * 8 / 1e9:.1f} GB")
+    params = dgr.default_params(**wl.params)
+    plan = dgr.Plan(wl.blocks, wl.n_pad)
+    out = {"workload": wl.desc, "interior_points": wl.npts, "steps": args.large_steps,
+           "state_GB": round(24 * wl.n_pad * 8 / 1e9, 1)}
+    sbytes = W_stage_bytes(wl.blocks)
+    for prec in ("fp64", "mixed"):
+        state = wl.device_state(dev)
+        plan.set_precision(prec)
+        torch.cuda.empty_cache()
+        plan.rk4_steps(state, wl.dt, params, nsteps=1, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.rk4_steps(state, wl.dt, params, nsteps=args.large_steps, stream=stream)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        if plan.status(stream) != 0:
+            raise RuntimeError(f"constraint violation in the large octree ({prec})")
+        finite = bool(torch.isfinite(state).all().item())
+        ham = torch.empty(wl.n_pad, dtype=torch.float64, device=dev)
+        plan.ham(ham, state, stream=stream)
+        norm = plan.constraint_norm(ham, stream=stream)
+        del ham, state
+        stage_s = el / args.large_steps / 4
+        out[prec] = {"value": round(wl.npts * 4 * args.large_steps / el / 1e6, 2), "unit": "Mpoints/s",
+                     "ms_per_step": round(el / args.large_steps * 1e3, 3),
+                     "stage_algorithmic_GBs": round(sbytes / stage_s / 1e9, 1),
+                     "stage_tflops": round(wl.npts * FLOP_PER_POINT_STAGE / stage_s / 1e12, 2),
+                     "all_finite": finite,
+                     f"ham_constraint_norm_after_{args.large_steps + 1}_steps": norm}
+        log(f"large octree {prec}: {out[prec]['value']} Mpoints/s, finite {finite}, ham norm {norm:.6e}")
+    n64 = out["fp64"][f"ham_constraint_norm_after_{args.large_steps + 1}_steps"]
+    n32 = out["mixed"][f"ham_constraint_norm_after_{args.large_steps + 1}_steps"]
+    out["mixed_speedup_vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+    out["ham_norm_relative_difference"] = abs(n32 - n64) / abs(n64)
+    plan.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def W_stage_bytes(blocks):
+    from dgr import workloads as W
+    return W.stage_bytes(blocks)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["uniform13", "puncture", "lattice13"], default="uniform13")
+    ap.add_argument("--workload", choices=["octree", "uniform13", "puncture", "lattice13"], default="octree",
+                    help="octree (default): the puncture octree (scale 0.42) as the headline, with its "
+                         "mixed-precision leg, the 120 M-point octree and the uniform 13^3 batch beside it; "
+                         "the others run one workload alone")
     ap.add_argument("--nblocks", type=int, default=4096, help="uniform13: number of blocks")
     ap.add_argument("--scale", type=float, default=0.42, help="puncture: block-count scale")
-    ap.add_argument("--cpu-blocks", type=int, default=1024, help="uniform13: CPU baseline sample")
+    ap.add_argument("--large-scale", type=float, default=1.68, help="octree: scale of the large octree")
+    ap.add_argument("--large-steps", type=int, default=50, help="octree: RK4 steps on the large octree")
+    ap.add_argument("--no-large", action="store_true", help="octree: skip the large octree")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 derivatives/equations/KO on offset-subtracted fields, FP64 state")
     ap.add_argument("--no-mixed", action="store_true",
@@ -369,109 +607,63 @@ def main():
         sys.exit(2)
 
     import torch
-    import dgr
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    wl = Workload(args.workload, args.nblocks, args.scale)
-    log(f"workload: {wl.desc}; {wl.npts} interior points, state {24 * wl.n_pad * 8 / 1e9:.2f} GB")
-    state = wl.device_state(dev)
-    # the mixed comparison and the CPU baseline are for the frozen-padding step
-    if wl.name == "lattice13":
-        args.no_mixed = args.no_cpu_baseline = True
-    state0 = state.clone() if (args.precision == "fp64" and not args.no_mixed) else None
-    plan = dgr.Plan(wl.blocks, wl.n_pad)
-    plan.set_precision(args.precision)
-    if wl.name == "lattice13":
-        plan.set_halo_refresh(True)
-    params = dgr.default_params(**wl.params)
     stream = torch.cuda.current_stream()
-    if not args.no_autotune:
-        # tile shapes per block size, timed on this workload's own data
-        scratch = torch.empty_like(state)
-        plan.autotune(scratch, state, params, stream=stream)
-        torch.cuda.synchronize()
-        del scratch
-    # the plan allocates its RK4 buffers with hipMalloc: hand torch's cached
-    # blocks back first (the 120 M-point octree needs ~200 GB of them)
-    torch.cuda.empty_cache()
-    tiling = {"x".join(str(n) for n in k): v for k, v in plan.tiling(args.precision).items()}
-    log(f"tiling per block size ({'autotuned' if not args.no_autotune else 'default'}): {tiling}")
-
-    # the RK4 step is captured once as a HIP graph and replayed (dgr_rk4_steps)
-    plan.rk4_steps(state, wl.dt, params, nsteps=max(args.warmup, 1), stream=stream)
-    torch.cuda.synchronize()
-    if plan.status(stream) != 0:
-        raise RuntimeError("constraint violation during warm-up")
-    t0 = time.perf_counter()
-    plan.rk4_steps(state, wl.dt, params, nsteps=args.steps, stream=stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    if plan.status(stream) != 0:
-        raise RuntimeError("constraint violation during the timed steps")
-    elapsed = t1 - t0
-    ms_step = elapsed / args.steps * 1e3
-    value = wl.npts * 4 * args.steps / elapsed / 1e6
-
-    # per-kernel times: the same steps launched eagerly with HIP events around each launch
-    ksteps = min(args.steps, 5)
-    plan.set_timing(True)
-    plan.kernel_times()
-    for _ in range(ksteps):
-        plan.rk4_step(state, wl.dt, params, enforce_per_stage=True, stream=stream)
-    kt = plan.kernel_times()
-    plan.set_timing(False)
-
-    # right-hand side alone (needs room for a padded output next to the state
-    # and the plan's RK4 buffers; skipped when HBM cannot hold it)
-    rhs_only = None
-    free, _ = torch.cuda.mem_get_info()
-    if free > state.numel() * state.element_size() * 1.05:
-        out = torch.empty_like(state)
-        for _ in range(2):
-            plan.rhs(out, state, params, stream=stream)
-        torch.cuda.synchronize()
-        r0 = time.perf_counter()
-        nrhs = 4 * args.steps
-        for _ in range(nrhs):
-            plan.rhs(out, state, params, stream=stream)
-        torch.cuda.synchronize()
-        rhs_only = round(wl.npts * nrhs / (time.perf_counter() - r0) / 1e6, 2)
-        del out
-    else:
-        log("right-hand-side-only rate skipped: no room for a second padded state")
-    roof = roofline(wl, kt, 4 * ksteps, args.precision)
-    constraints = constraint_rates(wl, plan, state, stream)
-    halo = halo_rate(wl, plan, state, stream) if wl.name == "lattice13" else None
-    plan.close()
+    composite = args.workload == "octree"
+    wl = Workload("puncture" if composite else args.workload, args.nblocks, args.scale)
+    if wl.name == "lattice13":  # the comparisons are for the frozen-padding step
+        args.no_mixed = args.no_cpu_baseline = True
+    want_mixed = args.precision == "fp64" and not args.no_mixed
+    res, state0 = run_fp64(wl, args, stream, dev, args.precision, keep_state0=want_mixed)
 
     mixed = None
-    if state0 is not None:
-        mixed = mixed_comparison(wl, state0, params, args.steps, value, stream, not args.no_autotune)
-    del state
+    if want_mixed:
+        import dgr
+        mixed = mixed_comparison(wl, state0, dgr.default_params(**wl.params), args.steps, res["value"], stream,
+                                 not args.no_autotune)
+        log(f"mixed precision: {mixed['value']} Mpoints/s ({mixed['speedup_vs_fp64']}x), errors "
+            f"{json.dumps(mixed['error_vs_fp64'])}")
+    del state0
+    torch.cuda.empty_cache()
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(wl, args.cpu_blocks, threads)
-        try:
-            cpu["cpu_model"] = subprocess.run(["bash", "-c", "lscpu | grep 'Model name' | head -1"],
-                                              capture_output=True, text=True).stdout.split(":")[-1].strip()
-        except Exception:
-            pass
+    uniform = large = None
+    if composite:
+        # the uniform 13^3 batch beside the octree: its rate, roofline and mixed errors
+        wu = Workload("uniform13", args.nblocks, args.scale)
+        ru, su = run_fp64(wu, args, stream, dev, "fp64", keep_state0=not args.no_mixed)
+        uniform = {"workload": wu.desc, "interior_points": wu.npts, "value": ru["value"], "unit": "Mpoints/s",
+                   "ms_per_step": ru["ms_per_step"], "rhs_only_mpoints_s": ru["rhs_only_mpoints_s"],
+                   "roofline": ru["roofline"], "constraints": ru["constraints"], "tiling": ru["tiling"]}
+        if su is not None:
+            import dgr
+            uniform["mixed_precision"] = mixed_comparison(wu, su, dgr.default_params(**wu.params), args.steps,
+                                                          ru["value"], stream, not args.no_autotune)
+        del su
+        torch.cuda.empty_cache()
+        if not args.no_large:
+            free, _ = torch.cuda.mem_get_info()
+            need = 24 * 8 * (2 * Workload("puncture", 0, args.large_scale).n_pad) * 1.3
+            if free > need:
+                large = large_octree(args, stream, dev)
+            else:
+                log(f"large octree skipped: {free / 1e9:.0f} GB free, about {need / 1e9:.0f} GB needed")
 
-    line = {"metric": METRIC, "value": round(value, 2), "unit": "Mpoints/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+    cpu = None if args.no_cpu_baseline else cpu_baseline(wl)
+
+    line = {"metric": METRIC, "value": res["value"], "unit": "Mpoints/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64" if args.precision == "fp64" else "f32 arithmetic, f64 state",
             "data": "synthetic",
             "config": {"workload": wl.desc, "blocks": len(wl.blocks), "interior_points": wl.npts,
                        "precision": args.precision, "rhs_evals_per_step": 4, "rk4_step": "hipGraph replay",
-                       "tiling": tiling, "autotuned": not args.no_autotune,
+                       "tiling": res["tiling"], "autotuned": not args.no_autotune,
                        "dt": wl.dt, **wl.params},
-            "rhs_only_mpoints_s": rhs_only, "constraints": constraints, "halo_fill": halo,
-            "roofline": roof, "cpu_baseline": cpu, "mixed_precision": mixed}
+            "rhs_only_mpoints_s": res["rhs_only_mpoints_s"], "constraints": res["constraints"],
+            "halo_fill": res["halo_fill"], "roofline": res["roofline"], "cpu_baseline": cpu,
+            "mixed_precision": mixed, "uniform13": uniform, "large_octree": large}
     print(json.dumps(line), flush=True)
 
 

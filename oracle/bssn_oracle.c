@@ -4,11 +4,13 @@
  * expression rounds in the order written, which is the order of the reference
  * expressions cited beside it.
  */
+#define _POSIX_C_SOURCE 199309L /* clock_gettime */
 #include "bssn_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -588,6 +590,38 @@ void orc_ham_points(const double *in, int64_t npts, double *ham) {
 static orc_point_fn g_point_fn = orc_bssn_point;
 void orc_set_point_algebra(orc_point_fn fn) { g_point_fn = fn ? fn : orc_bssn_point; }
 
+/* The whole-block algebra hook: when set, orc_bssnrhs_block hands the block's
+ * field and derivative arrays to it instead of running the point loop (the
+ * CPU baseline runs the reference's generated equations over the block this
+ * way, as rhs.cpp:193-269 does). */
+static orc_block_algebra_fn g_block_fn = NULL;
+void orc_set_block_algebra(orc_block_algebra_fn fn) { g_block_fn = fn; }
+
+/* Wall-clock seconds per phase of orc_bssnrhs_block, summed over blocks and
+ * threads, at the reference's timer boundaries: t_deriv around the derivative
+ * calls (rhs.cpp:171-182), t_rhs around the equations (:193-270), t_bdyc
+ * around the boundary pass (:273-329), then the KO derivatives and sum
+ * (:332-492; the reference books those under t_deriv and t_rhs, kept apart
+ * here as "ko"). */
+static double g_phase[4];
+static double wall(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+static void phase_add(int k, double dt) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+    g_phase[k] += dt;
+}
+void orc_phase_times(double *out4, int reset) {
+    for (int k = 0; k < 4; k++) {
+        out4[k] = g_phase[k];
+        if (reset) g_phase[k] = 0.0;
+    }
+}
+
 void orc_bssn_points(const double *in, int64_t npts, const orc_params *p, double *rhs) {
     for (int64_t q = 0; q < npts; q++) orc_bssn_point(in + q * ORC_POINT_NIN, p, rhs + q * 24);
 }
@@ -713,14 +747,21 @@ int orc_bssnrhs_block(double *const *rhs_all, const double *const *u_all, const 
     }
 
     /* derivative workspace: 72 first + 66 second (rhs.cpp:171-180) */
+    double t0 = wall();
     double *ws = scratch(0, (size_t)n * (72 + 66 + 3));
     if (!ws) return -2;
     double *d1[24][3], *d2[11][6];
     block_derivs(u, sz, bflag, hx, hy, hz, ws, d1, d2);
     double *kox = ws + (size_t)n * 138, *koy = kox + n, *koz = koy + n;
+    double t1 = wall();
+    phase_add(0, t1 - t0);
 
     /* pointwise equations over the interior (rhs.cpp:194-269) */
     double in[ORC_POINT_NIN], out[24];
+    if (g_block_fn) {
+        const double hs[3] = {hx, hy, hz};
+        g_block_fn(u, (const double *const *)&d1[0][0], (const double *const *)&d2[0][0], sz, pmin, hs, p, rhs);
+    } else
     for (int k = PW; k < nz - PW; k++)
         for (int j = PW; j < ny - PW; j++)
             for (int i = PW; i < nx - PW; i++) {
@@ -743,6 +784,9 @@ int orc_bssnrhs_block(double *const *rhs_all, const double *const *u_all, const 
                 for (int v = 0; v < 24; v++) rhs[v][pp] = out[v];
             }
 
+    t0 = wall();
+    phase_add(1, t0 - t1);
+
     /* boundary fix-up (rhs.cpp:272-330) */
     if (bflag != 0) {
         for (int v = 0; v < 24; v++) {
@@ -751,6 +795,9 @@ int orc_bssnrhs_block(double *const *rhs_all, const double *const *u_all, const 
             bssn_bcs(rhs[v], u[v], d1[v][0], d1[v][1], d1[v][2], pmin, pmax, fo, fa, sz, bflag);
         }
     }
+
+    t1 = wall();
+    phase_add(2, t1 - t0);
 
     /* Kreiss-Oliger dissipation (rhs.cpp:333, 338-492) */
     for (int v = 0; v < 24; v++) {
@@ -769,6 +816,7 @@ int orc_bssnrhs_block(double *const *rhs_all, const double *const *u_all, const 
                     rhs[v][pp] += sigma * (kox[pp] + koy[pp] + koz[pp]);
                 }
     }
+    phase_add(3, wall() - t1);
     return 0;
 }
 
@@ -868,8 +916,26 @@ int orc_ham_batch(double *ham, const double *const *u, const orc_block *blks, in
  * mean of c^2 over the nodes (here: interior points of every block) whose
  * distance to BH 1 is >= r1 and to BH 2 is >= r2. bh = {x1,y1,z1,x2,y2,z2}.
  * Returns -1 when no point qualifies. */
-double orc_constraint_l2_norm(const double *c_all, const orc_block *blks, int64_t nblocks, const double *bh,
-                              double r1, double r2) {
+/* A node on a low face, edge or corner of block b (index PW along each such
+ * axis) is also a node of every neighbour lying in a direction d with d_a = -1
+ * only along those axes; it is counted by the block with no such neighbour,
+ * so a node shared by neighbouring blocks is counted once, as
+ * computeConstraintL2Norm counts each zipped node once (grUtils.tcc:60-110,
+ * its `accumulated` flags). nbr: 27 per block, -1 for none (NULL: blocks are
+ * taken as disjoint). */
+static int owned_elsewhere(const int32_t *nbr, int64_t b, int lox, int loy, int loz) {
+    if (!nbr || !(lox | loy | loz)) return 0;
+    for (int dz = loz ? -1 : 0; dz <= 0; dz++)
+        for (int dy = loy ? -1 : 0; dy <= 0; dy++)
+            for (int dx = lox ? -1 : 0; dx <= 0; dx++) {
+                if (!(dx | dy | dz)) continue;
+                if (nbr[27 * b + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] >= 0) return 1;
+            }
+    return 0;
+}
+
+double orc_constraint_l2_norm_nbr(const double *c_all, const orc_block *blks, int64_t nblocks, const int32_t *nbr,
+                                  const double *bh, double r1, double r2) {
     double l2 = 0.0;
     int64_t cnt = 0;
     for (int64_t b = 0; b < nblocks; b++) {
@@ -883,6 +949,7 @@ double orc_constraint_l2_norm(const double *c_all, const orc_block *blks, int64_
         for (int k = 3; k < nz - 3; k++)
             for (int j = 3; j < ny - 3; j++)
                 for (int i = 3; i < nx - 3; i++) {
+                    if (owned_elsewhere(nbr, b, i == 3, j == 3, k == 3)) continue;
                     const double x = blk->pmin[0] + i * hx, y = blk->pmin[1] + j * hy, z = blk->pmin[2] + k * hz;
                     const double d1 = sqrt((x - bh[0]) * (x - bh[0]) + (y - bh[1]) * (y - bh[1]) + (z - bh[2]) * (z - bh[2]));
                     const double d2 = sqrt((x - bh[3]) * (x - bh[3]) + (y - bh[4]) * (y - bh[4]) + (z - bh[5]) * (z - bh[5]));
@@ -893,6 +960,11 @@ double orc_constraint_l2_norm(const double *c_all, const orc_block *blks, int64_
                 }
     }
     return cnt ? sqrt(l2 / (double)cnt) : -1.0;
+}
+
+double orc_constraint_l2_norm(const double *c_all, const orc_block *blks, int64_t nblocks, const double *bh,
+                              double r1, double r2) {
+    return orc_constraint_l2_norm_nbr(c_all, blks, nblocks, NULL, bh, r1, r2);
 }
 
 /* ======================================================================== */

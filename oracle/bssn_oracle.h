@@ -149,6 +149,10 @@ int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, 
  * r1 of BH 1 and r2 of BH 2 (bh = x1,y1,z1,x2,y2,z2); -1 if none qualifies. */
 double orc_constraint_l2_norm(const double *c, const orc_block *blks, int64_t nblocks, const double *bh,
                               double r1, double r2);
+/* The same with the neighbour table of orc_halo_fill (27 per block, -1 for
+ * none): a node shared by neighbouring blocks is counted once. */
+double orc_constraint_l2_norm_nbr(const double *c, const orc_block *blks, int64_t nblocks, const int32_t *nbr,
+                                  const double *bh, double r1, double r2);
 /* Replace the algebra orc_bssnrhs_block evaluates per point (NULL restores
  * orc_bssn_point). Process-global: fixture generation only, single-threaded. */
 typedef void (*orc_point_fn)(const double *in, const orc_params *p, double *rhs24);
@@ -157,6 +161,18 @@ void orc_set_point_algebra(orc_point_fn fn);
 /* ---- per-block RHS (rhs.cpp:80-504) and batch (rhs.cpp:10-73) ----------- */
 int orc_bssnrhs_block(double *const *rhs, const double *const *u, const orc_block *blk,
                       const orc_params *p);
+/* Whole-block replacement of the point loop (NULL restores it): the fields,
+ * the 72 first-derivative arrays (24 vars x 3, d1[3v+d]) and the 66 second-
+ * derivative arrays (11 vars x 6, d2[6m+c]) of one block, its sizes, padded
+ * corner and spacing; writes the 24 right-hand sides at the interior points.
+ * Process-global: set before the threads start. */
+typedef void (*orc_block_algebra_fn)(const double *const *u, const double *const *d1, const double *const *d2,
+                                     const uint32_t *sz, const double *pmin, const double *h,
+                                     const orc_params *p, double *const *rhs);
+void orc_set_block_algebra(orc_block_algebra_fn fn);
+/* Seconds per phase (deriv, rhs, bc, ko) of orc_bssnrhs_block summed over
+ * blocks and threads since the last reset; reset != 0 clears them. */
+void orc_phase_times(double *out4, int reset);
 int orc_bssnrhs_batch(double *const *rhs, const double *const *u, const orc_block *blks,
                       int64_t nblocks, const orc_params *p, int nthreads);
 

@@ -89,6 +89,8 @@ def _configure(L):
     L.orc_constraints_batch.restype = C.c_int
     L.orc_constraint_l2_norm.argtypes = [dp, C.c_void_p, C.c_int64, dp, C.c_double, C.c_double]
     L.orc_constraint_l2_norm.restype = C.c_double
+    L.orc_constraint_l2_norm_nbr.argtypes = [dp, C.c_void_p, C.c_int64, C.c_void_p, dp, C.c_double, C.c_double]
+    L.orc_constraint_l2_norm_nbr.restype = C.c_double
     L.orc_enforce_constraints_point.argtypes = [C.c_void_p, C.c_uint64, C.c_double]
     L.orc_enforce_constraints_point.restype = C.c_int
     for nm in ("orc_deriv644_x", "orc_deriv644_y", "orc_deriv644_z", "orc_deriv644_xx",
@@ -100,6 +102,8 @@ def _configure(L):
                                                C.c_uint64, C.c_double]
     L.orc_init_brill_lindquist.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
                                            C.c_double, C.c_double]
+    L.orc_set_block_algebra.argtypes = [C.c_void_p]
+    L.orc_phase_times.argtypes = [dp, C.c_int]
     return L
 
 
@@ -142,6 +146,32 @@ def ref_lib():
         R.ref_constraints_xyz.argtypes = [dp, dp, C.c_int64, dp, dp, dp]
         _ref = R
     return _ref
+
+
+PHASES = ("deriv", "rhs", "bc", "ko")
+
+
+def phase_times(reset: bool = True) -> dict:
+    """Seconds per phase of the block right-hand side (summed over blocks and
+    threads) since the last reset, at the reference's timer boundaries."""
+    out = (C.c_double * 4)()
+    lib().orc_phase_times(out, int(reset))
+    return {k: out[i] for i, k in enumerate(PHASES)}
+
+
+@contextlib.contextmanager
+def reference_block_algebra():
+    """Within the block, the block right-hand side evaluates the reference's own
+    generated equations (bssneqs_eta_const_standard_gauge.cpp compiled in
+    place, oracle/_ref) over each block instead of the oracle's tensor loops."""
+    R = ref_lib()
+    if R is None:
+        raise RuntimeError("oracle/_ref not built (needs the reference checkout)")
+    lib().orc_set_block_algebra(C.cast(R.ref_bssn_block_algebra, C.c_void_p))
+    try:
+        yield
+    finally:
+        lib().orc_set_block_algebra(None)
 
 
 def _dp(a: np.ndarray):
@@ -371,12 +401,19 @@ def constraints_batch(state: np.ndarray, blocks: np.ndarray, nthreads: int = 0, 
 
 
 def constraint_l2_norm(c: np.ndarray, blocks: np.ndarray, bh=((3.0, 0, 0), (-3.0, 0, 0)),
-                       r1: float = 5.0, r2: float = 5.0) -> float:
-    """computeConstraintL2Norm over interior points outside r1/r2 of the two BHs."""
+                       r1: float = 5.0, r2: float = 5.0, nbr: np.ndarray | None = None) -> float:
+    """computeConstraintL2Norm over interior points outside r1/r2 of the two BHs;
+    with a neighbour table (nblocks x 27 int32) a node shared by neighbouring
+    blocks counts once."""
     c = np.ascontiguousarray(c, dtype=np.float64)
     blocks = np.ascontiguousarray(blocks)
     b = np.ascontiguousarray(np.asarray(bh, dtype=np.float64).ravel())
-    return float(lib().orc_constraint_l2_norm(_dp(c), blocks.ctypes.data, len(blocks), _dp(b), r1, r2))
+    if nbr is None:
+        return float(lib().orc_constraint_l2_norm(_dp(c), blocks.ctypes.data, len(blocks), _dp(b), r1, r2))
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    assert nbr.shape == (len(blocks), 27)
+    return float(lib().orc_constraint_l2_norm_nbr(_dp(c), blocks.ctypes.data, len(blocks), nbr.ctypes.data,
+                                                  _dp(b), r1, r2))
 
 
 def enforce_point(state: np.ndarray, node: int, chi_floor: float) -> int:

@@ -130,3 +130,64 @@ extern "C" int ref_constraint_points(const double *in, int64_t npts, double *ham
 extern "C" int ref_ham_points(const double *in, int64_t npts, double *ham_out) {
     return ref_constraint_points(in, npts, ham_out, nullptr);
 }
+
+// The reference's generated equations over one block, with the loop and the
+// per-point coordinates and RIT eta of BSSN_GR/src/rhs.cpp:193-269, on the
+// field and derivative arrays the oracle computed for the block (its
+// restatement of derivs.cpp). Installed with orc_set_block_algebra, it makes
+// orc_bssnrhs_block the reference's bssnrhs with only the stencils, the
+// boundary pass and the KO sum restated: the CPU baseline times it.
+#include "bssn_oracle.h"
+
+extern "C" void ref_bssn_block_algebra(const double *const *u, const double *const *d1, const double *const *d2,
+                                       const uint32_t *sz, const double *pmin, const double *hs,
+                                       const orc_params *p, double *const *rhs) {
+    const unsigned int nx = sz[0], ny = sz[1], nz = sz[2];
+    const unsigned int PW = 3;
+    const double hx = hs[0], hy = hs[1], hz = hs[2];
+    const unsigned int lambda[4] = {(unsigned)p->lambda[0], (unsigned)p->lambda[1], (unsigned)p->lambda[2],
+                                    (unsigned)p->lambda[3]};
+    const double lambda_f[2] = {p->lambda_f[0], p->lambda_f[1]};
+#define BF(name, v) const double *const name = u[v];
+#define BG(name, v) \
+    const double *const grad_0_##name = d1[3 * (v)]; \
+    const double *const grad_1_##name = d1[3 * (v) + 1]; \
+    const double *const grad_2_##name = d1[3 * (v) + 2];
+#define BG2(name, m) \
+    const double *const grad2_0_0_##name = d2[6 * (m)]; const double *const grad2_0_1_##name = d2[6 * (m) + 1]; \
+    const double *const grad2_0_2_##name = d2[6 * (m) + 2]; const double *const grad2_1_1_##name = d2[6 * (m) + 3]; \
+    const double *const grad2_1_2_##name = d2[6 * (m) + 4]; const double *const grad2_2_2_##name = d2[6 * (m) + 5];
+#define BO(name, v) double *const name = rhs[v];
+    BF(alpha, 0) BF(chi, 1) BF(K, 2) BF(Gt0, 3) BF(Gt1, 4) BF(Gt2, 5)
+    BF(beta0, 6) BF(beta1, 7) BF(beta2, 8) BF(B0, 9) BF(B1, 10) BF(B2, 11)
+    BF(gt0, 12) BF(gt1, 13) BF(gt2, 14) BF(gt3, 15) BF(gt4, 16) BF(gt5, 17)
+    BF(At0, 18) BF(At1, 19) BF(At2, 20) BF(At3, 21) BF(At4, 22) BF(At5, 23)
+    BG(alpha, 0) BG(chi, 1) BG(K, 2) BG(Gt0, 3) BG(Gt1, 4) BG(Gt2, 5)
+    BG(beta0, 6) BG(beta1, 7) BG(beta2, 8) BG(B0, 9) BG(B1, 10) BG(B2, 11)
+    BG(gt0, 12) BG(gt1, 13) BG(gt2, 14) BG(gt3, 15) BG(gt4, 16) BG(gt5, 17)
+    BG(At0, 18) BG(At1, 19) BG(At2, 20) BG(At3, 21) BG(At4, 22) BG(At5, 23)
+    BG2(alpha, 0) BG2(chi, 1) BG2(beta0, 2) BG2(beta1, 3) BG2(beta2, 4)
+    BG2(gt0, 5) BG2(gt1, 6) BG2(gt2, 7) BG2(gt3, 8) BG2(gt4, 9) BG2(gt5, 10)
+    BO(a_rhs, 0) BO(chi_rhs, 1) BO(K_rhs, 2) BO(Gt_rhs0, 3) BO(Gt_rhs1, 4) BO(Gt_rhs2, 5)
+    BO(b_rhs0, 6) BO(b_rhs1, 7) BO(b_rhs2, 8) BO(B_rhs0, 9) BO(B_rhs1, 10) BO(B_rhs2, 11)
+    BO(gt_rhs00, 12) BO(gt_rhs01, 13) BO(gt_rhs02, 14) BO(gt_rhs11, 15) BO(gt_rhs12, 16)
+    BO(gt_rhs22, 17) BO(At_rhs00, 18) BO(At_rhs01, 19) BO(At_rhs02, 20) BO(At_rhs11, 21)
+    BO(At_rhs12, 22) BO(At_rhs22, 23)
+#undef BF
+#undef BG
+#undef BG2
+#undef BO
+    for (unsigned int k = PW; k < nz - PW; k++)
+        for (unsigned int j = PW; j < ny - PW; j++)
+            for (unsigned int i = PW; i < nx - PW; i++) {
+                const double x = pmin[0] + i * hx;
+                const double y = pmin[1] + j * hy;
+                const double z = pmin[2] + k * hz;
+                const unsigned int pp = i + nx * (j + ny * k);
+                const double r_coord = sqrt(x * x + y * y + z * z);
+                const double w = r_coord / p->eta_width;
+                const double arg = -w * w * w * w;
+                const double eta = (p->eta_central - p->eta_outer) * exp(arg) + p->eta_outer;
+#include "bssneqs_eta_const_standard_gauge.cpp"
+            }
+}

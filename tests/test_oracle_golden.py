@@ -192,3 +192,24 @@ def test_enforce_constraints_rejects_negative_determinant():
     st = np.zeros((24, 1))
     st[12, 0], st[15, 0], st[17, 0] = 1.0, 1.0, -1.0
     assert O.enforce_point(st, 0, 0.1) == -1
+
+
+def test_reference_block_algebra_matches_oracle_block():
+    """The reference's generated equations run over whole blocks (the CPU
+    baseline's configuration) give the oracle's block right-hand side, also
+    with faces flagged and on uneven blocks, and the phase timers add up."""
+    if O.ref_lib() is None:
+        pytest.skip("reference checkout not present")
+    for t in (W.single_block_19(63), W.uniform_13(4)):
+        st = np.zeros((24, W.total_padded(t)))
+        O.init_perturbed_minkowski(st, t, seed=3, amp=1e-3)
+        p = O.default_params(ko_sigma=0.4, eta_central=2.0, eta_outer=0.25, eta_width=1.0)
+        a = O.bssnrhs_batch(st, t, p, nthreads=1)
+        O.phase_times(reset=True)
+        with O.reference_block_algebra():
+            b = O.bssnrhs_batch(st, t, p, nthreads=1)
+        ph = O.phase_times(reset=True)
+        m = O.interior_mask(t, st.shape[1])
+        err = normwise(b[:, m], a[:, m], axis=1)
+        assert err.max() < 1e-12, err
+        assert all(v >= 0 for v in ph.values()) and ph["deriv"] > 0 and ph["rhs"] > 0 and ph["ko"] > 0
