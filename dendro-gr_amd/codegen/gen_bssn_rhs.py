@@ -15,9 +15,11 @@ The emitted function reads inputs through an accessor object
 and hands each right-hand side to `out.template put<V>(value)`, so a kernel
 decides where inputs come from and what happens to each output.
 
-Usage: python gen_bssn_rhs.py [--cse=local|global] [--phased | --ham] [output.h]
+Usage: python gen_bssn_rhs.py [--cse=local|global] [--phased | --ham | --cons-phased [--search]] [output.h]
 (--phased: the right-hand side in the phase order of the fused kernel;
- --ham: the Hamiltonian constraint)
+ --ham: the constraints (Hamiltonian, momentum, psi4);
+ --cons-phased: the constraints in the phase order of the fused constraint
+ kernel, --search to look for that order again)
 (local: common subexpressions are shared within one equation or intermediate
 only, which bounds the live set; global: shared across all of them.)
 """
@@ -724,6 +726,116 @@ DGR_HD DGR_INLINE void bssn_ham_point(const In &in, Out &out) {{
           f"{time.time() - t0:.1f}s")
 
 
+# Phase order of the fused constraint kernel (the variables whose derivatives
+# the constraints read): chosen by local search for the smallest live set of
+# the schedule (see search_cons_order).
+CONS_PHASE_ORDER = ["gt3", "gt1", "gt5", "gt2", "gt4", "Gt2", "Gt0", "Gt1", "gt0", "chi", "At5", "K",
+                    "At4", "At0", "At2", "At3", "At1"]
+
+
+def _max_live(prog, defs, outs_r):
+    last_use, live, max_live = {}, set(), 0
+    for k, (kind, x) in enumerate(prog):
+        e = defs[x] if kind == "tmp" else outs_r[x] if kind == "out" else None
+        if e is not None:
+            for y in e.free_symbols:
+                last_use[y] = k
+    for k, (kind, x) in enumerate(prog):
+        if kind in ("in", "tmp") and not x.name.startswith("f_"):
+            live.add(x)
+        max_live = max(max_live, len(live))
+        live = {y for y in live if last_use.get(y, -1) > k}
+    return max_live
+
+
+def search_cons_order(defs, outs_r, start, rounds=60, seed=1):
+    """Local search over phase orders (swaps of two positions), keeping a swap
+    when the live set of the phased schedule does not grow."""
+    import random
+    rnd = random.Random(seed)
+    best = list(start)
+    best_live = _max_live(schedule_phased(defs, outs_r, best), defs, outs_r)
+    for _ in range(rounds):
+        cand = list(best)
+        i, j = rnd.sample(range(len(cand)), 2)
+        cand[i], cand[j] = cand[j], cand[i]
+        lv = _max_live(schedule_phased(defs, outs_r, cand), defs, outs_r)
+        if lv <= best_live:
+            best, best_live = cand, lv
+    return best, best_live
+
+
+def generate_cons_phased(path, search=False):
+    t0 = time.time()
+    defs, outs_r, ops_after = eliminate([HAM] + MOM + psi4_exprs())
+    used = {_var_of(x) for e in list(defs.values()) + outs_r for x in e.free_symbols if _var_of(x)}
+    order = [v for v in CONS_PHASE_ORDER if v in used]
+    assert set(order) == used, (sorted(used), order)
+    if search:
+        order, lv = search_cons_order(defs, outs_r, order)
+        print(f"phase order {order}: max live {lv}")
+    prog = schedule_phased(defs, outs_r, order)
+    vidx = [VIDX[v] for v in order]
+
+    def phase_line(p):
+        if p == len(order):
+            return "    ctx.finish();"
+        nxt = vidx[p + 1] if p + 1 < len(vidx) else -1
+        return f"    ctx.template phase<{vidx[p]}, {nxt}, {p}>();  // {order[p]}"
+
+    def phased_input(x):
+        n = x.name
+        if n.startswith("f_"):
+            return f"ctx.template fld<{VIDX[n[2:]]}>()"
+        return input_expr(x).replace("in.template", "ctx.template")
+
+    lines, ntmp, _ = emit(prog, defs, outs_r, phased_input, phase_line, inline_fields=True, pin=True,
+                          out_names=["ham", "mom0", "mom1", "mom2", "psi4_real", "psi4_img"])
+    max_live = _max_live(prog, defs, outs_r)
+    hdr = """// Generated by dendro-gr_amd/codegen/gen_bssn_rhs.py --cons-phased -- do not edit.
+//
+// The constraints of bssn_ham_generated.h (CodeGen/constraints.py:42-282:
+// Hamiltonian, momentum, psi4) in the phased form of bssn_rhs_phased.h: the
+// variables whose derivatives they read stream through the kernel one at a
+// time, in the order """ + ", ".join(order) + f""".
+// ctx.phase<V, NEXT, K>() makes variable V's derivatives available at this
+// thread's point; ctx.d1 / ctx.d2 only in V's phase; ctx.fld<V>() and
+// ctx.crd<D>() (the point's coordinates, read by psi4) anywhere. Every thread
+// of a workgroup must execute the whole function. Outputs: put<0>(ham),
+// put<1..3>(mom0..2), put<4..5>(psi4 real, imaginary).
+// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values
+// (fields excluded) live at once in the emitted order.
+#pragma once
+
+#ifndef DGR_HD
+#define DGR_HD
+#endif
+#ifndef DGR_INLINE
+#define DGR_INLINE inline
+#endif
+#ifndef DGR_PIN
+#define DGR_PIN(x) ((void)0)
+#endif
+
+namespace dgr {{
+
+// phases of the constraint kernel and the variable of each
+constexpr int kConsPhases = {len(vidx)};
+constexpr int kConsPhaseVar[{len(vidx)}] = {{{', '.join(str(v) for v in vidx)}}};
+
+template <typename T, typename Ctx, typename Out>
+DGR_HD DGR_INLINE void bssn_cons_phased(Ctx &ctx, Out &out) {{
+{chr(10).join(lines)}
+}}
+
+}}  // namespace dgr
+"""
+    with open(path, "w") as fh:
+        fh.write(hdr)
+    print(f"generated {path}: {ntmp} temporaries, ops {ops_after}, max live {max_live} (phased), "
+          f"{time.time() - t0:.1f}s")
+
+
 CSE_MODE = "local"
 
 if __name__ == "__main__":
@@ -733,6 +845,8 @@ if __name__ == "__main__":
             CSE_MODE = a.split("=", 1)[1]
     if "--phased" in sys.argv:
         generate_phased(args[0] if args else "bssn_rhs_phased.h")
+    elif "--cons-phased" in sys.argv:
+        generate_cons_phased(args[0] if args else "bssn_cons_phased.h", search="--search" in sys.argv)
     elif "--ham" in sys.argv:
         generate_ham(args[0] if args else "bssn_ham_generated.h")
     else:

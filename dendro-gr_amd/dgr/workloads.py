@@ -96,6 +96,51 @@ def neighbors(t: np.ndarray, tol: float = 1e-6) -> np.ndarray:
     return out
 
 
+def two_level_lattice(h: float = 0.05, origin=(-0.41, -0.29, -0.17), fine_cell=(1, 1, 1)) -> np.ndarray:
+    """A 2x2x2 lattice of 13^3 blocks of spacing 2h (interiors 12h wide,
+    neighbours sharing boundary nodes) with one cell refined: in place of the
+    coarse block at `fine_cell`, 2x2x2 blocks of 13^3 of spacing h (interiors
+    6h wide). Faces between the refined cell and its coarse neighbours are
+    level jumps; blocks sharing a level and a size link as neighbours."""
+    o = np.asarray(origin, dtype=np.float64)
+    sizes, pmins, hs = [], [], []
+    for c in range(2):
+        for b in range(2):
+            for a in range(2):
+                cell = np.array([a, b, c], dtype=np.float64)
+                if (a, b, c) == tuple(fine_cell):
+                    for r in range(2):
+                        for q in range(2):
+                            for p in range(2):
+                                sizes.append(13)
+                                hs.append(h)
+                                pmins.append(o + 12 * h * cell + 6 * h * np.array([p, q, r]) - PW * h)
+                else:
+                    sizes.append(13)
+                    hs.append(2 * h)
+                    pmins.append(o + 12 * h * cell - PW * 2 * h)
+    return _table(sizes, pmins, hs)
+
+
+def mixed_size_lattice(h: float = 0.05, origin=(-0.43, -0.31, -0.19)) -> np.ndarray:
+    """Blocks of one spacing h but two sizes: a 19^3 block (interior 12h wide)
+    at the origin, eight 13^3 blocks (interiors 6h wide) filling the 12h cube
+    next to it along +x, and one more 19^3 block beyond them; every 19^3 face
+    towards the 13^3 blocks meets four of them."""
+    o = np.asarray(origin, dtype=np.float64)
+    sizes, pmins, hs = [19], [o - PW * h], [h]
+    for r in range(2):
+        for q in range(2):
+            for p in range(2):
+                sizes.append(13)
+                hs.append(h)
+                pmins.append(o + np.array([12 * h, 0.0, 0.0]) + 6 * h * np.array([p, q, r]) - PW * h)
+    sizes.append(19)
+    hs.append(h)
+    pmins.append(o + np.array([24 * h, 0.0, 0.0]) - PW * h)
+    return _table(sizes, pmins, hs)
+
+
 def total_padded(t: np.ndarray) -> int:
     """Length each variable array needs (max offset + block size)."""
     if len(t) == 0:

@@ -1130,6 +1130,133 @@ void orc_halo_fill(double *const *u, const orc_block *blks, int64_t nblocks, con
     }
 }
 
+/* Lagrange weights of the 7 nodes of an order-6 element at t = code/2 (code in
+ * 0..12), the element-local interpolation an unzip uses where a block's
+ * padding lies inside a coarser block (the octree code is in dendrolib,
+ * absent; this is the interpolation of the element's nodal polynomial). */
+void orc_lagrange6_half(int code, double w[7]) {
+    const double t = 0.5 * (double)code;
+    for (int m = 0; m < 7; m++) {
+        double num = 1.0, den = 1.0;
+        for (int l = 0; l < 7; l++) {
+            if (l == m) continue;
+            num *= (t - (double)l);
+            den *= (double)(m - l);
+        }
+        w[m] = num / den;
+    }
+}
+
+/* Where padding point X of block b takes its value when no same-size
+ * neighbour in its direction holds it: the block whose interior contains X,
+ * preferring the same spacing (a neighbour of another size: copy), then half
+ * the spacing (a finer neighbour: X is one of its nodes, injection), then
+ * twice the spacing (a coarser neighbour: order-6 interpolation inside the
+ * element of that block containing X); ties go to the lower block index.
+ * Returns the kind (0 none, 1 copy, 2 interpolation) and the source. */
+static int padding_source(const orc_block *blks, int64_t nblocks, int64_t b, const double X[3], int64_t *src_blk,
+                          int q_out[3], int code[3]) {
+    double hb[3];
+    for (int a = 0; a < 3; a++) hb[a] = (blks[b].pmax[a] - blks[b].pmin[a]) / (blks[b].sz[a] - 1);
+    int best = 3;
+    for (int64_t c = 0; c < nblocks; c++) {
+        if (c == b) continue;
+        const orc_block *nb = &blks[c];
+        double hc[3];
+        for (int a = 0; a < 3; a++) hc[a] = (nb->pmax[a] - nb->pmin[a]) / (nb->sz[a] - 1);
+        int rel = -1; /* 0 same, 1 finer, 2 coarser */
+        if (fabs(hc[0] - hb[0]) <= 1e-9 * hb[0]) rel = 0;
+        else if (fabs(2.0 * hc[0] - hb[0]) <= 1e-9 * hb[0]) rel = 1;
+        else if (fabs(hc[0] - 2.0 * hb[0]) <= 1e-9 * hb[0]) rel = 2;
+        if (rel < 0 || rel >= best) continue;
+        int ok = 1, q[3], cd[3];
+        for (int a = 0; a < 3 && ok; a++) {
+            const double qa = (X[a] - nb->pmin[a]) / hc[a];
+            const double q2 = 2.0 * qa;
+            const double r2 = floor(q2 + 0.5);
+            if (fabs(q2 - r2) > 2e-6) { ok = 0; break; }
+            const long i2 = (long)r2;
+            if (i2 < 2 * 3 || i2 > 2 * ((long)nb->sz[a] - 4)) { ok = 0; break; }
+            if (rel < 2 && (i2 & 1)) { ok = 0; break; }
+            if (rel < 2) {
+                q[a] = (int)(i2 / 2);
+            } else {
+                /* element e of the block's interior (6 intervals each) holding X */
+                const long t2 = i2 - 2 * 3, ne = ((long)nb->sz[a] - 7) / 6;
+                long e = t2 / 12;
+                if (e > ne - 1) e = ne - 1;
+                q[a] = (int)(3 + 6 * e);     /* the element's first node */
+                cd[a] = (int)(t2 - 12 * e);  /* 2 t, t the element-local coordinate */
+            }
+        }
+        if (!ok) continue;
+        best = rel;
+        *src_blk = c;
+        for (int a = 0; a < 3; a++) {
+            q_out[a] = q[a];
+            code[a] = rel == 2 ? cd[a] : 0;
+        }
+    }
+    return best == 3 ? 0 : (best < 2 ? 1 : 2);
+}
+
+/* orc_halo_fill, then (general != 0) every padding point without a same-size
+ * neighbour in its direction from padding_source: the copy part of the
+ * reference's unzip for neighbours of any size and across one level jump
+ * (BSSN_GR/src/bssnCtx.cpp:75; the octree code is in dendrolib, absent).
+ * Reads interior points only and writes padding points only. */
+void orc_padding_fill(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general) {
+    const int PW = 3;
+    orc_halo_fill(u, blks, nblocks, nbr);
+    if (!general) return;
+    for (int64_t b = 0; b < nblocks; b++) {
+        const orc_block *blk = &blks[b];
+        const int nx = (int)blk->sz[0], ny = (int)blk->sz[1], nz = (int)blk->sz[2];
+        double h[3];
+        for (int a = 0; a < 3; a++) h[a] = (blk->pmax[a] - blk->pmin[a]) / (blk->sz[a] - 1);
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    const int di = i < PW ? -1 : (i >= nx - PW ? 1 : 0);
+                    const int dj = j < PW ? -1 : (j >= ny - PW ? 1 : 0);
+                    const int dk = k < PW ? -1 : (k >= nz - PW ? 1 : 0);
+                    if (!di && !dj && !dk) continue;
+                    if (nbr[27 * b + (di + 1) + 3 * (dj + 1) + 9 * (dk + 1)] >= 0) continue;
+                    const double X[3] = {blk->pmin[0] + i * h[0], blk->pmin[1] + j * h[1], blk->pmin[2] + k * h[2]};
+                    int64_t c = -1;
+                    int q[3], cd[3];
+                    const int kind = padding_source(blks, nblocks, b, X, &c, q, cd);
+                    if (!kind) continue;
+                    const orc_block *nb = &blks[c];
+                    const int64_t mx = nb->sz[0], my = nb->sz[1];
+                    const int64_t dst = (int64_t)blk->offset + i + (int64_t)nx * (j + (int64_t)ny * k);
+                    const int64_t s0 = (int64_t)nb->offset + q[0] + mx * (q[1] + my * q[2]);
+                    if (kind == 1) {
+                        for (int v = 0; v < 24; v++) u[v][dst] = u[v][s0];
+                        continue;
+                    }
+                    double wx[7], wy[7], wz[7];
+                    orc_lagrange6_half(cd[0], wx);
+                    orc_lagrange6_half(cd[1], wy);
+                    orc_lagrange6_half(cd[2], wz);
+                    for (int v = 0; v < 24; v++) {
+                        double sz_ = 0.0;
+                        for (int c3 = 0; c3 < 7; c3++) {
+                            double sy = 0.0;
+                            for (int b2 = 0; b2 < 7; b2++) {
+                                double sx = 0.0;
+                                const double *row = u[v] + s0 + mx * (b2 + my * c3);
+                                for (int a1 = 0; a1 < 7; a1++) sx += wx[a1] * row[a1];
+                                sy += wy[b2] * sx;
+                            }
+                            sz_ += wz[c3] * sy;
+                        }
+                        u[v][dst] = sz_;
+                    }
+                }
+    }
+}
+
 /* RK4 (rkBSSN.cpp:764-860) with the padding of every stage state refreshed
  * from the neighbours before its right-hand side, as the reference's unzip
  * does between stages: u's padding before stage 0, each later stage's state
@@ -1137,6 +1264,11 @@ void orc_halo_fill(double *const *u, const orc_block *blks, int64_t nblocks, con
  * neighbour is frozen at its value at the start of the step. */
 int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, double dt,
                       const orc_params *p, int enforce_per_stage, int nthreads) {
+    return orc_rk4_step_fill(u, blks, nblocks, nbr, 0, dt, p, enforce_per_stage, nthreads);
+}
+
+int orc_rk4_step_fill(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general,
+                      double dt, const orc_params *p, int enforce_per_stage, int nthreads) {
     const int PW = 3;
     int64_t n = 0;
     for (int64_t b = 0; b < nblocks; b++) {
@@ -1146,7 +1278,7 @@ int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, 
     double *buf = (double *)malloc(sizeof(double) * (size_t)n * 24 * 3);
     if (!buf) return -2;
     double *u0[24], *us[24], *k[24];
-    orc_halo_fill(u, blks, nblocks, nbr);
+    orc_padding_fill(u, blks, nblocks, nbr, general);
     for (int v = 0; v < 24; v++) {
         u0[v] = buf + (size_t)n * v;
         us[v] = buf + (size_t)n * (24 + v);
@@ -1157,7 +1289,7 @@ int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, 
     }
     int err = 0;
     for (int s = 0; s < 4 && !err; s++) {
-        if (s > 0) orc_halo_fill(us, blks, nblocks, nbr);
+        if (s > 0) orc_padding_fill(us, blks, nblocks, nbr, general);
 #ifdef _OPENMP
         if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
@@ -1193,7 +1325,7 @@ int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, 
                     if (orc_enforce_constraints_point(u, (uint64_t)pp, p->chi_floor)) err = 1;
                 }
     }
-    if (!err) orc_halo_fill(u, blks, nblocks, nbr);
+    if (!err) orc_padding_fill(u, blks, nblocks, nbr, general);
     free(buf);
     return err ? -1 : 0;
 }

@@ -141,6 +141,16 @@ int orc_constraints_batch(double *ham, double *const *mom, double *const *psi4, 
 /* Padding refresh from same-level neighbours (the copy part of unzip): nbr
  * has 27 entries per block, index (di+1)+3(dj+1)+9(dk+1), -1 for none. */
 void orc_halo_fill(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr);
+/* orc_halo_fill, then (general != 0) the padding points without a same-size
+ * neighbour in their direction from the block whose interior holds them:
+ * same spacing (copy), half the spacing (injection) or twice the spacing
+ * (order-6 Lagrange interpolation in that block's element). */
+void orc_padding_fill(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general);
+/* The 7 Lagrange weights of an order-6 element at t = code/2, code 0..12. */
+void orc_lagrange6_half(int code, double w[7]);
+/* orc_rk4_step_halo with orc_padding_fill(general) as the refresh. */
+int orc_rk4_step_fill(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general,
+                      double dt, const orc_params *p, int enforce_per_stage, int nthreads);
 /* One RK4 step with every stage state's padding refreshed by orc_halo_fill
  * before its right-hand side, and u's padding after the step. */
 int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, double dt,

@@ -79,6 +79,11 @@ def _configure(L):
     L.orc_rk4_step_halo.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_double,
                                     C.POINTER(Params), C.c_int, C.c_int]
     L.orc_rk4_step_halo.restype = C.c_int
+    L.orc_padding_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int]
+    L.orc_rk4_step_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_double,
+                                    C.POINTER(Params), C.c_int, C.c_int]
+    L.orc_rk4_step_fill.restype = C.c_int
+    L.orc_lagrange6_half.argtypes = [C.c_int, dp]
     L.orc_ham_points.argtypes = [dp, C.c_int64, dp]
     L.orc_ham_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
     L.orc_ham_batch.restype = C.c_int
@@ -357,6 +362,35 @@ def halo_fill(state: np.ndarray, blocks: np.ndarray, nbr: np.ndarray) -> None:
     nbr = np.ascontiguousarray(nbr, dtype=np.int32)
     assert nbr.shape == (len(blocks), 27)
     lib().orc_halo_fill(_ptrs(state_views(state)), blocks.ctypes.data, len(blocks), nbr.ctypes.data)
+
+
+def padding_fill(state: np.ndarray, blocks: np.ndarray, nbr: np.ndarray, general: bool = True) -> None:
+    """In place: orc_halo_fill, then (general) every other padding point from
+    the block whose interior holds it: same spacing or half of it (copy) or
+    twice it (order-6 interpolation in that block's element)."""
+    blocks = np.ascontiguousarray(blocks)
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    assert nbr.shape == (len(blocks), 27)
+    lib().orc_padding_fill(_ptrs(state_views(state)), blocks.ctypes.data, len(blocks), nbr.ctypes.data,
+                           int(general))
+
+
+def lagrange6_half(code: int) -> np.ndarray:
+    w = np.zeros(7)
+    lib().orc_lagrange6_half(int(code), _dp(w))
+    return w
+
+
+def rk4_step_fill(state: np.ndarray, blocks: np.ndarray, nbr: np.ndarray, dt: float, p: Params,
+                  general: bool = True, enforce_per_stage: bool = True, nthreads: int = 0) -> None:
+    """rk4_step_halo with padding_fill(general) as the refresh."""
+    blocks = np.ascontiguousarray(blocks)
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    assert nbr.shape == (len(blocks), 27)
+    err = lib().orc_rk4_step_fill(_ptrs(state_views(state)), blocks.ctypes.data, len(blocks), nbr.ctypes.data,
+                                  int(general), dt, C.byref(p), int(enforce_per_stage), nthreads)
+    if err != 0:
+        raise RuntimeError("orc_rk4_step_fill failed (constraint violation)")
 
 
 def rk4_step_halo(state: np.ndarray, blocks: np.ndarray, nbr: np.ndarray, dt: float, p: Params,
