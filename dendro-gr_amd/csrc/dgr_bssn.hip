@@ -44,6 +44,7 @@ __host__ __device__ inline T dgr_sqrt(T x) {
 #include "bssn_rhs_generated.h"
 #include "stencils.h"
 #include "bssn_rhs_phased.h"
+#include "bssn_cons_phased.h"
 #include "bssn_ham_generated.h"
 
 namespace {
@@ -597,12 +598,28 @@ struct StageIO {
     const int64_t *nbo;
 };
 
+// Phase tables of the fused kernels: the variables in the order they stream
+// through LDS (one phase each), and whether a phase also evaluates the KO term
+// and, on flagged faces, the boundary value (the right-hand side does; the
+// constraint diagnostic reads derivatives only).
+struct RhsPhases {
+    static constexpr int N = NV;
+    static constexpr bool KO = true;
+    static constexpr int var(int k) { return dgr::kBssnPhaseVar[k]; }
+};
+struct ConsPhases {
+    static constexpr int N = dgr::kConsPhases;
+    static constexpr bool KO = false;
+    static constexpr int var(int k) { return dgr::kConsPhaseVar[k]; }
+};
+
 // FACE: the tile's block has flagged faces (closures and boundary values are
 // needed); interior tiles run a body without any of that code.
 // T: the arithmetic type of derivatives, equations and KO (double, or float
 // for the mixed-precision path, which stages each variable in LDS with a
 // per-tile offset subtracted: the value at the region's first point).
-template <bool FACE, class C, typename T>
+// P: the phase table.
+template <bool FACE, class C, typename T, class P = RhsPhases>
 struct FusedCtx {
     static constexpr int FT = C::FT, NREG = C::NREG, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT, PD = C::PD;
     const StageIO &io;
@@ -668,7 +685,7 @@ struct FusedCtx {
     // region of the variable of phase K into prefetch slot K % PD
     template <int K>
     __device__ __forceinline__ void fetch() {
-        constexpr int v = dgr::kBssnPhaseVar[K];
+        constexpr int v = P::var(K);
         double *dst = pre[K % PD];
         const double *__restrict__ sb = io.src.p[v] + boff;
         if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = sb[corner];
@@ -701,7 +718,7 @@ struct FusedCtx {
     // buffer; buffers sit at multiples of NREG doubles whatever the staging type
     template <int K>
     __device__ __forceinline__ void stash() {
-        constexpr int V = dgr::kBssnPhaseVar[K];
+        constexpr int V = P::var(K);
         using S = Stage<V>;
         static_assert(sizeof(S) == sizeof(T) || DEEP || K % 2 == 0,
                       "FP64-staged variables need the FP64-sized (even) buffer");
@@ -728,7 +745,7 @@ struct FusedCtx {
     template <int K>
     __device__ __forceinline__ void stage_next() {
         stash<K>();
-        if constexpr (K + PD < NV) fetch<K + PD>();
+        if constexpr (K + PD < P::N) fetch<K + PD>();
         __syncthreads();
     }
 
@@ -740,7 +757,7 @@ struct FusedCtx {
     // and boundary value.
     template <int V, int NEXT, int KPH>
     __device__ __forceinline__ void phase() {
-        static_assert(dgr::kBssnPhaseVar[KPH] == V, "phase order");
+        static_assert(P::var(KPH) == V, "phase order");
         using S = Stage<V>;
         S *t = reinterpret_cast<S *>(s_tile + rbuf<KPH>() * NREG);
         const dgr::AxisConsts &kx = K[0], &ky = K[1], &kz = K[2];
@@ -788,10 +805,10 @@ struct FusedCtx {
             r = dgr::d1_in<FRXY>(c, c_d1<2, S>());
             if (FACE && (loz || hiz)) r = dgr::d1_edge<FRXY>(c, r, az, iez, loz, hiz, S(kz.d1_12));
             d1v[2] = T(r);
-            if constexpr (!FACE) ko_term<V, S>(c, kx, ky, kz);
+            if constexpr (!FACE && P::KO) ko_term<V, S>(c, kx, ky, kz);
             if constexpr (DEEP && NEXT >= 0) {
                 stash<KPH + 1>();
-                if constexpr (KPH + 1 + PD < NV) fetch<KPH + 1 + PD>();
+                if constexpr (KPH + 1 + PD < P::N) fetch<KPH + 1 + PD>();
             }
             __syncthreads();
             const S *cx = sdx + dxo;
@@ -827,7 +844,7 @@ struct FusedCtx {
             for (int k = 0; k < 6; k++) DGR_PIN(d2v[k]);
         }
         // on a flagged face the boundary value needs the first derivatives
-        if constexpr (m2 < 0 || FACE) ko_term<V, S>(c, kx, ky, kz);
+        if constexpr (P::KO && (m2 < 0 || FACE)) ko_term<V, S>(c, kx, ky, kz);
         if constexpr (!DEEP || m2 < 0) {
             if constexpr (NEXT >= 0) stage_next<KPH + 1>();
         }
@@ -867,6 +884,10 @@ struct FusedCtx {
     __device__ __forceinline__ T d2() const { return d2v[K2]; }
     template <int V>
     __device__ __forceinline__ T fld() const { return f[V]; }
+    // the point's coordinates (the constraint kernel sets them)
+    double xc[3];
+    template <int D>
+    __device__ __forceinline__ T crd() const { return T(xc[D]); }
     __device__ __forceinline__ void finish() {}
 };
 
@@ -944,18 +965,14 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const T *s_ko,
     }
 }
 
-template <bool FACE, class C, typename T, bool HALO = false>
-__device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &tl, const DBlock &B,
-                                           const dgr::AxisConsts *K, const uint16_t *__restrict__ tabs,
-                                           int dtab, double *s_tile, double *s_dx, double *s_dy, T *s_ko) {
+// Tile geometry, this thread's point, the region elements it fetches and its
+// D_x / D_y slots (common to the fused kernels): idx is the point's padded
+// index, q its interior index, active false for shadow lanes.
+template <bool FACE, class C, typename T, bool HALO, class P>
+__device__ __forceinline__ void tile_setup(FusedCtx<FACE, C, T, P> &ctx, const StageIO &io, const Tile &tl,
+                                           const DBlock &B, const uint16_t *__restrict__ tabs, int dtab,
+                                           uint64_t &idx_out, int64_t &q_out, bool &active_out) {
     constexpr int FT = C::FT, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT;
-    FusedCtx<FACE, C, T> ctx(io);
-    ctx.K = K;
-    ctx.load_consts();
-    ctx.s_tile = s_tile;
-    ctx.s_dx = s_dx;
-    ctx.s_dy = s_dy;
-    ctx.s_ko = s_ko;
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const int ex = tl.e[0], ey = tl.e[1], ez = tl.e[2];
     ctx.ox = tl.o[0];
@@ -1034,6 +1051,26 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     for (int it = 0; it < DXIT; it++) ctx.dxw[it] = tabs[dtab + it * FT + tid];
 #pragma unroll
     for (int it = 0; it < DYIT; it++) ctx.dyw[it] = tabs[dtab + (DXIT + it) * FT + tid];
+    idx_out = idx;
+    q_out = q;
+    active_out = active;
+}
+
+template <bool FACE, class C, typename T, bool HALO = false>
+__device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &tl, const DBlock &B,
+                                           const dgr::AxisConsts *K, const uint16_t *__restrict__ tabs,
+                                           int dtab, double *s_tile, double *s_dx, double *s_dy, T *s_ko) {
+    FusedCtx<FACE, C, T> ctx(io);
+    ctx.K = K;
+    ctx.load_consts();
+    ctx.s_tile = s_tile;
+    ctx.s_dx = s_dx;
+    ctx.s_dy = s_dy;
+    ctx.s_ko = s_ko;
+    uint64_t idx;
+    int64_t q;
+    bool active;
+    tile_setup<FACE, C, T, HALO>(ctx, io, tl, B, tabs, dtab, idx, q, active);
     // this point's field values (registers; the compiler parks them in AGPRs)
     {
         T *f = ctx.f;
@@ -1058,7 +1095,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     ctx.begin();
     FusedOut<FACE, C, T> out(ctx);
     dgr::bssn_rhs_phased<T>(ctx, eta, lam, lamf, out);
-    if (active) fused_epilogue<C, T>(io, s_ko, ctx.f, tid, idx, q, prm.chi_floor);
+    if (active) fused_epilogue<C, T>(io, s_ko, ctx.f, ctx.tid, idx, q, prm.chi_floor);
 }
 
 // One launch per tile kind: FACE = false over the tiles of blocks without
@@ -1085,6 +1122,66 @@ __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES 
     const DBlock B = blocks[tl.block];
     fused_tile<FACE, C, T, HALO>(io, prm, tl, B, bconst + 3 * (size_t)tl.block, tabs, dtab, s_tile, s_dx, s_dy,
                                  s_ko);
+}
+
+struct ConsPtrs {
+    double *c[6];  // ham, mom0, mom1, mom2, psi4 real, psi4 imaginary (any may be null)
+};
+
+// The constraint diagnostic as one fused kernel per tile kind (the k_fused
+// machinery with the constraints' phase table, bssn_cons_phased.h): the 17
+// variables whose derivatives physical_constraints reads (physcon.cpp:12-89)
+// stream through LDS one at a time and the generated constraints run as far
+// as those derivatives allow; no derivative workspace. Outputs go straight to
+// the caller's arrays (null ones are skipped); psi4 is zero within 1e-7 of
+// the z axis (physcon.cpp:82-85).
+struct ConsFusedOut {
+    double v[6];
+    template <int I>
+    __device__ __forceinline__ void put(double x) { v[I] = x; }
+};
+
+template <bool FACE, class C>
+__global__ __launch_bounds__(C::FT, 1) void k_cons_fused(const StageIO io, ConsPtrs o,
+                                                        const DBlock *__restrict__ blocks,
+                                                        const dgr::AxisConsts *__restrict__ bconst,
+                                                        const Tile *__restrict__ tiles,
+                                                        const uint16_t *__restrict__ tabs, int dtab) {
+    using Ctx = FusedCtx<FACE, C, double, ConsPhases>;
+    constexpr bool DEEP = Ctx::DEEP;
+    __shared__ double s_tile[(DEEP ? 3 : 2) * C::NREG];
+    __shared__ double s_dx[(DEEP ? 2 : 1) * C::NSDX];
+    __shared__ double s_dy[(DEEP ? 2 : 1) * C::NSDY];
+    const Tile tl = tiles[blockIdx.x];
+    const DBlock B = blocks[tl.block];
+    Ctx ctx(io);
+    ctx.K = bconst + 3 * (size_t)tl.block;
+    ctx.load_consts();
+    ctx.s_tile = s_tile;
+    ctx.s_dx = s_dx;
+    ctx.s_dy = s_dy;
+    ctx.s_ko = nullptr;
+    uint64_t idx;
+    int64_t q;
+    bool active;
+    tile_setup<FACE, C, double, false>(ctx, io, tl, B, tabs, dtab, idx, q, active);
+#pragma unroll
+    for (int v = 0; v < NV; v++) ctx.f[v] = io.src.p[v][idx];
+    // x = pmin + i h as physcon.cpp:75-77 computes it
+    ctx.xc[0] = __dadd_rn(B.pmin[0], __dmul_rn((double)ctx.ax, B.h[0]));
+    ctx.xc[1] = __dadd_rn(B.pmin[1], __dmul_rn((double)ctx.ay, B.h[1]));
+    ctx.xc[2] = __dadd_rn(B.pmin[2], __dmul_rn((double)ctx.az, B.h[2]));
+    ctx.template fetch<0>();
+    if constexpr (C::PD > 1) ctx.template fetch<1>();
+    if constexpr (C::PD > 2) ctx.template fetch<2>();
+    ctx.begin();
+    ConsFusedOut out;
+    dgr::bssn_cons_phased<double>(ctx, out);
+    if (!active) return;
+    const bool axis = fabs(ctx.xc[0]) <= 1e-7 && fabs(ctx.xc[1]) <= 1e-7;
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+        if (o.c[i]) o.c[i][idx] = (i >= 4 && axis) ? 0.0 : out.v[i];
 }
 
 // Copies the padding points of every block of src into dst (one workgroup per
@@ -1148,6 +1245,78 @@ __global__ __launch_bounds__(256) void k_halo_fill(VarPtrs u, const DBlock *__re
     }
 }
 
+// The rest of the padding refresh (the interpolating part of unzip,
+// BSSN_GR/src/bssnCtx.cpp:75; the octree code is in dendrolib): one record
+// per padding point that has no same-size neighbour in its direction but lies
+// in the interior of another block, which gives the value: a block of the same
+// spacing (any size) or of half the spacing (a finer block: the point is one of
+// its nodes, injection) by a copy, a block of twice the spacing (a coarser
+// block) by order-6 Lagrange interpolation in its element holding the point.
+// Records read interior points only and write padding points only.
+struct FillRec {
+    uint64_t dst;   // the padding point
+    uint64_t src;   // copy: the source node; interpolation: the element's first node
+    uint32_t mx;    // interpolation: x and xy strides of the source block
+    uint32_t mxy;
+    uint32_t code;  // bits 0-3, 4-7, 8-11: 2t per axis, t in [0, 6] the element-local
+                    // coordinate; bit 12: interpolation (else copy)
+    uint32_t pad;
+};
+constexpr uint32_t FILL_INTERP = 1u << 12;
+
+// Weight of node m of the 7-node order-6 element at t = code/2 (the Lagrange
+// basis, rounded as the oracle's orc_lagrange6_half).
+__device__ __forceinline__ double lag6_weight(int code, int m) {
+#pragma clang fp contract(off)
+    const double t = 0.5 * (double)code;
+    double num = 1.0, den = 1.0;
+#pragma unroll
+    for (int l = 0; l < 7; l++) {
+        if (l == m) continue;
+        num *= (t - (double)l);
+        den *= (double)(m - l);
+    }
+    return num / den;
+}
+
+__global__ __launch_bounds__(256) void k_fill_general(VarPtrs u, const FillRec *__restrict__ recs, int64_t n) {
+#pragma clang fp contract(off)
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const FillRec r = recs[q];
+    if (!(r.code & FILL_INTERP)) {
+        double t[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) t[v] = u.p[v][r.src];
+#pragma unroll
+        for (int v = 0; v < NV; v++) u.p[v][r.dst] = t[v];
+        return;
+    }
+    double wx[7], wy[7], wz[7];
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+        wx[m] = lag6_weight((int)(r.code & 15u), m);
+        wy[m] = lag6_weight((int)((r.code >> 4) & 15u), m);
+        wz[m] = lag6_weight((int)((r.code >> 8) & 15u), m);
+    }
+    for (int v = 0; v < NV; v++) {
+        const double *base = u.p[v] + r.src;
+        double sz = 0.0;
+        for (int c3 = 0; c3 < 7; c3++) {
+            double sy = 0.0;
+            for (int b2 = 0; b2 < 7; b2++) {
+                const double *row = base + (size_t)r.mx * b2 + (size_t)r.mxy * c3;
+                double sx = 0.0;
+#pragma unroll
+                for (int a1 = 0; a1 < 7; a1++) sx += wx[a1] * row[a1];
+                sy += wy[b2] * sx;
+            }
+            sz += wz[c3] * sy;
+        }
+        u.p[v][r.dst] = sz;
+    }
+}
+
 // -------------------------------------------------------------------------
 // Hamiltonian constraint (C_HAM of physical_constraints, physcon.cpp:12-89):
 // the generated constraint at every interior point of a chunk, from the
@@ -1172,9 +1341,6 @@ struct ConsOut {
     }
 };
 
-struct ConsPtrs {
-    double *c[6];  // ham, mom0, mom1, mom2, psi4 real, psi4 imaginary (any may be null)
-};
 
 template <int MASK>
 __global__ __launch_bounds__(NTHREADS) void k_constraints(VarPtrs u, ConsPtrs o,
@@ -1220,11 +1386,34 @@ struct NormMask {
     double r1, r2;
 };
 
+// With a neighbour table (27 per block, -1 for none), a node on a face, edge
+// or corner of block b's interior (side s_a = -1 at index PW, +1 at n-PW-1) is
+// shared with the neighbours in the directions d with d_a in {0, s_a}; these
+// are linked to each other, so all of them see the same set, and the one with
+// the lowest block index counts the node: each zipped node is counted once, as
+// computeConstraintL2Norm does (grUtils.tcc:60-110).
+__device__ __forceinline__ bool owned_elsewhere(const int *__restrict__ nbr, int b, const PointGeom &g,
+                                                const DBlock &B) {
+    if (!nbr) return false;
+    const int sx = g.i == PW ? -1 : (g.i == B.n[0] - PW - 1 ? 1 : 0);
+    const int sy = g.j == PW ? -1 : (g.j == B.n[1] - PW - 1 ? 1 : 0);
+    const int sz = g.k == PW ? -1 : (g.k == B.n[2] - PW - 1 ? 1 : 0);
+    if (!(sx | sy | sz)) return false;
+    for (int dz = min(sz, 0); dz <= max(sz, 0); dz++)
+        for (int dy = min(sy, 0); dy <= max(sy, 0); dy++)
+            for (int dx = min(sx, 0); dx <= max(sx, 0); dx++) {
+                if (!(dx | dy | dz)) continue;
+                const int o = nbr[27 * b + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+                if (o >= 0 && o < b) return true;
+            }
+    return false;
+}
+
 __global__ __launch_bounds__(NTHREADS) void k_norm_partial(const double *__restrict__ c, NormMask m,
                                                           const DBlock *__restrict__ blocks,
                                                           const Tile *__restrict__ tiles,
                                                           const int *__restrict__ wg_tile, int tile_end, int64_t P,
-                                                          double *__restrict__ psum,
+                                                          const int *__restrict__ nbr, double *__restrict__ psum,
                                                           unsigned long long *__restrict__ pcnt) {
 #pragma clang fp contract(off)
     __shared__ double s_sum[NTHREADS];
@@ -1245,7 +1434,7 @@ __global__ __launch_bounds__(NTHREADS) void k_norm_partial(const double *__restr
                                (z - m.bh[2]) * (z - m.bh[2]));
         const double d2 = sqrt((x - m.bh[3]) * (x - m.bh[3]) + (y - m.bh[4]) * (y - m.bh[4]) +
                                (z - m.bh[5]) * (z - m.bh[5]));
-        if (!(d1 < m.r1 || d2 < m.r2)) {
+        if (!(d1 < m.r1 || d2 < m.r2) && !owned_elsewhere(nbr, T.block, g, B)) {
             const double v = c[g.idx];
             v2 = v * v;
             cnt = 1;
@@ -1267,9 +1456,11 @@ __global__ __launch_bounds__(NTHREADS) void k_norm_partial(const double *__restr
     }
 }
 
+// out[0], out[1] = sum and count; with norm_out, also *norm_out = sqrt(sum /
+// count), or -1 when no point qualifies.
 __global__ __launch_bounds__(NTHREADS) void k_norm_final(const double *__restrict__ psum,
                                                         const unsigned long long *__restrict__ pcnt, int n,
-                                                        double *__restrict__ out) {
+                                                        double *__restrict__ out, double *__restrict__ norm_out) {
 #pragma clang fp contract(off)
     __shared__ double s_sum[NTHREADS];
     __shared__ unsigned long long s_cnt[NTHREADS];
@@ -1292,6 +1483,7 @@ __global__ __launch_bounds__(NTHREADS) void k_norm_final(const double *__restric
     if (threadIdx.x == 0) {
         out[0] = s_sum[0];
         out[1] = (double)s_cnt[0];
+        if (norm_out) *norm_out = s_cnt[0] ? sqrt(s_sum[0] / (double)s_cnt[0]) : -1.0;
     }
 }
 
@@ -1352,6 +1544,7 @@ extern "C" __attribute__((visibility("hidden"))) void dgr_internal_fused_f32_ker
 #else
 extern "C" void dgr_internal_fused_f32_kernels(const void **k);
 
+
 struct dgr_plan {
     int device = 0;
     uint64_t szpdof = 0;
@@ -1382,6 +1575,10 @@ struct dgr_plan {
     int64_t *d_nbo = nullptr;   // the same as neighbour block offsets (-1 for none), for k_fused
     int64_t nlinks = -1;        // links in the table (-1: not built yet)
     bool halo = false;          // dgr_rk4_step(s): refresh padding from neighbours before each stage
+    bool fill_general = false;  // ... and from blocks of other sizes or levels (k_fill_general)
+    FillRec *d_fill = nullptr;  // general fill records (built on first use)
+    int64_t nfill = -1;         // records (-1: not built yet)
+    int64_t nfill_interp = 0;   // of which interpolations
     double *d_istage = nullptr; // k_fused RK4: 2 * 24 * interior (u0 copy, accumulator)
     double *d_wstate = nullptr; // k_fused RK4: 24 * szpdof, the padded work state W
     int *d_err = nullptr;
@@ -1394,6 +1591,7 @@ struct dgr_plan {
         dgr_bssn_params prm;
         int enforce;
         bool halo;
+        bool fill_general;
     } rk4_key{};
     int nwg_total = 0;                    // k_rhs-style workgroups over all chunks
     double *d_norm = nullptr;             // norm partials: nwg_total sums, nwg_total counts, 2 results
@@ -1613,6 +1811,7 @@ hipError_t build_ftiles(dgr_plan *plan, int p) {
 }
 
 int link_neighbors(dgr_plan *plan);
+int build_fill(dgr_plan *plan);
 
 }  // namespace
 
@@ -1827,6 +2026,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     (void)hipFree(plan->d_wstate);
     (void)hipFree(plan->d_nbr);
     (void)hipFree(plan->d_nbo);
+    (void)hipFree(plan->d_fill);
     (void)hipFree(plan->d_err);
     (void)hipFree(plan->d_norm);
     if (plan->rk4_exec) (void)hipGraphExecDestroy(plan->rk4_exec);
@@ -1863,6 +2063,7 @@ int dgr_plan_move_block(dgr_plan *plan, uint32_t b, uint64_t offset, const doubl
     if (plan->nlinks >= 0) {  // the neighbour table follows the geometry
         int rc = link_neighbors(plan);
         if (rc) return rc;
+        if (plan->fill_general && (rc = build_fill(plan))) return rc;
     }
     return DGR_OK;
 }
@@ -2303,6 +2504,202 @@ int link_neighbors(dgr_plan *plan) {
     }
     if (nb) HIP_OK(hipMemcpy(plan->d_nbo, off.data(), sizeof(int64_t) * 27 * nb, hipMemcpyHostToDevice));
     plan->nlinks = links;
+    plan->nfill = -1;  // the general fill records follow the links
+    return DGR_OK;
+}
+
+// Where a padding point at coordinate X along axis a lies on block C's grid,
+// within C's interior: mirrors the oracle's padding_source. rel 0 (same
+// spacing) and 1 (C finer) need a node of C: q is that node; rel 2 (C coarser)
+// gives the first node q of C's element (6 intervals) holding X and cd = 2t,
+// t the element-local coordinate.
+bool fill_axis(const DBlock &C, int a, double X, int rel, int &q, int &cd) {
+    const double qa = (X - C.pmin[a]) / C.h[a];
+    const double q2 = 2.0 * qa;
+    const double r2 = std::floor(q2 + 0.5);
+    if (std::fabs(q2 - r2) > 2e-6) return false;
+    const long i2 = (long)r2;
+    if (i2 < 2 * PW || i2 > 2 * ((long)C.n[a] - PW - 1)) return false;
+    if (rel < 2 && (i2 & 1)) return false;
+    if (rel < 2) {
+        q = (int)(i2 / 2);
+        cd = 0;
+    } else {
+        const long t2 = i2 - 2 * PW, ne = ((long)C.n[a] - 2 * PW - 1) / 6;
+        long e = t2 / 12;
+        if (e > ne - 1) e = ne - 1;
+        q = (int)(PW + 6 * e);
+        cd = (int)(t2 - 12 * e);
+    }
+    return true;
+}
+
+// Spacing of C relative to B along x: 0 same, 1 half (C finer), 2 twice (C
+// coarser), -1 other (the oracle's rule, relative tolerance 1e-9).
+int fill_rel(const DBlock &B, const DBlock &C) {
+    const double hb = B.h[0], hc = C.h[0];
+    if (std::fabs(hc - hb) <= 1e-9 * hb) return 0;
+    if (std::fabs(2.0 * hc - hb) <= 1e-9 * hb) return 1;
+    if (std::fabs(hc - 2.0 * hb) <= 1e-9 * hb) return 2;
+    return -1;
+}
+
+// The general fill records: for every padding point of every block whose
+// direction has no same-size neighbour (the link table must be built), the
+// block whose interior holds it, preferring the same spacing, then half, then
+// twice the spacing, ties to the lower block index (as the oracle's
+// orc_padding_fill). Candidates come from a hash of the blocks' interior boxes
+// per spacing; the point set of a direction's padding region is a tensor
+// product, so each candidate is matched one axis at a time.
+int build_fill(dgr_plan *plan) {
+    const size_t nb = plan->hblocks.size();
+    const auto &H = plan->hblocks;
+    std::vector<double> lvh;  // distinct x spacings
+    std::vector<int> lv(nb);
+    for (size_t b = 0; b < nb; b++) {
+        int l = -1;
+        for (size_t k = 0; k < lvh.size() && l < 0; k++)
+            if (std::fabs(H[b].h[0] - lvh[k]) <= 1e-12 * lvh[k]) l = (int)k;
+        if (l < 0) {
+            l = (int)lvh.size();
+            lvh.push_back(H[b].h[0]);
+        }
+        lv[b] = l;
+    }
+    std::vector<double> cell(lvh.size(), 0.0);  // hash cell edge per spacing: its largest interior extent
+    for (size_t b = 0; b < nb; b++)
+        for (int a = 0; a < 3; a++) cell[lv[b]] = std::max(cell[lv[b]], (H[b].n[a] - 2 * PW - 1) * H[b].h[a]);
+    std::map<std::array<int64_t, 4>, std::vector<int>> grid;
+    auto cell_of = [&](int l, double x) { return (int64_t)std::floor(x / cell[l]); };
+    for (size_t b = 0; b < nb; b++) {
+        const int l = lv[b];
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            const double e = 1e-6 * cell[l];
+            lo[a] = cell_of(l, H[b].pmin[a] + PW * H[b].h[a] - e);
+            hi[a] = cell_of(l, H[b].pmin[a] + (H[b].n[a] - PW - 1) * H[b].h[a] + e);
+        }
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++) grid[{l, x, y, z}].push_back((int)b);
+    }
+    struct Best {
+        int rel, c, q[3], cd[3];
+    };
+    std::vector<FillRec> recs;
+    int64_t ninterp = 0;
+    std::vector<Best> best;
+    std::vector<int> cand;
+    for (size_t b = 0; b < nb; b++) {
+        const DBlock &B = H[b];
+        for (int d = 0; d < 27; d++) {
+            const int dd[3] = {d % 3 - 1, (d / 3) % 3 - 1, d / 9 - 1};
+            if (!(dd[0] | dd[1] | dd[2]) || plan->hnbr[27 * b + d] >= 0) continue;
+            int r0[3], r1[3];  // the region's index range per axis
+            double xlo[3], xhi[3];
+            for (int a = 0; a < 3; a++) {
+                r0[a] = dd[a] < 0 ? 0 : (dd[a] == 0 ? PW : B.n[a] - PW);
+                r1[a] = dd[a] < 0 ? PW : (dd[a] == 0 ? B.n[a] - PW : B.n[a]);
+                xlo[a] = B.pmin[a] + r0[a] * B.h[a];
+                xhi[a] = B.pmin[a] + (r1[a] - 1) * B.h[a];
+            }
+            cand.clear();
+            for (int l = 0; l < (int)lvh.size(); l++) {
+                const double h = lvh[l];
+                if (!(std::fabs(h - B.h[0]) <= 1e-8 * B.h[0] || std::fabs(2.0 * h - B.h[0]) <= 1e-8 * B.h[0] ||
+                      std::fabs(h - 2.0 * B.h[0]) <= 1e-8 * B.h[0]))
+                    continue;
+                int64_t lo[3], hi[3];
+                for (int a = 0; a < 3; a++) {
+                    const double e = 1e-6 * cell[l];
+                    lo[a] = cell_of(l, xlo[a] - e);
+                    hi[a] = cell_of(l, xhi[a] + e);
+                }
+                for (int64_t z = lo[2]; z <= hi[2]; z++)
+                    for (int64_t y = lo[1]; y <= hi[1]; y++)
+                        for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                            auto it = grid.find({l, x, y, z});
+                            if (it == grid.end()) continue;
+                            for (int c : it->second)
+                                if (c != (int)b) cand.push_back(c);
+                        }
+            }
+            if (cand.empty()) continue;
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+            const int ex = r1[0] - r0[0], ey = r1[1] - r0[1], ez = r1[2] - r0[2];
+            best.assign((size_t)ex * ey * ez, Best{3, -1, {0, 0, 0}, {0, 0, 0}});
+            bool any = false;
+            for (int c : cand) {
+                const DBlock &C = H[c];
+                const int rel = fill_rel(B, C);
+                if (rel < 0) continue;
+                std::vector<std::array<int, 3>> ax[3];  // (local index, q, cd) per axis
+                bool ok = true;
+                for (int a = 0; a < 3 && ok; a++) {
+                    for (int i = r0[a]; i < r1[a]; i++) {
+                        int q, cd;
+                        if (fill_axis(C, a, B.pmin[a] + i * B.h[a], rel, q, cd)) ax[a].push_back({i - r0[a], q, cd});
+                    }
+                    ok = !ax[a].empty();
+                }
+                if (!ok) continue;
+                for (const auto &z : ax[2])
+                    for (const auto &y : ax[1])
+                        for (const auto &x : ax[0]) {
+                            Best &bb = best[(size_t)x[0] + (size_t)ex * (y[0] + (size_t)ey * z[0])];
+                            if (rel < bb.rel || (rel == bb.rel && c < bb.c)) {
+                                bb = Best{rel, c, {x[1], y[1], z[1]}, {x[2], y[2], z[2]}};
+                                any = true;
+                            }
+                        }
+            }
+            if (!any) continue;
+            for (int k = 0; k < ez; k++)
+                for (int j = 0; j < ey; j++)
+                    for (int i = 0; i < ex; i++) {
+                        const Best &bb = best[(size_t)i + (size_t)ex * (j + (size_t)ey * k)];
+                        if (bb.rel == 3) continue;
+                        const DBlock &C = H[bb.c];
+                        FillRec r{};
+                        r.dst = B.offset + (uint64_t)(r0[0] + i) +
+                                (uint64_t)B.n[0] * ((uint64_t)(r0[1] + j) + (uint64_t)B.n[1] * (r0[2] + k));
+                        r.src = C.offset + (uint64_t)bb.q[0] +
+                                (uint64_t)C.n[0] * ((uint64_t)bb.q[1] + (uint64_t)C.n[1] * bb.q[2]);
+                        r.mx = (uint32_t)C.n[0];
+                        r.mxy = (uint32_t)C.n[0] * (uint32_t)C.n[1];
+                        r.code = bb.rel == 2 ? (uint32_t)bb.cd[0] | ((uint32_t)bb.cd[1] << 4) |
+                                                   ((uint32_t)bb.cd[2] << 8) | FILL_INTERP
+                                             : 0u;
+                        ninterp += bb.rel == 2;
+                        recs.push_back(r);
+                    }
+        }
+    }
+    if (plan->rk4_exec) {  // a captured step may hold the old records
+        HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
+        plan->rk4_exec = nullptr;
+    }
+    if (plan->d_fill) {
+        HIP_OK(hipFree(plan->d_fill));
+        plan->d_fill = nullptr;
+    }
+    if (!recs.empty()) {
+        hipError_t e = hipMalloc(&plan->d_fill, sizeof(FillRec) * recs.size());
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "padding fill records");
+        HIP_OK(hipMemcpy(plan->d_fill, recs.data(), sizeof(FillRec) * recs.size(), hipMemcpyHostToDevice));
+    }
+    plan->nfill = (int64_t)recs.size();
+    plan->nfill_interp = ninterp;
+    return DGR_OK;
+}
+
+int launch_fill_general(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
+    if (plan->nfill <= 0) return DGR_OK;
+    const int64_t nwg = (plan->nfill + 255) / 256;
+    if (nwg > INT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many records");
+    hipLaunchKernelGGL(k_fill_general, dim3((uint32_t)nwg), dim3(256), 0, st, u, plan->d_fill, plan->nfill);
+    HIP_OK(hipGetLastError());
     return DGR_OK;
 }
 
@@ -2317,6 +2714,14 @@ int launch_halo_fill(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
                        (int)nb);
     HIP_OK(hipGetLastError());
     return DGR_OK;
+}
+
+// The whole refresh of u's padding: from same-size neighbours and, when the
+// plan's refresh covers them, the general records.
+int launch_padding_refresh(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
+    int rc = launch_halo_fill(plan, u, st);
+    if (!rc && plan->fill_general) rc = launch_fill_general(plan, u, st);
+    return rc;
 }
 
 // With the padding refresh on, FP64 stages read linked padding straight from
@@ -2355,9 +2760,12 @@ int rk4_fused_launches(dgr_plan *plan, const RkBuffers &buf, const VarPtrs &u, d
             // entry point (before the first step) or the previous step has refreshed
             return rc;
         }
+        // padding across level jumps and from blocks of other sizes: written
+        // into the stage input before every stage
+        if (plan->halo && plan->fill_general && (rc = launch_fill_general(plan, io.src, st))) return rc;
         if ((rc = run_fused(plan, io, prm, st))) return rc;
     }
-    if (plan->halo && !halo_in_kernel(plan)) return launch_halo_fill(plan, u, st);
+    if (plan->halo && !halo_in_kernel(plan)) return launch_padding_refresh(plan, u, st);
     return DGR_OK;
 }
 
@@ -2380,7 +2788,7 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
             (rc = rk4_fused_launches(plan, plan_rk_buffers(plan), u, dt, to_params(params), enforce_per_stage, st)))
             return rc;
         // in-kernel refresh: the caller's padding, refreshed from the new interiors
-        return halo_in_kernel(plan) ? launch_halo_fill(plan, u, st) : DGR_OK;
+        return halo_in_kernel(plan) ? launch_padding_refresh(plan, u, st) : DGR_OK;
     }
     if (plan->halo) return fail(DGR_E_INVALID, "the padding refresh needs the fused path (DGR_KERNEL=split is set)");
     const size_t n = plan->szpdof;
@@ -2429,6 +2837,7 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
     auto &key = plan->rk4_key;
     const int enf = enforce_per_stage ? 1 : 0;
     bool same = plan->rk4_exec && key.dt == dt && key.enforce == enf && key.halo == plan->halo &&
+                key.fill_general == plan->fill_general &&
                 std::memcmp(&key.prm, params, sizeof(*params)) == 0;
     for (int v = 0; same && v < NV; v++) same = key.u[v] == u.p[v];
     if (!same) {
@@ -2457,13 +2866,14 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         key.prm = *params;
         key.enforce = enf;
         key.halo = plan->halo;
+        key.fill_general = plan->fill_general;
     }
     if (nsteps > 0 && ((plan->halo && !halo_in_kernel(plan) && (rc = launch_halo_fill(plan, u, (hipStream_t)stream))) ||
                        (rc = launch_pad_copy(plan, plan_rk_buffers(plan), u, (hipStream_t)stream))))
         return rc;
     for (int i = 0; i < nsteps; i++) HIP_OK(hipGraphLaunch(plan->rk4_exec, (hipStream_t)stream));
     // in-kernel refresh: the caller's padding, refreshed from the new interiors
-    if (nsteps > 0 && halo_in_kernel(plan)) return launch_halo_fill(plan, u, (hipStream_t)stream);
+    if (nsteps > 0 && halo_in_kernel(plan)) return launch_padding_refresh(plan, u, (hipStream_t)stream);
     return DGR_OK;
 }
 
@@ -2489,14 +2899,42 @@ int dgr_halo_fill(dgr_plan *plan, double *const *d_u, void *stream) {
     return launch_halo_fill(plan, u, (hipStream_t)stream);
 }
 
-int dgr_plan_set_halo_refresh(dgr_plan *plan, int on) {
+int dgr_plan_set_halo_refresh(dgr_plan *plan, int mode) {
     g_err.clear();
     if (!plan) return fail(DGR_E_INVALID, "null plan");
-    if (on && !plan->fused) return fail(DGR_E_INVALID, "the padding refresh needs the fused path (DGR_KERNEL=split is set)");
+    if (mode < DGR_REFRESH_OFF || mode > DGR_REFRESH_FULL) return fail(DGR_E_INVALID, "unknown refresh mode");
+    if (mode && !plan->fused) return fail(DGR_E_INVALID, "the padding refresh needs the fused path (DGR_KERNEL=split is set)");
     HIP_OK(hipSetDevice(plan->device));
     int rc;
-    if (on && plan->nlinks < 0 && (rc = link_neighbors(plan))) return rc;
-    plan->halo = on != 0;
+    if (mode && plan->nlinks < 0 && (rc = link_neighbors(plan))) return rc;
+    if (mode == DGR_REFRESH_FULL && plan->nfill < 0 && (rc = build_fill(plan))) return rc;
+    plan->halo = mode != DGR_REFRESH_OFF;
+    plan->fill_general = mode == DGR_REFRESH_FULL;
+    return DGR_OK;
+}
+
+int dgr_padding_fill(dgr_plan *plan, double *const *d_u, void *stream) {
+    g_err.clear();
+    if (!plan) return fail(DGR_E_INVALID, "null plan");
+    VarPtrs u;
+    int rc;
+    if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
+    HIP_OK(hipSetDevice(plan->device));
+    if (plan->nlinks < 0 && (rc = link_neighbors(plan))) return rc;
+    if (plan->nfill < 0 && (rc = build_fill(plan))) return rc;
+    if ((rc = launch_halo_fill(plan, u, (hipStream_t)stream))) return rc;
+    return launch_fill_general(plan, u, (hipStream_t)stream);
+}
+
+int dgr_plan_fill_sources(dgr_plan *plan, int64_t *ncopy, int64_t *ninterp) {
+    g_err.clear();
+    if (!plan) return fail(DGR_E_INVALID, "null plan");
+    HIP_OK(hipSetDevice(plan->device));
+    int rc;
+    if (plan->nlinks < 0 && (rc = link_neighbors(plan))) return rc;
+    if (plan->nfill < 0 && (rc = build_fill(plan))) return rc;
+    if (ncopy) *ncopy = plan->nfill - plan->nfill_interp;
+    if (ninterp) *ninterp = plan->nfill_interp;
     return DGR_OK;
 }
 
@@ -2532,6 +2970,29 @@ int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, double 
     int rc;
     if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
     HIP_OK(hipSetDevice(plan->device));
+    hipStream_t st = (hipStream_t)stream;
+    if (plan->fused) {
+        // one fused kernel per tile kind over the FP64 tile list; no workspace
+        StageIO io{};
+        io.src = u;
+        io.stage = -1;
+        const Tile *tiles = plan->d_ftiles[0];
+        const int *fg = plan->fgroup[0];
+        int n = fg[FG_WIDE + 1] - fg[FG_WIDE];
+        if (n)
+            hipLaunchKernelGGL((k_cons_fused<false, CfgWide>), dim3(n), dim3(CfgWide::FT), 0, st, io, o,
+                               plan->d_blocks, plan->d_bconst, tiles + fg[FG_WIDE], plan->d_tabs, plan->dtab_wide);
+        n = fg[FG_THIN + 1] - fg[FG_THIN];
+        if (n)
+            hipLaunchKernelGGL((k_cons_fused<false, CfgThin>), dim3(n), dim3(CfgThin::FT), 0, st, io, o,
+                               plan->d_blocks, plan->d_bconst, tiles + fg[FG_THIN], plan->d_tabs, plan->dtab_thin);
+        n = fg[FG_FACE + 1] - fg[FG_FACE];
+        if (n)
+            hipLaunchKernelGGL((k_cons_fused<true, CfgWide>), dim3(n), dim3(CfgWide::FT), 0, st, io, o,
+                               plan->d_blocks, plan->d_bconst, tiles + fg[FG_FACE], plan->d_tabs, plan->dtab_wide);
+        HIP_OK(hipGetLastError());
+        return DGR_OK;
+    }
     if (!plan->d_W && plan->max_chunk_points) {
         hipError_t e = hipMalloc(&plan->d_W, sizeof(double) * NSLOT * (size_t)plan->max_chunk_points);
         if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "derivative workspace");
@@ -2541,7 +3002,6 @@ int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, double 
         (mask & CONS_PSI4) ? k_constraints<CONS_HAM | CONS_MOM | CONS_PSI4>
         : (mask & CONS_MOM) ? k_constraints<CONS_HAM | CONS_MOM>
                             : k_constraints<CONS_HAM>;
-    hipStream_t st = (hipStream_t)stream;
     for (const Chunk &c : plan->chunks) {
         const int ntile = c.tile_end - c.tile_begin;
         hipLaunchKernelGGL(k_derivs, dim3(ntile, NV), dim3(NTHREADS), 0, st, u, plan->d_blocks, plan->d_tiles,
@@ -2561,11 +3021,41 @@ int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, 
     return dgr_constraints(plan, d_ham, nullptr, nullptr, d_u, stream);
 }
 
+}  // extern "C"
+
+namespace {
+// The two norm kernels over every chunk: partials into psum/pcnt (one per
+// k_rhs-style workgroup, nwg_total of each), then the sum and count into
+// res[0..1] and, with d_norm, the norm into d_norm[0]. Linked neighbours
+// (when the table holds links) count each shared node once.
+int launch_norm(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2, double *psum,
+                unsigned long long *pcnt, double *res, double *d_norm, hipStream_t st) {
+    NormMask m;
+    for (int i = 0; i < 6; i++) m.bh[i] = bh[i];
+    m.r1 = r1;
+    m.r2 = r2;
+    const int *nbr = plan->nlinks > 0 ? plan->d_nbr : nullptr;
+    for (const Chunk &c : plan->chunks)
+        hipLaunchKernelGGL(k_norm_partial, dim3(c.nwg), dim3(NTHREADS), 0, st, d_c, m, plan->d_blocks, plan->d_tiles,
+                           plan->d_wg_tile + c.wg_begin, c.tile_end, (int64_t)c.npoints, nbr, psum + c.wg_begin,
+                           pcnt + c.wg_begin);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NTHREADS), 0, st, psum, pcnt, plan->nwg_total, res, d_norm);
+    HIP_OK(hipGetLastError());
+    return DGR_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int dgr_constraint_l2_norm(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2,
                            double *norm, void *stream) {
     g_err.clear();
     if (!plan || !d_c || !bh || !norm) return fail(DGR_E_INVALID, "null argument");
     HIP_OK(hipSetDevice(plan->device));
+    if (plan->nlinks < 0) {
+        const int rc = link_neighbors(plan);
+        if (rc) return rc;
+    }
     const int nwg = plan->nwg_total;
     if (!plan->d_norm) {
         hipError_t e = hipMalloc(&plan->d_norm, sizeof(double) * (2 * (size_t)nwg + 2));
@@ -2574,17 +3064,9 @@ int dgr_constraint_l2_norm(dgr_plan *plan, const double *d_c, const double *bh, 
     double *psum = plan->d_norm;
     auto *pcnt = reinterpret_cast<unsigned long long *>(plan->d_norm + nwg);
     double *res = plan->d_norm + 2 * (size_t)nwg;
-    NormMask m;
-    for (int i = 0; i < 6; i++) m.bh[i] = bh[i];
-    m.r1 = r1;
-    m.r2 = r2;
     hipStream_t st = (hipStream_t)stream;
-    for (const Chunk &c : plan->chunks)
-        hipLaunchKernelGGL(k_norm_partial, dim3(c.nwg), dim3(NTHREADS), 0, st, d_c, m, plan->d_blocks, plan->d_tiles,
-                           plan->d_wg_tile + c.wg_begin, c.tile_end, (int64_t)c.npoints, psum + c.wg_begin,
-                           pcnt + c.wg_begin);
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NTHREADS), 0, st, psum, pcnt, nwg, res);
-    HIP_OK(hipGetLastError());
+    const int rc = launch_norm(plan, d_c, bh, r1, r2, psum, pcnt, res, nullptr, st);
+    if (rc) return rc;
     double h[2] = {0.0, 0.0};
     HIP_OK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));

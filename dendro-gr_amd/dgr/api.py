@@ -24,6 +24,7 @@ SYMBOLS = [
     "dgr_ham_constraint", "dgr_constraint_l2_norm", "dgr_rk4_steps", "dgr_plan_set_precision",
     "dgr_plan_autotune", "dgr_plan_tiling", "dgr_plan_set_tiling", "dgr_constraints",
     "dgr_plan_link_neighbors", "dgr_halo_fill", "dgr_plan_set_halo_refresh",
+    "dgr_padding_fill", "dgr_plan_fill_sources",
 ]
 PRECISION = {"fp64": 0, "mixed": 1}
 TILINGS = ("wt", "w", "full")  # DGR_TILING_WT, _W, _FULL
@@ -82,6 +83,8 @@ def load():
         L.dgr_plan_link_neighbors.argtypes = [vp, vp, C.POINTER(C.c_int64)]
         L.dgr_halo_fill.argtypes = [vp, vp, vp]
         L.dgr_plan_set_halo_refresh.argtypes = [vp, C.c_int]
+        L.dgr_padding_fill.argtypes = [vp, vp, vp]
+        L.dgr_plan_fill_sources.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.dgr_plan_autotune.argtypes = [vp, vp, vp, C.POINTER(BssnParams), vp]
         L.dgr_plan_tiling.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]
         L.dgr_plan_set_tiling.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
@@ -203,9 +206,25 @@ class Plan:
         """Padding of `state` refreshed in place from same-level neighbours."""
         _check(load().dgr_halo_fill(self._h, var_pointers(state), _stream_ptr(stream)), "dgr_halo_fill")
 
-    def set_halo_refresh(self, on: bool):
-        """RK4 entry points refresh the padding from neighbours before every stage."""
-        _check(load().dgr_plan_set_halo_refresh(self._h, int(bool(on))), "dgr_plan_set_halo_refresh")
+    def padding_fill(self, state, stream=None):
+        """All of `state`'s padding refreshed in place: from same-level neighbours,
+        and elsewhere from the block of the same, half or twice the spacing
+        whose interior holds the point (copy, injection, order-6 interpolation)."""
+        _check(load().dgr_padding_fill(self._h, var_pointers(state), _stream_ptr(stream)), "dgr_padding_fill")
+
+    def fill_sources(self):
+        """(copied, interpolated) padding points of the refresh beyond the
+        same-level neighbours."""
+        a, b = C.c_int64(), C.c_int64()
+        _check(load().dgr_plan_fill_sources(self._h, C.byref(a), C.byref(b)), "dgr_plan_fill_sources")
+        return int(a.value), int(b.value)
+
+    def set_halo_refresh(self, on):
+        """RK4 entry points refresh the padding before every stage: False/0 off,
+        True/1 from same-level neighbours, 2 or "full" also across level jumps
+        and from blocks of other sizes."""
+        mode = 2 if on == "full" else int(on)
+        _check(load().dgr_plan_set_halo_refresh(self._h, mode), "dgr_plan_set_halo_refresh")
 
     def constraint_norm(self, c, bh=((3.0, 0.0, 0.0), (-3.0, 0.0, 0.0)), r1: float = 5.0,
                         r2: float = 5.0, stream=None) -> float:

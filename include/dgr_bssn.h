@@ -182,7 +182,8 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
 
 /* One classic RK4 step (BSSN_GR/include/rkBSSN.h:50-53, src/rkBSSN.cpp:764-860)
  * of every block, in place on d_u. Padding points keep their values (blocks are
- * advanced without re-filling padding between stages). With
+ * advanced without re-filling padding between stages) unless the padding
+ * refresh is on (dgr_plan_set_halo_refresh). With
  * enforce_per_stage != 0 the algebraic constraints are enforced on every stage
  * state as rkBSSN.cpp:810-812 does; they are always enforced on the result
  * (rkBSSN.cpp:858, bssnCtx.cpp:1174-1182). Uses plan-owned buffers,
@@ -191,7 +192,8 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
  * interior-only states (the step's initial state and the accumulator,
  * 2 x 24 x interior-points doubles). The stages alternate between d_u and the
  * work state, so d_u's interior holds an intermediate stage state while the
- * call runs (its padding is never written). */
+ * call runs (its padding is written only by the padding refresh, when that is
+ * on). */
 int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
                  int enforce_per_stage, void *stream);
 
@@ -204,30 +206,49 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
 int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
                   int enforce_per_stage, int nsteps, void *stream);
 
-/* Padding refresh from same-level neighbours: the part of the reference's
- * unzip (BSSN_GR/src/bssnCtx.cpp:75, octree code in dendrolib) that needs no
- * interpolation. Block nb is block b's neighbour in direction
+/* Padding refresh, the reference's unzip (BSSN_GR/src/bssnCtx.cpp:75; the
+ * octree code is in dendrolib) restated on the block table.
+ *
+ * Same-level neighbours: block nb is block b's neighbour in direction
  * (di,dj,dk) in {-1,0,1}^3 when both have the same sizes and spacing and
  * pmin_nb = pmin_b + d (n - 7) h (the two share their boundary nodes); the
  * table is found from the block geometry. Padding points of b inside such a
- * neighbour take its value at the same coordinates; padding across a level
- * jump or the domain boundary (no same-level neighbour) is left as it is.
+ * neighbour take its value at the same coordinates (a copy).
+ *
+ * The rest (DGR_REFRESH_FULL, dgr_padding_fill): a padding point whose
+ * direction has no such neighbour takes its value from the block whose
+ * interior holds it: a block of the same spacing and another size (copy), of
+ * half the spacing (a finer block; the point is one of its nodes: injection)
+ * or of twice the spacing (a coarser block: order-6 Lagrange interpolation in
+ * its element of 6 intervals holding the point); the same spacing wins over
+ * half, half over twice, ties go to the lower block index. Padding with no
+ * such block (the domain boundary) is left as it is.
  *
  * dgr_plan_link_neighbors builds (again) the table; nbr (27 int32 per block,
  * index (di+1)+3(dj+1)+9(dk+1), -1 for none; may be NULL) and nlinks (may be
- * NULL) receive it. dgr_halo_fill refreshes the padding of d_u in place
- * (asynchronous on `stream`; builds the table on first use).
- * dgr_plan_set_halo_refresh(plan, 1) makes every stage of dgr_rk4_step(s)
- * see its input state's padding refreshed from the neighbours, as the
- * reference's unzip between stages does (rkBSSN.cpp:764-860): FP64 stages
- * read linked padding from the neighbours inside the stage kernel, mixed
- * precision copies it in before each stage. When the call returns, the
- * caller's padding is refreshed from the new interiors. 0 (the default) keeps
- * every block's padding at its value at the start of the call. Needs the
- * fused path. */
+ * NULL) receive it. dgr_halo_fill refreshes the padding of d_u from the
+ * same-level neighbours in place, dgr_padding_fill all of it (asynchronous on
+ * `stream`; both build what they need on first use). dgr_plan_fill_sources
+ * builds the table of the rest (kept in the plan) and reports how many
+ * padding points it copies and how many it interpolates.
+ * dgr_plan_set_halo_refresh(plan, mode) makes every stage of
+ * dgr_rk4_step(s) see its input state's padding refreshed, as the
+ * reference's unzip between stages does (rkBSSN.cpp:764-860):
+ * DGR_REFRESH_LINKED from same-level neighbours (FP64 stages read that
+ * padding from the neighbours inside the stage kernel, mixed precision copies
+ * it in before each stage), DGR_REFRESH_FULL also the rest (written into the
+ * stage input before each stage). When the call returns, the caller's padding
+ * is refreshed from the new interiors the same way. DGR_REFRESH_OFF (the
+ * default) keeps every block's padding at its value at the start of the call.
+ * Needs the fused path. */
+#define DGR_REFRESH_OFF 0
+#define DGR_REFRESH_LINKED 1
+#define DGR_REFRESH_FULL 2
 int dgr_plan_link_neighbors(dgr_plan *plan, int32_t *nbr, int64_t *nlinks);
 int dgr_halo_fill(dgr_plan *plan, double *const *d_u, void *stream);
-int dgr_plan_set_halo_refresh(dgr_plan *plan, int on);
+int dgr_padding_fill(dgr_plan *plan, double *const *d_u, void *stream);
+int dgr_plan_fill_sources(dgr_plan *plan, int64_t *ncopy, int64_t *ninterp);
+int dgr_plan_set_halo_refresh(dgr_plan *plan, int mode);
 
 /* enforce_bssn_constraints (BSSN_GR/include/bssn_constraints.h:15-172) on the
  * interior points of every block: det(gt) = 1, tr(At) = 0, chi and alpha
@@ -239,7 +260,7 @@ int dgr_enforce_constraints(dgr_plan *plan, double *const *d_u, double chi_floor
  * variable; padding not written). Replaces the C_HAM output of
  * physical_constraints (BSSN_GR/src/physcon.cpp:12-89, generated equations
  * physconeqs.cpp:1373, derivatives constraint_derivs.h). Asynchronous on
- * `stream`; uses the plan's derivative workspace (allocated on first use). */
+ * `stream`; dgr_constraints with only d_ham. */
 int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, void *stream);
 
 /* physical_constraints (BSSN_GR/src/physcon.cpp:12-89) at the interior points
@@ -255,8 +276,10 @@ int dgr_ham_constraint(dgr_plan *plan, double *d_ham, const double *const *d_u, 
  * Equations from CodeGen/constraints.py:42-282 (generated into physconeqs.cpp).
  * d_mom and d_psi4 are host arrays of 3 and 2 device pointers, or NULL; d_ham
  * and any entry may be NULL (that output is skipped), but not all six.
- * Asynchronous on `stream`; uses the plan's derivative workspace (allocated on
- * first use). */
+ * Asynchronous on `stream`. The fused path evaluates derivatives and
+ * constraints in one kernel per tile kind with no workspace (all six are
+ * computed whatever is asked for); DGR_KERNEL=split uses the plan's
+ * derivative workspace (allocated on first use). */
 int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, double *const *d_psi4,
                     const double *const *d_u, void *stream);
 
@@ -264,10 +287,15 @@ int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, double 
  * of c^2 over the interior points whose distance to (bh[0],bh[1],bh[2]) is >= r1
  * and to (bh[3],bh[4],bh[5]) is >= r2 (the reference's BSSN_BH_LOC and
  * BSSN_BH{1,2}_CONSTRAINT_R, default 5.0, parameters.cpp:71-72). `d_c` is laid
- * out like a variable (e.g. the output of dgr_ham_constraint). Waits for
- * `stream`; *norm = -1 when no point qualifies. */
+ * out like a variable (e.g. the output of dgr_ham_constraint). A node shared
+ * by same-level neighbours (see dgr_plan_link_neighbors; the table is built on
+ * first use) is counted once, by the neighbour with the lowest block index, as
+ * the reference counts each zipped node once (grUtils.tcc:60-110); nodes
+ * shared across a level jump or between blocks of different sizes count once
+ * per block. Waits for `stream`; *norm = -1 when no point qualifies. */
 int dgr_constraint_l2_norm(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2,
                            double *norm, void *stream);
+
 
 /* Waits for `stream`, then reports and clears constraint violations met by
  * dgr_rk4_step / dgr_enforce_constraints since the last call: DGR_E_CONSTRAINT

@@ -916,20 +916,24 @@ int orc_ham_batch(double *ham, const double *const *u, const orc_block *blks, in
  * mean of c^2 over the nodes (here: interior points of every block) whose
  * distance to BH 1 is >= r1 and to BH 2 is >= r2. bh = {x1,y1,z1,x2,y2,z2}.
  * Returns -1 when no point qualifies. */
-/* A node on a low face, edge or corner of block b (index PW along each such
- * axis) is also a node of every neighbour lying in a direction d with d_a = -1
- * only along those axes; it is counted by the block with no such neighbour,
- * so a node shared by neighbouring blocks is counted once, as
- * computeConstraintL2Norm counts each zipped node once (grUtils.tcc:60-110,
- * its `accumulated` flags). nbr: 27 per block, -1 for none (NULL: blocks are
- * taken as disjoint). */
-static int owned_elsewhere(const int32_t *nbr, int64_t b, int lox, int loy, int loz) {
-    if (!nbr || !(lox | loy | loz)) return 0;
-    for (int dz = loz ? -1 : 0; dz <= 0; dz++)
-        for (int dy = loy ? -1 : 0; dy <= 0; dy++)
-            for (int dx = lox ? -1 : 0; dx <= 0; dx++) {
+/* A node on a face, edge or corner of block b's interior (index PW or n-PW-1
+ * along each such axis, side s_a = -1 or +1) is also a node of every
+ * neighbour lying in a direction d with d_a in {0, s_a} along those axes and
+ * d_a = 0 along the others; same-size neighbours holding one node are all
+ * linked to each other, so each of them sees the same set, and the node is
+ * counted by the one with the lowest block index. A node shared by
+ * neighbouring blocks is thus counted once, as computeConstraintL2Norm counts
+ * each zipped node once (grUtils.tcc:60-110, its `accumulated` flags), also
+ * where a lattice has holes. nbr: 27 per block, -1 for none (NULL: blocks
+ * are taken as disjoint). */
+static int owned_elsewhere(const int32_t *nbr, int64_t b, int sx, int sy, int sz) {
+    if (!nbr || !(sx | sy | sz)) return 0;
+    for (int dz = sz < 0 ? sz : 0; dz <= (sz > 0 ? sz : 0); dz++)
+        for (int dy = sy < 0 ? sy : 0; dy <= (sy > 0 ? sy : 0); dy++)
+            for (int dx = sx < 0 ? sx : 0; dx <= (sx > 0 ? sx : 0); dx++) {
                 if (!(dx | dy | dz)) continue;
-                if (nbr[27 * b + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] >= 0) return 1;
+                const int32_t o = nbr[27 * b + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+                if (o >= 0 && o < b) return 1;
             }
     return 0;
 }
@@ -949,7 +953,9 @@ double orc_constraint_l2_norm_nbr(const double *c_all, const orc_block *blks, in
         for (int k = 3; k < nz - 3; k++)
             for (int j = 3; j < ny - 3; j++)
                 for (int i = 3; i < nx - 3; i++) {
-                    if (owned_elsewhere(nbr, b, i == 3, j == 3, k == 3)) continue;
+                    if (owned_elsewhere(nbr, b, i == 3 ? -1 : (i == nx - 4 ? 1 : 0), j == 3 ? -1 : (j == ny - 4 ? 1 : 0),
+                                        k == 3 ? -1 : (k == nz - 4 ? 1 : 0)))
+                        continue;
                     const double x = blk->pmin[0] + i * hx, y = blk->pmin[1] + j * hy, z = blk->pmin[2] + k * hz;
                     const double d1 = sqrt((x - bh[0]) * (x - bh[0]) + (y - bh[1]) * (y - bh[1]) + (z - bh[2]) * (z - bh[2]));
                     const double d2 = sqrt((x - bh[3]) * (x - bh[3]) + (y - bh[4]) * (y - bh[4]) + (z - bh[5]) * (z - bh[5]));

@@ -171,3 +171,79 @@ def test_refresh_on_a_lattice_of_19_cubed_blocks():
     log(f"  interior increments normwise {e:.3e}")
     assert e <= 1e-11
     plan.close()
+
+
+# --- the rest of the refresh: blocks of other sizes and across level jumps ---
+
+def fill_cases():
+    """A two-level lattice (one coarse cell refined: copies, injections from
+    the finer blocks, interpolations from the coarser ones), a lattice of one
+    spacing and two block sizes (copies only), and the lattice with a hole
+    and a face-flagged block (diagonal neighbours fill padding next to the
+    hole)."""
+    return {"two_level": W.two_level_lattice(), "mixed_size": W.mixed_size_lattice(), "holes": lattice()}
+
+
+@pytest.mark.parametrize("case", ["two_level", "mixed_size", "holes"])
+def test_padding_fill_is_bit_exact(case):
+    """dgr_padding_fill equals the oracle's orc_padding_fill(general) bit for
+    bit at every point (copies, and interpolations summed in the oracle's
+    order with contraction off)."""
+    import torch
+    log(f"padding fill: {case}")
+    t = fill_cases()[case]
+    st = np.random.default_rng(5).standard_normal((24, W.total_padded(t)))
+    plan = dgr.Plan(t, st.shape[1])
+    ncopy, ninterp = plan.fill_sources()
+    log(f"  {ncopy} copied, {ninterp} interpolated padding points")
+    if case == "two_level":
+        assert ncopy > 0 and ninterp > 0
+    if case == "mixed_size":
+        assert ncopy > 0 and ninterp == 0
+    u = to_dev(st)
+    plan.padding_fill(u)
+    torch.cuda.synchronize()
+    ref = st.copy()
+    O.padding_fill(ref, t, W.neighbors(t), general=True)
+    got = u.cpu().numpy()
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    plan.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_rk4_with_full_refresh_matches_oracle(graph):
+    """RK4 on the two-level lattice with the padding of every stage refreshed
+    across the level jump (DGR_REFRESH_FULL) against the oracle's
+    orc_rk4_step_fill(general): interior increments within 1e-11 normwise,
+    the refreshed padding within 1e-11 of the increments' scale."""
+    import torch
+    log(f"full refresh: rk4 graph={graph}")
+    t = W.two_level_lattice()
+    st = smooth_state(t, amp=1e-2)
+    nbr = W.neighbors(t)
+    p = dgr.default_params(ko_sigma=0.4)
+    plan = dgr.Plan(t, st.shape[1])
+    plan.set_halo_refresh("full")
+    u = to_dev(st)
+    dt = 0.02
+    if graph:
+        plan.rk4_steps(u, dt, p, nsteps=2)
+    else:
+        for _ in range(2):
+            plan.rk4_step(u, dt, p)
+    torch.cuda.synchronize()
+    assert plan.status() == 0
+    ref = st.copy()
+    for _ in range(2):
+        O.rk4_step_fill(ref, t, nbr, dt, oracle_params(p), general=True, enforce_per_stage=True, nthreads=8)
+    got = u.cpu().numpy()
+    e = normwise(interior_values(got - st, t), interior_values(ref - st, t)).max()
+    log(f"  interior increments normwise {e:.3e}")
+    assert e <= 1e-11
+    assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref - st).max()
+    # the level-jump refresh changes the answer: without it the steps differ
+    ref2 = st.copy()
+    for _ in range(2):
+        O.rk4_step_halo(ref2, t, nbr, dt, oracle_params(p), enforce_per_stage=True, nthreads=8)
+    assert np.abs(ref2 - ref).max() > 1e-9
+    plan.close()

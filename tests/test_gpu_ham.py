@@ -89,6 +89,27 @@ def test_constraint_norm_matches_oracle():
     plan.close()
 
 
+def test_constraint_norm_counts_shared_nodes_once():
+    """On lattices whose neighbours share their boundary nodes (full, and with
+    holes so that some edge nodes are shared by diagonal neighbours only), the
+    norm counts each node once and equals the oracle's with the neighbour
+    table (which is pinned to a brute-force norm over the distinct nodes)."""
+    import torch
+    log("constraint norm on lattices (shared nodes)")
+    for shape, holes in (((2, 2, 2), ()), ((3, 2, 2), ((1, 0, 0), (0, 1, 1)))):
+        t = W.lattice(shape=shape, n=13, h=0.05, holes=holes)
+        nbr = W.neighbors(t)
+        n = W.total_padded(t)
+        c = np.cos(np.arange(n) * 0.37) + 0.2
+        plan = dgr.Plan(t, n)
+        dc = torch.from_numpy(c).cuda()
+        for bh, r in ((((30.0, 0, 0), (-30.0, 0, 0)), 5.0), (((0.2, 0.1, 0.0), (0.0, 0.0, 0.3)), 0.12)):
+            got = plan.constraint_norm(dc, bh=bh, r1=r, r2=r)
+            want = O.constraint_l2_norm(c, t, bh=bh, r1=r, r2=r, nbr=nbr)
+            assert got == pytest.approx(want, rel=1e-14, abs=0), (shape, holes, r)
+        plan.close()
+
+
 def test_ham_norm_unchanged_over_an_rk4_step():
     """The constraint norm of the state after one GPU RK4 step equals the
     oracle's (same step on the CPU) to far better than 3 digits."""

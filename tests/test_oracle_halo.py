@@ -150,36 +150,48 @@ def test_morton_order_links_the_same_lattice():
     assert {tuple(c) for c in cells} == {(i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)}
 
 
-def test_norm_counts_shared_nodes_once():
-    """computeConstraintL2Norm counts each zipped node once (grUtils.tcc:60-110):
-    on a 2x2x2 lattice whose neighbours share their boundary nodes, the norm
-    with the neighbour table equals a brute-force norm over the distinct
-    nodes (by their integer lattice position), with and without the mask; the
-    norm without the table counts a face node twice and differs."""
-    h, n = 0.05, 13
-    origin = np.array([-0.37, -0.21, -0.13])
-    t = W.lattice(shape=(2, 2, 2), n=n, h=h, origin=tuple(origin))
-    nbr = W.neighbors(t)
+def _distinct_nodes(t, n, h, origin, f):
+    """Values of f at the distinct interior nodes of lattice t (keyed by their
+    integer position), and the padded array of f over every block."""
     c = np.zeros(W.total_padded(t))
     nodes = {}
     for b in t:
         off = int(b["offset"])
         i = np.arange(n)
         z, y, x = np.meshgrid(*(b["pmin"][a] + i * h for a in (2, 1, 0)), indexing="ij")
-        f = np.cos(3.1 * x + 0.4) * np.sin(2.3 * y - 0.2) + 0.7 * z * z + 0.1
-        c[off:off + n ** 3] = f.ravel()
+        fv = f(x, y, z)
+        c[off:off + n ** 3] = fv.ravel()
         for k in range(3, n - 3):
             for j in range(3, n - 3):
                 for ii in range(3, n - 3):
                     key = tuple(np.rint((np.array([x[k, j, ii], y[k, j, ii], z[k, j, ii]]) - origin) / h).astype(int))
-                    nodes.setdefault(key, (f[k, j, ii], (x[k, j, ii], y[k, j, ii], z[k, j, ii])))
-    assert len(nodes) == (2 * (n - 7) + 1) ** 3  # distinct nodes of the lattice
-    for bh, r in ((((30.0, 0, 0), (-30.0, 0, 0)), 5.0), (((3.0, 0, 0), (-30.0, 0, 0)), 3.0),
-                  (((0.2, 0.1, 0.0), (0.0, 0.0, 0.3)), 0.12)):
-        vals = [v for v, p in nodes.values()
-                if np.linalg.norm(np.subtract(p, bh[0])) >= r and np.linalg.norm(np.subtract(p, bh[1])) >= r]
-        want = float(np.sqrt(np.mean(np.square(vals))))
-        got = O.constraint_l2_norm(c, t, bh=bh, r1=r, r2=r, nbr=nbr)
-        assert abs(got - want) <= 1e-14 * want, (got, want)
-        loose = O.constraint_l2_norm(c, t, bh=bh, r1=r, r2=r)
-        assert abs(loose - want) > 1e-6 * want
+                    nodes.setdefault(key, (fv[k, j, ii], (x[k, j, ii], y[k, j, ii], z[k, j, ii])))
+    return c, nodes
+
+
+def test_norm_counts_shared_nodes_once():
+    """computeConstraintL2Norm counts each zipped node once (grUtils.tcc:60-110):
+    on lattices whose neighbours share their boundary nodes (a full 2x2x2 one,
+    and ones with holes, where a node on an edge is shared by blocks that are
+    diagonal neighbours only), the norm with the neighbour table equals a
+    brute-force norm over the distinct nodes (by their integer lattice
+    position), with and without the mask; the norm without the table counts
+    a face node twice and differs."""
+    h, n = 0.05, 13
+    origin = np.array([-0.37, -0.21, -0.13])
+    f = lambda x, y, z: np.cos(3.1 * x + 0.4) * np.sin(2.3 * y - 0.2) + 0.7 * z * z + 0.1
+    for shape, holes in (((2, 2, 2), ()), ((2, 2, 2), ((0, 0, 0),)), ((3, 2, 2), ((1, 0, 0), (0, 1, 1)))):
+        t = W.lattice(shape=shape, n=n, h=h, origin=tuple(origin), holes=holes)
+        nbr = W.neighbors(t)
+        c, nodes = _distinct_nodes(t, n, h, origin, f)
+        if not holes:
+            assert len(nodes) == (2 * (n - 7) + 1) ** 3  # distinct nodes of the lattice
+        for bh, r in ((((30.0, 0, 0), (-30.0, 0, 0)), 5.0), (((3.0, 0, 0), (-30.0, 0, 0)), 3.0),
+                      (((0.2, 0.1, 0.0), (0.0, 0.0, 0.3)), 0.12)):
+            vals = [v for v, p in nodes.values()
+                    if np.linalg.norm(np.subtract(p, bh[0])) >= r and np.linalg.norm(np.subtract(p, bh[1])) >= r]
+            want = float(np.sqrt(np.mean(np.square(vals))))
+            got = O.constraint_l2_norm(c, t, bh=bh, r1=r, r2=r, nbr=nbr)
+            assert abs(got - want) <= 1e-14 * want, (shape, holes, got, want)
+            loose = O.constraint_l2_norm(c, t, bh=bh, r1=r, r2=r)
+            assert abs(loose - want) > 1e-6 * want
