@@ -539,7 +539,8 @@ def large_octree(args, stream, dev):
         el = time.perf_counter() - t0
         if plan.status(stream) != 0:
             raise RuntimeError(f"constraint violation in the large octree ({prec})")
-        finite = bool(torch.isfinite(state).all().item())
+        # one variable at a time: a whole-state temporary would not fit beside the plan's buffers
+        finite = all(bool(torch.isfinite(state[v]).all().item()) for v in range(24))
         ham = torch.empty(wl.n_pad, dtype=torch.float64, device=dev)
         plan.ham(ham, state, stream=stream)
         norm = plan.constraint_norm(ham, stream=stream)
