@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1544,6 +1545,37 @@ extern "C" __attribute__((visibility("hidden"))) void dgr_internal_fused_f32_ker
 #else
 extern "C" void dgr_internal_fused_f32_kernels(const void **k);
 
+namespace {
+// Device scratch of dgr_constraint_l2_norm_noalloc: the norm's per-workgroup
+// partial sums (NORM_SCRATCH_WG doubles), counts (as many 64-bit integers)
+// and the final sum and count. Each GPU holds one copy until the process
+// ends (16 MiB), so that entry point allocates nothing; every call on a GPU
+// shares it. In host code `scratch` is only a stand-in for the symbol: kernels
+// get the device address that scratch_address() looked up for the device.
+constexpr int NORM_SCRATCH_WG = DGR_NOALLOC_NORM_MAX_WORKGROUPS;
+static __device__ __align__(256) double scratch[2 * (size_t)NORM_SCRATCH_WG + 2];
+
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;  // device -> address of its `scratch`
+
+// The device address of `scratch` on `device`, looked up once per device
+// with that device current (dgr_plan_create calls this after hipSetDevice).
+hipError_t scratch_address(int device, double **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    double *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    *out = ptr;
+    return hipSuccess;
+}
+}  // namespace
+
 
 struct dgr_plan {
     int device = 0;
@@ -1595,6 +1627,7 @@ struct dgr_plan {
     } rk4_key{};
     int nwg_total = 0;                    // k_rhs-style workgroups over all chunks
     double *d_norm = nullptr;             // norm partials: nwg_total sums, nwg_total counts, 2 results
+    double *d_scratch = nullptr;          // this device's `scratch` (dgr_constraint_l2_norm_noalloc)
     // optional per-kernel timing: event pairs recorded around each launch
     bool timing = false;
     std::vector<hipEvent_t> event_pool;
@@ -1977,6 +2010,7 @@ int dgr_plan_create(int device, const dgr_block *blocks, uint32_t nblocks, uint6
     if (const char *k = std::getenv("DGR_KERNEL")) plan->fused = std::string(k) != "split";
 
     hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = scratch_address(device, &plan->d_scratch);
     if (e == hipSuccess && nblocks) e = hipMalloc(&plan->d_blocks, sizeof(DBlock) * hb.size());
     if (e == hipSuccess && !tiles.empty()) e = hipMalloc(&plan->d_tiles, sizeof(Tile) * tiles.size());
     if (e == hipSuccess && !wg_tile.empty()) e = hipMalloc(&plan->d_wg_tile, sizeof(int) * wg_tile.size());
@@ -3072,6 +3106,25 @@ int dgr_constraint_l2_norm(dgr_plan *plan, const double *d_c, const double *bh, 
     HIP_OK(hipStreamSynchronize(st));
     *norm = h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : -1.0;
     return DGR_OK;
+}
+
+int dgr_constraint_l2_norm_noalloc(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2,
+                                   double *d_norm, void *stream) {
+    g_err.clear();
+    if (!plan || !d_c || !bh || !d_norm) return fail(DGR_E_INVALID, "null argument");
+    if (plan->nwg_total > NORM_SCRATCH_WG)
+        return fail(DGR_E_INVALID, "dgr_constraint_l2_norm_noalloc: the plan needs " + std::to_string(plan->nwg_total) +
+                                       " norm workgroups (256 interior points each); the scratch holds " +
+                                       std::to_string(NORM_SCRATCH_WG));
+    if (plan->nlinks < 0)
+        return fail(DGR_E_INVALID, "dgr_constraint_l2_norm_noalloc: the neighbour table is not built; call "
+                                   "dgr_plan_link_neighbors once at set-up");
+    if (!plan->d_scratch) return fail(DGR_E_INVALID, "dgr_constraint_l2_norm_noalloc: no scratch address");
+    HIP_OK(hipSetDevice(plan->device));
+    double *psum = plan->d_scratch;
+    auto *pcnt = reinterpret_cast<unsigned long long *>(plan->d_scratch + NORM_SCRATCH_WG);
+    double *res = plan->d_scratch + 2 * (size_t)NORM_SCRATCH_WG;
+    return launch_norm(plan, d_c, bh, r1, r2, psum, pcnt, res, d_norm, (hipStream_t)stream);
 }
 
 // Synchronizes `stream` and reports constraint violations met since the last

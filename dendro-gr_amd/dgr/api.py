@@ -24,7 +24,7 @@ SYMBOLS = [
     "dgr_ham_constraint", "dgr_constraint_l2_norm", "dgr_rk4_steps", "dgr_plan_set_precision",
     "dgr_plan_autotune", "dgr_plan_tiling", "dgr_plan_set_tiling", "dgr_constraints",
     "dgr_plan_link_neighbors", "dgr_halo_fill", "dgr_plan_set_halo_refresh",
-    "dgr_padding_fill", "dgr_plan_fill_sources",
+    "dgr_constraint_l2_norm_noalloc", "dgr_padding_fill", "dgr_plan_fill_sources",
 ]
 PRECISION = {"fp64": 0, "mixed": 1}
 TILINGS = ("wt", "w", "full")  # DGR_TILING_WT, _W, _FULL
@@ -90,6 +90,8 @@ def load():
         L.dgr_plan_set_tiling.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
         L.dgr_constraint_l2_norm.argtypes = [vp, vp, C.POINTER(C.c_double), C.c_double, C.c_double,
                                              C.POINTER(C.c_double), vp]
+        L.dgr_constraint_l2_norm_noalloc.argtypes = [vp, vp, C.POINTER(C.c_double), C.c_double, C.c_double,
+                                                     vp, vp]
         _lib = L
     return _lib
 
@@ -233,6 +235,15 @@ class Plan:
         _check(load().dgr_constraint_l2_norm(self._h, C.c_void_p(c.data_ptr()), b, r1, r2, C.byref(res),
                                              _stream_ptr(stream)), "dgr_constraint_l2_norm")
         return res.value
+
+    def constraint_norm_noalloc(self, out, c, bh=((3.0, 0.0, 0.0), (-3.0, 0.0, 0.0)), r1: float = 5.0,
+                                r2: float = 5.0, stream=None):
+        """dgr_constraint_l2_norm_noalloc: the norm into the device double out[0],
+        asynchronously, with no allocation (needs link_neighbors() first)."""
+        b = (C.c_double * 6)(*[float(v) for p in bh for v in p])
+        _check(load().dgr_constraint_l2_norm_noalloc(self._h, C.c_void_p(c.data_ptr()), b, r1, r2,
+                                                     C.c_void_p(out.data_ptr()), _stream_ptr(stream)),
+               "dgr_constraint_l2_norm_noalloc")
 
     def status(self, stream=None) -> int:
         return int(load().dgr_plan_status(self._h, _stream_ptr(stream)))

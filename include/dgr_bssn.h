@@ -296,6 +296,26 @@ int dgr_constraints(dgr_plan *plan, double *d_ham, double *const *d_mom, double 
 int dgr_constraint_l2_norm(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2,
                            double *norm, void *stream);
 
+/* Largest plan dgr_constraint_l2_norm_noalloc serves: workgroups of 256
+ * interior points, counted per chunk of blocks (about 268 M interior points). */
+#define DGR_NOALLOC_NORM_MAX_WORKGROUPS 1048576
+
+/* dgr_constraint_l2_norm that allocates nothing and does not wait. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same kernels and same result as
+ * dgr_constraint_l2_norm, but asynchronous on `stream`: the norm (or -1) is
+ * written to d_norm[0], a caller-owned device double. The partial sums live in
+ * a fixed device array of the library (16 MiB per GPU, held until the process
+ * ends) instead of plan memory; every call on one GPU shares it, so calls
+ * that overlap on one GPU (other streams or host threads) race and must be
+ * ordered by the caller. Needs the plan's neighbour table to exist
+ * (dgr_plan_link_neighbors, or one call of dgr_constraint_l2_norm, at set-up);
+ * DGR_E_INVALID for plans of more than DGR_NOALLOC_NORM_MAX_WORKGROUPS norm
+ * workgroups or without the table. */
+int dgr_constraint_l2_norm_noalloc(dgr_plan *plan, const double *d_c, const double *bh, double r1, double r2,
+                                   double *d_norm, void *stream);
+
 
 /* Waits for `stream`, then reports and clears constraint violations met by
  * dgr_rk4_step / dgr_enforce_constraints since the last call: DGR_E_CONSTRAINT
