@@ -86,6 +86,17 @@ class Workload:
                          "one RK4 step per step")
             self.dt = 0.25 * 0.05
             self.params = dict(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0)
+        elif name == "refined":
+            # a 2:1-balanced octree of 13^3 blocks refined toward the punctures:
+            # RK4 with the padding of every stage refreshed across level jumps
+            self.blocks = W.refined_octree(4, 9, 4.0)
+            self.key = "refined4-9"
+            h = (self.blocks["pmax"][:, 0] - self.blocks["pmin"][:, 0]) / 12
+            self.desc = (f"2:1-balanced octree of {len(self.blocks)} blocks of 13^3 on six levels refined toward "
+                         f"two punctures (Brill-Lindquist), padding of every RK4 stage refreshed across level "
+                         f"jumps (copy, injection, order-6 interpolation), one RK4 step per step")
+            self.dt = 0.25 * float(h.min())
+            self.params = dict(ko_sigma=0.4)
         elif name == "puncture":
             self.blocks = W.puncture_octree(scale)
             self.key = f"puncture{scale:g}"
@@ -401,6 +412,39 @@ def halo_rate(wl, plan, state, stream, calls=20):
                     "runs once per RK4 call, leaving the caller's padding refreshed")}
 
 
+def fill_rate(wl, plan, state, stream, calls=10):
+    """dgr_padding_fill (the whole refresh: same-level copies, then the other
+    sizes and levels) alone: average time from HIP events on the launch
+    stream, the number of padding points each part writes, and the copy-rate
+    equivalent (every written point read once and written once, 24 variables
+    x 16 B)."""
+    import torch
+    nbr = plan.link_neighbors()
+    ncopy, ninterp = plan.fill_sources()
+    sz = wl.blocks["sz"].astype(np.int64)
+    linked = 0
+    for d in range(27):
+        dd = (d % 3 - 1, (d // 3) % 3 - 1, d // 9 - 1)
+        if not any(dd):
+            continue
+        rows = nbr[:, d] >= 0
+        ext = [np.full(int(rows.sum()), 3) if dd[a] else sz[rows, a] - 6 for a in range(3)]
+        linked += int((ext[0] * ext[1] * ext[2]).sum())
+    plan.padding_fill(state, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(calls):
+        plan.padding_fill(state, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / calls
+    pts = linked + ncopy + ninterp
+    return {"linked_points": linked, "copied_points": ncopy, "interpolated_points": ninterp,
+            "ms_per_call": round(ms, 4), "copy_equivalent_GBs": round(pts * 24 * 16 / ms / 1e6, 1),
+            "use": ("RK4 stages read same-level padding inside k_fused (HALO variant) and get the rest "
+                    "written before each stage by k_fill_general; this full fill runs once per RK4 call")}
+
+
 def constraint_rates(wl, plan, state, stream, calls=5):
     """The constraint diagnostic (dgr_constraints) on the state after the timed
     steps: interior points per second with all six outputs (ham, mom0..2,
@@ -428,8 +472,9 @@ def constraint_rates(wl, plan, state, stream, calls=5):
             res["all_outputs_ms_per_call"] = round(dt * 1e3, 4)
             byts = 24 * 8 * wl.n_pad + 6 * 8 * wl.npts
             res["all_outputs_algorithmic_gb_s"] = round(byts / dt / 1e9, 1)
-    res["note"] = ("derivatives staged through the split path's 162-slot workspace, then the "
-                   "generated constraint expressions per point; FP64")
+    res["note"] = ("one fused kernel per tile kind (k_cons_fused): the 17 variables the constraints "
+                   "differentiate stream through LDS, derivatives in registers, no workspace; all six "
+                   "outputs are computed whatever is asked for; FP64")
     del out
     return res
 
@@ -447,6 +492,8 @@ def run_fp64(wl, args, stream, dev, precision="fp64", keep_state0=True):
     plan.set_precision(precision)
     if wl.name == "lattice13":
         plan.set_halo_refresh(True)
+    if wl.name == "refined":
+        plan.set_halo_refresh("full")
     params = dgr.default_params(**wl.params)
     if not args.no_autotune:
         # tile shapes per block size, timed on this workload's own data
@@ -506,6 +553,16 @@ def run_fp64(wl, args, stream, dev, precision="fp64", keep_state0=True):
     res["roofline"] = roofline(wl, kt, 4 * ksteps, precision)
     res["constraints"] = constraint_rates(wl, plan, state, stream)
     res["halo_fill"] = halo_rate(wl, plan, state, stream) if wl.name == "lattice13" else None
+    if wl.name == "refined":
+        res["padding_fill"] = fill_rate(wl, plan, state, stream)
+        # the same steps with frozen padding, for the cost of the refresh
+        plan.set_halo_refresh(False)
+        plan.rk4_steps(state, wl.dt, params, nsteps=1, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.rk4_steps(state, wl.dt, params, nsteps=args.steps, stream=stream)
+        torch.cuda.synchronize()
+        res["frozen_padding_mpoints_s"] = round(wl.npts * 4 * args.steps / (time.perf_counter() - t0) / 1e6, 2)
     plan.close()
     del state
     torch.cuda.empty_cache()
@@ -572,7 +629,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["octree", "uniform13", "puncture", "lattice13"], default="octree",
+    ap.add_argument("--workload", choices=["octree", "uniform13", "puncture", "lattice13", "refined"], default="octree",
                     help="octree (default): the puncture octree (scale 0.42) as the headline, with its "
                          "mixed-precision leg, the 120 M-point octree and the uniform 13^3 batch beside it; "
                          "the others run one workload alone")
@@ -614,7 +671,7 @@ def main():
     stream = torch.cuda.current_stream()
     composite = args.workload == "octree"
     wl = Workload("puncture" if composite else args.workload, args.nblocks, args.scale)
-    if wl.name == "lattice13":  # the comparisons are for the frozen-padding step
+    if wl.name in ("lattice13", "refined"):  # the comparisons are for the frozen-padding step
         args.no_mixed = args.no_cpu_baseline = True
     want_mixed = args.precision == "fp64" and not args.no_mixed
     res, state0 = run_fp64(wl, args, stream, dev, args.precision, keep_state0=want_mixed)
@@ -629,7 +686,7 @@ def main():
     del state0
     torch.cuda.empty_cache()
 
-    uniform = large = None
+    uniform = large = refined = None
     if composite:
         # the uniform 13^3 batch beside the octree: its rate, roofline and mixed errors
         wu = Workload("uniform13", args.nblocks, args.scale)
@@ -642,6 +699,14 @@ def main():
             uniform["mixed_precision"] = mixed_comparison(wu, su, dgr.default_params(**wu.params), args.steps,
                                                           ru["value"], stream, not args.no_autotune)
         del su
+        torch.cuda.empty_cache()
+        # the refined octree: RK4 with the padding refreshed across level jumps
+        wr = Workload("refined", 0, 0.0)
+        rr, _ = run_fp64(wr, args, stream, dev, "fp64", keep_state0=False)
+        refined = {"workload": wr.desc, "blocks": len(wr.blocks), "interior_points": wr.npts,
+                   "value": rr["value"], "unit": "Mpoints/s", "ms_per_step": rr["ms_per_step"],
+                   "frozen_padding_mpoints_s": rr["frozen_padding_mpoints_s"],
+                   "padding_fill": rr["padding_fill"], "roofline": rr["roofline"]}
         torch.cuda.empty_cache()
         if not args.no_large:
             free, _ = torch.cuda.mem_get_info()
@@ -664,7 +729,7 @@ def main():
                        "dt": wl.dt, **wl.params},
             "rhs_only_mpoints_s": res["rhs_only_mpoints_s"], "constraints": res["constraints"],
             "halo_fill": res["halo_fill"], "roofline": res["roofline"], "cpu_baseline": cpu,
-            "mixed_precision": mixed, "uniform13": uniform, "large_octree": large}
+            "mixed_precision": mixed, "uniform13": uniform, "refined_octree": refined, "large_octree": large}
     print(json.dumps(line), flush=True)
 
 

@@ -1265,57 +1265,69 @@ struct FillRec {
 };
 constexpr uint32_t FILL_INTERP = 1u << 12;
 
-// Weight of node m of the 7-node order-6 element at t = code/2 (the Lagrange
-// basis, rounded as the oracle's orc_lagrange6_half).
-__device__ __forceinline__ double lag6_weight(int code, int m) {
+// The weights of the 7 nodes of an order-6 element at t = code/2, code 0..12
+// (the Lagrange basis), computed on the host in the oracle's operation order
+// (orc_lagrange6_half) and passed by value.
+struct Lag6Table {
+    double w[13][7];
+};
+
+// One interpolation record for variable array uv, odd (OX, OY, OZ) or even
+// code per axis. At a node of the coarse element (even code) the weights are
+// exactly 1 at that node and +-0 elsewhere, so only that tap is summed: the
+// partial sums are the same numbers the full 7-tap sums produce (adding a
+// signed zero leaves a finite sum unchanged), in the same order (x innermost,
+// then y, then z).
+template <bool OX, bool OY, bool OZ>
+__device__ __forceinline__ double fill_interp(const double *__restrict__ uv, const FillRec &r, const Lag6Table &tab,
+                                              int cx, int cy, int cz) {
 #pragma clang fp contract(off)
-    const double t = 0.5 * (double)code;
-    double num = 1.0, den = 1.0;
+    constexpr int NX = OX ? 7 : 1, NY = OY ? 7 : 1, NZ = OZ ? 7 : 1;
+    const double *base = uv + r.src + (OX ? 0 : (size_t)(cx >> 1)) + (OY ? 0 : (size_t)r.mx * (cy >> 1)) +
+                         (OZ ? 0 : (size_t)r.mxy * (cz >> 1));
+    double sz = 0.0;
 #pragma unroll
-    for (int l = 0; l < 7; l++) {
-        if (l == m) continue;
-        num *= (t - (double)l);
-        den *= (double)(m - l);
+    for (int c3 = 0; c3 < NZ; c3++) {
+        double sy = 0.0;
+#pragma unroll
+        for (int b2 = 0; b2 < NY; b2++) {
+            const double *row = base + (size_t)r.mx * b2 + (size_t)r.mxy * c3;
+            double sx = 0.0;
+#pragma unroll
+            for (int a1 = 0; a1 < NX; a1++) sx += (OX ? tab.w[cx][a1] : 1.0) * row[a1];
+            sy += (OY ? tab.w[cy][b2] : 1.0) * sx;
+        }
+        sz += (OZ ? tab.w[cz][c3] : 1.0) * sy;
     }
-    return num / den;
+    return sz;
 }
 
-__global__ __launch_bounds__(256) void k_fill_general(VarPtrs u, const FillRec *__restrict__ recs, int64_t n) {
-#pragma clang fp contract(off)
+// One thread per (record, variable): blockIdx.y is the variable. Records are
+// sorted (copies, then interpolations by which axes interpolate), so a
+// wavefront takes one path.
+__global__ __launch_bounds__(256) void k_fill_general(VarPtrs u, const FillRec *__restrict__ recs, int64_t n,
+                                                      const Lag6Table tab) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
+    double *__restrict__ uv = u.p[blockIdx.y];
     const FillRec r = recs[q];
     if (!(r.code & FILL_INTERP)) {
-        double t[NV];
-#pragma unroll
-        for (int v = 0; v < NV; v++) t[v] = u.p[v][r.src];
-#pragma unroll
-        for (int v = 0; v < NV; v++) u.p[v][r.dst] = t[v];
+        uv[r.dst] = uv[r.src];
         return;
     }
-    double wx[7], wy[7], wz[7];
-#pragma unroll
-    for (int m = 0; m < 7; m++) {
-        wx[m] = lag6_weight((int)(r.code & 15u), m);
-        wy[m] = lag6_weight((int)((r.code >> 4) & 15u), m);
-        wz[m] = lag6_weight((int)((r.code >> 8) & 15u), m);
+    const int cx = (int)(r.code & 15u), cy = (int)((r.code >> 4) & 15u), cz = (int)((r.code >> 8) & 15u);
+    double val;
+    switch ((cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2)) {
+        case 0: val = fill_interp<false, false, false>(uv, r, tab, cx, cy, cz); break;
+        case 1: val = fill_interp<true, false, false>(uv, r, tab, cx, cy, cz); break;
+        case 2: val = fill_interp<false, true, false>(uv, r, tab, cx, cy, cz); break;
+        case 3: val = fill_interp<true, true, false>(uv, r, tab, cx, cy, cz); break;
+        case 4: val = fill_interp<false, false, true>(uv, r, tab, cx, cy, cz); break;
+        case 5: val = fill_interp<true, false, true>(uv, r, tab, cx, cy, cz); break;
+        case 6: val = fill_interp<false, true, true>(uv, r, tab, cx, cy, cz); break;
+        default: val = fill_interp<true, true, true>(uv, r, tab, cx, cy, cz); break;
     }
-    for (int v = 0; v < NV; v++) {
-        const double *base = u.p[v] + r.src;
-        double sz = 0.0;
-        for (int c3 = 0; c3 < 7; c3++) {
-            double sy = 0.0;
-            for (int b2 = 0; b2 < 7; b2++) {
-                const double *row = base + (size_t)r.mx * b2 + (size_t)r.mxy * c3;
-                double sx = 0.0;
-#pragma unroll
-                for (int a1 = 0; a1 < 7; a1++) sx += wx[a1] * row[a1];
-                sy += wy[b2] * sx;
-            }
-            sz += wz[c3] * sy;
-        }
-        u.p[v][r.dst] = sz;
-    }
+    uv[r.dst] = val;
 }
 
 // -------------------------------------------------------------------------
@@ -2710,6 +2722,13 @@ int build_fill(dgr_plan *plan) {
                     }
         }
     }
+    // copies first, then the interpolations grouped by which axes interpolate
+    // (the parity of each axis' code): a wavefront then takes one path of
+    // k_fill_general; within a group the records keep their block order
+    auto cls = [](const FillRec &r) {
+        return (r.code & FILL_INTERP) ? 1 + (int)((r.code & 1u) | ((r.code >> 3) & 2u) | ((r.code >> 6) & 4u)) : 0;
+    };
+    std::stable_sort(recs.begin(), recs.end(), [&](const FillRec &a, const FillRec &b) { return cls(a) < cls(b); });
     if (plan->rk4_exec) {  // a captured step may hold the old records
         HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
         plan->rk4_exec = nullptr;
@@ -2728,11 +2747,33 @@ int build_fill(dgr_plan *plan) {
     return DGR_OK;
 }
 
+// Lagrange weights of the order-6 element at t = code/2, in the operation
+// order of the oracle's orc_lagrange6_half (so the bits are the oracle's).
+Lag6Table lag6_table() {
+#pragma clang fp contract(off)
+    Lag6Table tab;
+    for (int code = 0; code < 13; code++) {
+        const double t = 0.5 * (double)code;
+        for (int m = 0; m < 7; m++) {
+            double num = 1.0, den = 1.0;
+            for (int l = 0; l < 7; l++) {
+                if (l == m) continue;
+                num *= (t - (double)l);
+                den *= (double)(m - l);
+            }
+            tab.w[code][m] = num / den;
+        }
+    }
+    return tab;
+}
+
 int launch_fill_general(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
     if (plan->nfill <= 0) return DGR_OK;
     const int64_t nwg = (plan->nfill + 255) / 256;
     if (nwg > INT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many records");
-    hipLaunchKernelGGL(k_fill_general, dim3((uint32_t)nwg), dim3(256), 0, st, u, plan->d_fill, plan->nfill);
+    static const Lag6Table tab = lag6_table();
+    hipLaunchKernelGGL(k_fill_general, dim3((uint32_t)nwg, NV), dim3(256), 0, st, u, plan->d_fill, plan->nfill,
+                       tab);
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }

@@ -6,4 +6,4 @@ computation runs in the HIP library lib/libdgr_bssn.so, which must be built
 from .api import (  # noqa: F401
     BssnParams, Plan, NVARS, VAR_NAMES, default_params, lib_path, load, var_pointers,
 )
-from . import parfile, workloads  # noqa: F401
+from . import checkpoint, parfile, workloads  # noqa: F401

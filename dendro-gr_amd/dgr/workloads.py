@@ -141,6 +141,68 @@ def mixed_size_lattice(h: float = 0.05, origin=(-0.43, -0.31, -0.19)) -> np.ndar
     return _table(sizes, pmins, hs)
 
 
+def refined_octree(base: int = 4, max_level: int = 9, shell: float = 4.0, half_width: float = 16.0,
+                   punctures=((3.0, 0.0, 0.0), (-3.0, 0.0, 0.0))) -> np.ndarray:
+    """A 2:1-balanced octree over [-half_width, half_width]^3 refined toward
+    the punctures, one 13^3 block per leaf octant (its 7^3 interior nodes are
+    the octant's order-6 element; neighbours share their boundary nodes).
+    Every octant of level `base` is a leaf to start with; an octant of level l
+    < max_level is split into 8 when its distance to the nearest puncture is
+    below shell x its edge; then octants are split until every leaf's face,
+    edge and corner neighbours are at most one level coarser (2:1 balance).
+    Padding across a level jump lies in a coarser or finer neighbour: the
+    workload of the full padding refresh. Blocks are in Morton order of their
+    octants (Dendro's block order along its space-filling curve)."""
+    import itertools
+    P = np.asarray(punctures, dtype=np.float64)
+    size = lambda lev: 2.0 * half_width / 2 ** lev
+    leaves = {(base, i, j, k) for i in range(2 ** base) for j in range(2 ** base) for k in range(2 ** base)}
+
+    def dist(key):
+        lev, i, j, k = key
+        lo = np.array([i, j, k], dtype=np.float64) * size(lev) - half_width
+        d = np.maximum(0.0, np.maximum(lo - P, P - (lo + size(lev))))
+        return float(np.sqrt((d * d).sum(axis=1)).min())
+
+    def split(key):
+        leaves.remove(key)
+        lev, i, j, k = key
+        for a, b, c in itertools.product((0, 1), repeat=3):
+            leaves.add((lev + 1, 2 * i + a, 2 * j + b, 2 * k + c))
+
+    for lev in range(base, max_level):
+        for key in [q for q in leaves if q[0] == lev and dist(q) < shell * size(lev)]:
+            split(key)
+    dirs = [d for d in itertools.product((-1, 0, 1), repeat=3) if d != (0, 0, 0)]
+    while True:
+        mark = set()
+        for lev, i, j, k in leaves:
+            n = 2 ** lev
+            for dx, dy, dz in dirs:
+                x, y, z = i + dx, j + dy, k + dz
+                if not (0 <= x < n and 0 <= y < n and 0 <= z < n):
+                    continue
+                for up in range(lev + 1):
+                    key = (lev - up, x >> up, y >> up, z >> up)
+                    if key in leaves:
+                        if up >= 2:
+                            mark.add(key)
+                        break
+        if not mark:
+            break
+        for key in mark:
+            split(key)
+    top = max(q[0] for q in leaves)
+    order = sorted(leaves, key=lambda q: _morton(q[1] << (top - q[0]), q[2] << (top - q[0]), q[3] << (top - q[0])))
+    sizes, pmins, hs = [], [], []
+    for lev, i, j, k in order:
+        h = size(lev) / 6.0
+        sizes.append(13)
+        hs.append(h)
+        pmins.append(np.array([i, j, k], dtype=np.float64) * size(lev) - half_width - PW * h)
+    return _table(sizes, pmins, hs)
+
+
 def total_padded(t: np.ndarray) -> int:
     """Length each variable array needs (max offset + block size)."""
     if len(t) == 0:

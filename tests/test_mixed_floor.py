@@ -99,3 +99,33 @@ def test_variables_change_by_several_times_their_size_per_step(block):
     umax = np.abs(inp[:, :24]).max(axis=0)
     growth = 0.025 * np.abs(r64).max(axis=0) / umax
     assert growth.max() > 5.0  # K, Gt, B, At: dt |RHS| / |u| up to ~9
+
+
+def test_puncture_noise_floor_exceeds_1e8_per_variable():
+    """The octree data (Brill-Lindquist background, 1e-6 white noise on K,
+    Gt, beta, B, At): the noise is not smooth, so no offset or low-order fit
+    subtracted per tile shrinks it. Rounding to FP32 only the derivative values
+    of the variables other than alpha and chi (which the mixed kernels stage
+    and differentiate in FP64) already moves the state one step later by more
+    than 1e-8 of its size per variable (Gt, B and the off-diagonal gt, ~4e-8:
+    their right-hand sides take derivatives of the noise)."""
+    t = W.puncture_octree(0.42)
+    b = int(np.nonzero(t["sz"][:, 0] == 19)[0][0])
+    one = t[b:b + 1].copy()
+    one["offset"][0] = 0
+    st = np.zeros((24, W.total_padded(one)))
+    O.init_brill_lindquist(st, one, seed=5, noise=1e-6)
+    h = float((one["pmax"][0, 0] - one["pmin"][0, 0]) / 18)
+    inp = point_inputs(st, one[0], eta=2.0)
+    r64 = algebra(inp)
+    dt = 0.25 * h
+    x = inp.copy()
+    keep = set(range(24, 30)) | set(range(96, 108))  # alpha's and chi's first and second derivatives
+    cols = [c for c in range(24, 162) if c not in keep]
+    x[:, cols] = x[:, cols].astype(np.float32).astype(np.float64)
+    # per variable against the size of the state after one (Euler) step, for
+    # the variables that are not identically zero
+    after = np.abs(inp[:, :24] + dt * r64).max(axis=0)
+    live = after > 0
+    est = (np.abs(algebra(x) - r64).max(axis=0)[live] * dt / after[live]).max()
+    assert est > 1e-8, est

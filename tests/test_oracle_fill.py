@@ -156,3 +156,49 @@ def test_same_size_fill_unchanged_without_general_sources():
     O.halo_fill(a, t, nbr)
     O.padding_fill(b, t, nbr, general=True)
     assert np.array_equal(a, b)
+
+
+def test_refined_octree_is_balanced_and_shares_nodes():
+    """The refined-octree workload: every pair of blocks whose interiors touch
+    (face, edge or corner) differs by at most one level, the leaves tile the
+    domain, and same-level neighbours share their boundary nodes (the link
+    table finds them)."""
+    t = W.refined_octree(2, 4, 1.0)
+    h = (t["pmax"][:, 0] - t["pmin"][:, 0]) / 12
+    lo = t["pmin"] + 3 * h[:, None]
+    hi = lo + 6 * h[:, None]
+    assert abs(float((6 * h) ** 3 @ np.ones(len(t))) - 32.0 ** 3) < 1e-9  # leaves tile [-16, 16]^3
+    touch = np.all((lo[:, None, :] <= hi[None, :, :] + 1e-9) & (lo[None, :, :] <= hi[:, None, :] + 1e-9), axis=2)
+    np.fill_diagonal(touch, False)
+    ratio = h[:, None] / h[None, :]
+    assert np.all(np.isclose(ratio[touch], 0.5) | np.isclose(ratio[touch], 1.0) | np.isclose(ratio[touch], 2.0))
+    assert (np.isclose(ratio[touch], 2.0)).any()  # level jumps exist
+    nbr = W.neighbors(t)
+    assert (nbr >= 0).sum() > 0
+
+
+def test_refined_octree_fill_reproduces_degree_6_polynomials():
+    """On the refined octree (three levels) the general fill reproduces a
+    polynomial of degree <= 6 at every padding point inside another block,
+    and leaves the padding outside the domain untouched."""
+    t = W.refined_octree(2, 4, 1.0)
+    nbr = W.neighbors(t)
+    f = lambda x, y, z: poly6(x / 16.0, y / 16.0, z / 16.0)  # noqa: E731
+    st = state_from(t, f)
+    O.padding_fill(st, t, nbr, general=True)
+    worst, nfill = 0.0, 0
+    for b, blk in enumerate(t):
+        n = int(blk["sz"][0])
+        off = int(blk["offset"])
+        got = st[0, off:off + n ** 3].reshape(n, n, n)
+        x, y, z = coords(blk)
+        filled = ~np.isnan(got)
+        filled[3:-3, 3:-3, 3:-3] = False
+        inside = (np.abs(x) <= 16 + 1e-9) & (np.abs(y) <= 16 + 1e-9) & (np.abs(z) <= 16 + 1e-9)
+        pad = np.ones((n, n, n), bool)
+        pad[3:-3, 3:-3, 3:-3] = False
+        assert np.array_equal(filled, pad & inside), b  # all in-domain padding has a source
+        if filled.any():
+            worst = max(worst, np.abs(got[filled] - f(x, y, z)[filled]).max())
+            nfill += int(filled.sum())
+    assert nfill > 0 and worst < 1e-12, worst
