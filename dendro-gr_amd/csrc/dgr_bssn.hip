@@ -619,8 +619,11 @@ struct ConsPhases {
 // T: the arithmetic type of derivatives, equations and KO (double, or float
 // for the mixed-precision path, which stages each variable in LDS with a
 // per-tile offset subtracted: the value at the region's first point).
-// P: the phase table.
-template <bool FACE, class C, typename T, class P = RhsPhases>
+// P: the phase table. HALO: region offsets index the whole variable (the
+// in-kernel padding refresh reads neighbours); otherwise they are byte
+// offsets within the block, so a staged load is a uniform base plus a 32-bit
+// offset (one address register, no 64-bit address arithmetic per load).
+template <bool FACE, class C, typename T, class P = RhsPhases, bool HALO = false>
 struct FusedCtx {
     static constexpr int FT = C::FT, NREG = C::NREG, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT, PD = C::PD;
     const StageIO &io;
@@ -690,8 +693,18 @@ struct FusedCtx {
         double *dst = pre[K % PD];
         const double *__restrict__ sb = io.src.p[v] + boff;
         if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = sb[corner];
+        if constexpr (HALO) {
 #pragma unroll
-        for (int c = 0; c < PREF; c++) dst[c] = sb[goff[c]];
+            for (int c = 0; c < PREF; c++) dst[c] = sb[goff[c]];
+        } else {
+            // raw buffer loads: the block's base in a resource (scalar registers),
+            // the 32-bit byte offset as the only per-lane address operand
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(sb), 0, 0x7fffffff, 0x00020000);
+#pragma unroll
+            for (int c = 0; c < PREF; c++)
+                dst[c] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, goff[c], 0, 0));
+        }
     }
 
     // The type variable V is staged and differentiated in. The mixed path
@@ -895,11 +908,11 @@ struct FusedCtx {
 // The equations' outputs: on a flagged face the boundary value (already
 // holding its KO term), elsewhere equations + KO term, kept in LDS until the
 // epilogue.
-template <bool FACE, class C, typename T>
+template <bool FACE, class C, typename T, bool HALO>
 struct FusedOut {
     static constexpr int FT = C::FT;
-    const FusedCtx<FACE, C, T> &ctx;
-    __device__ explicit FusedOut(const FusedCtx<FACE, C, T> &c) : ctx(c) {}
+    const FusedCtx<FACE, C, T, RhsPhases, HALO> &ctx;
+    __device__ explicit FusedOut(const FusedCtx<FACE, C, T, RhsPhases, HALO> &c) : ctx(c) {}
     template <int V>
     __device__ __forceinline__ void put(T val) {
 #pragma clang fp contract(off)
@@ -970,7 +983,7 @@ __device__ __forceinline__ void fused_epilogue(const StageIO &io, const T *s_ko,
 // D_x / D_y slots (common to the fused kernels): idx is the point's padded
 // index, q its interior index, active false for shadow lanes.
 template <bool FACE, class C, typename T, bool HALO, class P>
-__device__ __forceinline__ void tile_setup(FusedCtx<FACE, C, T, P> &ctx, const StageIO &io, const Tile &tl,
+__device__ __forceinline__ void tile_setup(FusedCtx<FACE, C, T, P, HALO> &ctx, const StageIO &io, const Tile &tl,
                                            const DBlock &B, const uint16_t *__restrict__ tabs, int dtab,
                                            uint64_t &idx_out, int64_t &q_out, bool &active_out) {
     constexpr int FT = C::FT, PREF = C::PREF, DXIT = C::DXIT, DYIT = C::DYIT;
@@ -1044,7 +1057,7 @@ __device__ __forceinline__ void tile_setup(FusedCtx<FACE, C, T, P> &ctx, const S
             const int jj = r / FRX, ii = r - jj * FRX;
             const bool in_region = ii < ex + 2 * PW && jj < ey + 2 * PW && kk < ez + 2 * PW;
             const int gi = ctx.ox - PW + ii, gj = ctx.oy - PW + jj, gk = ctx.oz - PW + kk;
-            ctx.goff[c] = in_region ? (uint32_t)(gi + nx * (gj + ny * gk)) : 0u;
+            ctx.goff[c] = in_region ? (uint32_t)(gi + nx * (gj + ny * gk)) * (uint32_t)sizeof(double) : 0u;
         }
     }
     // where this thread's D_x / D_y evaluations go (the config's slot table)
@@ -1061,7 +1074,7 @@ template <bool FACE, class C, typename T, bool HALO = false>
 __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &tl, const DBlock &B,
                                            const dgr::AxisConsts *K, const uint16_t *__restrict__ tabs,
                                            int dtab, double *s_tile, double *s_dx, double *s_dy, T *s_ko) {
-    FusedCtx<FACE, C, T> ctx(io);
+    FusedCtx<FACE, C, T, RhsPhases, HALO> ctx(io);
     ctx.K = K;
     ctx.load_consts();
     ctx.s_tile = s_tile;
@@ -1094,7 +1107,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     if constexpr (C::PD > 2) ctx.template fetch<2>();
     static_assert(C::PD <= 3, "prefetch distance");
     ctx.begin();
-    FusedOut<FACE, C, T> out(ctx);
+    FusedOut<FACE, C, T, HALO> out(ctx);
     dgr::bssn_rhs_phased<T>(ctx, eta, lam, lamf, out);
     if (active) fused_epilogue<C, T>(io, s_ko, ctx.f, ctx.tid, idx, q, prm.chi_floor);
 }
