@@ -1229,9 +1229,10 @@ __global__ __launch_bounds__(256) void k_pad_copy(VarPtrs dst, VarPtrs src, cons
 // arithmetic is done once for all 24 variables, whose loads are issued
 // together before the stores.
 __global__ __launch_bounds__(256) void k_halo_fill(VarPtrs u, const DBlock *__restrict__ blocks,
-                                                   const int *__restrict__ nbr, int nblocks) {
+                                                   const int *__restrict__ nbr, const int2 *__restrict__ wgs) {
     __shared__ int64_t s_off[27];
-    const int b = blockIdx.x % nblocks, chunk = blockIdx.x / nblocks;
+    const int2 w = wgs[blockIdx.x];  // (block, chunk of 256 points)
+    const int b = w.x, chunk = w.y;
     const DBlock &B = blocks[b];
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const int n = nx * ny * nz;
@@ -1631,6 +1632,8 @@ struct dgr_plan {
     int *d_nbr = nullptr;       // its device copy
     int64_t *d_nbo = nullptr;   // the same as neighbour block offsets (-1 for none), for k_fused
     int64_t nlinks = -1;        // links in the table (-1: not built yet)
+    int2 *d_halo_wg = nullptr;  // k_halo_fill's workgroups: (block, chunk) of the linked blocks
+    int64_t nhalo_wg = 0;
     bool halo = false;          // dgr_rk4_step(s): refresh padding from neighbours before each stage
     bool fill_general = false;  // ... and from blocks of other sizes or levels (k_fill_general)
     FillRec *d_fill = nullptr;  // general fill records (built on first use)
@@ -2086,6 +2089,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     (void)hipFree(plan->d_nbr);
     (void)hipFree(plan->d_nbo);
     (void)hipFree(plan->d_fill);
+    (void)hipFree(plan->d_halo_wg);
     (void)hipFree(plan->d_err);
     (void)hipFree(plan->d_norm);
     if (plan->rk4_exec) (void)hipGraphExecDestroy(plan->rk4_exec);
@@ -2564,6 +2568,37 @@ int link_neighbors(dgr_plan *plan) {
     if (nb) HIP_OK(hipMemcpy(plan->d_nbo, off.data(), sizeof(int64_t) * 27 * nb, hipMemcpyHostToDevice));
     plan->nlinks = links;
     plan->nfill = -1;  // the general fill records follow the links
+    // k_halo_fill's workgroups: (block, chunk of 256 points) for the blocks
+    // with at least one link, chunk-major (consecutive workgroups take
+    // different blocks)
+    std::vector<int2> wgs;
+    int maxch = 0;
+    std::vector<int> nch(nb, 0);
+    for (size_t b = 0; b < nb; b++) {
+        bool linked = false;
+        for (int d = 0; d < 27 && !linked; d++) linked = plan->hnbr[27 * b + d] >= 0;
+        const DBlock &B = plan->hblocks[b];
+        if (linked) nch[b] = (int)(((int64_t)B.n[0] * B.n[1] * B.n[2] + 255) / 256);
+        maxch = std::max(maxch, nch[b]);
+    }
+    for (int c = 0; c < maxch; c++)
+        for (size_t b = 0; b < nb; b++)
+            if (c < nch[b]) wgs.push_back(make_int2((int)b, c));
+    if (wgs.size() > (size_t)INT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many blocks x points");
+    if (plan->rk4_exec) {  // a captured step may hold the old table
+        HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
+        plan->rk4_exec = nullptr;
+    }
+    if (plan->d_halo_wg) {
+        HIP_OK(hipFree(plan->d_halo_wg));
+        plan->d_halo_wg = nullptr;
+    }
+    if (!wgs.empty()) {
+        hipError_t e = hipMalloc(&plan->d_halo_wg, sizeof(int2) * wgs.size());
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "padding fill table");
+        HIP_OK(hipMemcpy(plan->d_halo_wg, wgs.data(), sizeof(int2) * wgs.size(), hipMemcpyHostToDevice));
+    }
+    plan->nhalo_wg = (int64_t)wgs.size();
     return DGR_OK;
 }
 
@@ -2792,14 +2827,9 @@ int launch_fill_general(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
 }
 
 int launch_halo_fill(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
-    const uint32_t nb = (uint32_t)plan->hblocks.size();
-    if (!nb || plan->nlinks == 0) return DGR_OK;
-    int64_t maxn = 0;
-    for (const DBlock &d : plan->hblocks) maxn = std::max<int64_t>(maxn, (int64_t)d.n[0] * d.n[1] * d.n[2]);
-    const int64_t nch = (maxn + 255) / 256;
-    if ((int64_t)nb * nch > INT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many blocks x points");
-    hipLaunchKernelGGL(k_halo_fill, dim3((uint32_t)(nb * nch)), dim3(256), 0, st, u, plan->d_blocks, plan->d_nbr,
-                       (int)nb);
+    if (plan->nhalo_wg <= 0) return DGR_OK;
+    hipLaunchKernelGGL(k_halo_fill, dim3((uint32_t)plan->nhalo_wg), dim3(256), 0, st, u, plan->d_blocks, plan->d_nbr,
+                       plan->d_halo_wg);
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }

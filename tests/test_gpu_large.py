@@ -121,8 +121,10 @@ def test_large_octree_puncture_step_matches_oracle(octree):
             assert inc_err.max() < 5e-11, inc_err
             assert scale_err < 1e-14, scale_err
         else:
-            # FP32 arithmetic: within 3e-8 of the state's scale (measured ~1e-8)
-            assert scale_err < 3e-8, scale_err
+            # FP32 arithmetic: measured 8.6e-7 normwise per variable on the
+            # increments and 4.8e-11 of the state's scale
+            assert inc_err.max() < 3e-6, inc_err
+            assert scale_err < 2e-10, scale_err
         assert np.array_equal(got[:, ~m], host0[:, ~m])  # padding untouched
     plan.close()
     del state

@@ -42,6 +42,10 @@ CASES = {
                           dict(ko_sigma=0.4, eta_central=2.0, eta_outer=2.0), 0.025),
     "puncture": lambda: (uneven_sample(), dict(init="puncture", seed=5), dict(ko_sigma=0.4), None),
 }
+# bounds at 2-3x the measured errors (RHS normwise per variable, state after one
+# step normwise per variable, state against the state's scale); measured:
+# minkowski 6.1e-7, 2.8e-7, 9.1e-9; puncture 1.3e-6, 1.3e-6, 1.1e-10
+BOUNDS = {"minkowski": (1.5e-6, 7e-7, 2e-8), "puncture": (3e-6, 3e-6, 3e-10)}
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -77,11 +81,10 @@ def test_mixed_rhs_and_rk4_step_against_fp64(case):
         f"RK4 state per-variable max {e_u.max():.3e} "
         f"({[(dgr.api.VAR_NAMES[v], f'{e_u[v]:.2e}', f'|u|max {mag[v]:.1e}') for v in wu]}), "
         f"against the state scale {e_scale:.3e}")
-    # measured: RHS <= 1.3e-6 normwise per variable (FP32 arithmetic, alpha and
-    # chi differentiated in FP64); state after one step 1.0e-8 (perturbed
-    # Minkowski) and 1.1e-10 (punctures) of the state scale
-    assert e_rhs.max() < 5e-6
-    assert e_scale < 3e-8
+    b_rhs, b_u, b_scale = BOUNDS[case]
+    assert e_rhs.max() < b_rhs
+    assert e_u.max() < b_u
+    assert e_scale < b_scale
 
 
 def test_mixed_constraint_norm_after_ten_steps():
@@ -105,7 +108,7 @@ def test_mixed_constraint_norm_after_ten_steps():
         plan.close()
     rel = abs(norms["mixed"] - norms["fp64"]) / norms["fp64"]
     log(f"ham norm after 10 steps: fp64 {norms['fp64']:.9e} mixed {norms['mixed']:.9e} rel {rel:.3e}")
-    assert rel < 1e-5  # 3 digits asked; measured ~1e-7
+    assert rel < 3e-7  # 3 digits asked; measured 1.1e-7
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])

@@ -89,7 +89,7 @@ def test_mixed_tilings_stay_within_error():
         got = interior_values(rhs(plan, u, p), t)
         e = normwise(got, ref).max()
         log(f"mixed tiling {tiling}: RHS normwise vs FP64 {e:.3e}")
-        assert e < 5e-6
+        assert e < 3e-6  # measured 1.2e-6
     plan.close()
 
 
