@@ -1344,6 +1344,92 @@ __global__ __launch_bounds__(256) void k_fill_general(VarPtrs u, const FillRec *
     uv[r.dst] = val;
 }
 
+// Interpolation records that read one coarse element (at most 64, records
+// [first, first + count)), and the element's first node and strides.
+struct FillGroup {
+    uint64_t src;
+    uint32_t mx, mxy;
+    uint32_t first, count;
+};
+
+// The sums of fill_interp on the element staged in LDS (node (a, b, c) at
+// a + 7 b + 49 c): the same products and additions in the same order.
+template <bool OX, bool OY, bool OZ>
+__device__ __forceinline__ double fill_interp_lds(const double *s_e, const Lag6Table &tab, int cx, int cy, int cz) {
+#pragma clang fp contract(off)
+    constexpr int NX = OX ? 7 : 1, NY = OY ? 7 : 1, NZ = OZ ? 7 : 1;
+    const double *base = s_e + (OX ? 0 : (cx >> 1)) + (OY ? 0 : 7 * (cy >> 1)) + (OZ ? 0 : 49 * (cz >> 1));
+    double sz = 0.0;
+#pragma unroll
+    for (int c3 = 0; c3 < NZ; c3++) {
+        double sy = 0.0;
+#pragma unroll
+        for (int b2 = 0; b2 < NY; b2++) {
+            const double *row = base + 7 * b2 + 49 * c3;
+            double sx = 0.0;
+#pragma unroll
+            for (int a1 = 0; a1 < NX; a1++) sx += (OX ? tab.w[cx][a1] : 1.0) * row[a1];
+            sy += (OY ? tab.w[cy][b2] : 1.0) * sx;
+        }
+        sz += (OZ ? tab.w[cz][c3] : 1.0) * sy;
+    }
+    return sz;
+}
+
+// One workgroup (one wave) per group of interpolation records sharing a
+// coarse element: the lanes hold their record for all 24 variables; per
+// variable the element's 343 nodes are staged in LDS (the next variable's
+// loads in flight while this one is evaluated), then every lane evaluates its
+// record from there. Records are sorted by class within a group, so lanes
+// mostly take one path.
+constexpr int FILL_EPL = (343 + 63) / 64;  // element nodes each lane stages
+
+__global__ __launch_bounds__(64) void k_fill_interp(VarPtrs u, const FillRec *__restrict__ recs,
+                                                    const FillGroup *__restrict__ groups, const Lag6Table tab) {
+    __shared__ double s_e[2][343];
+    const FillGroup g = groups[blockIdx.x];
+    const int tid = threadIdx.x;
+    const bool has = tid < (int)g.count;
+    FillRec r{};
+    if (has) r = recs[g.first + tid];
+    const int cx = (int)(r.code & 15u), cy = (int)((r.code >> 4) & 15u), cz = (int)((r.code >> 8) & 15u);
+    const int cls = (cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2);
+    size_t off[FILL_EPL];
+#pragma unroll
+    for (int k = 0; k < FILL_EPL; k++) {
+        const int e = min(tid + 64 * k, 342);
+        off[k] = g.src + (size_t)(e % 7) + (size_t)g.mx * ((e / 7) % 7) + (size_t)g.mxy * (e / 49);
+    }
+    double pre[FILL_EPL];
+#pragma unroll
+    for (int k = 0; k < FILL_EPL; k++) pre[k] = u.p[0][off[k]];
+    for (int v = 0; v < NV; v++) {
+        double *e = s_e[v & 1];
+#pragma unroll
+        for (int k = 0; k < FILL_EPL; k++)
+            if (tid + 64 * k < 343) e[tid + 64 * k] = pre[k];
+        __syncthreads();
+        if (v + 1 < NV) {
+#pragma unroll
+            for (int k = 0; k < FILL_EPL; k++) pre[k] = u.p[v + 1][off[k]];
+        }
+        if (has) {
+            double val;
+            switch (cls) {
+                case 0: val = fill_interp_lds<false, false, false>(e, tab, cx, cy, cz); break;
+                case 1: val = fill_interp_lds<true, false, false>(e, tab, cx, cy, cz); break;
+                case 2: val = fill_interp_lds<false, true, false>(e, tab, cx, cy, cz); break;
+                case 3: val = fill_interp_lds<true, true, false>(e, tab, cx, cy, cz); break;
+                case 4: val = fill_interp_lds<false, false, true>(e, tab, cx, cy, cz); break;
+                case 5: val = fill_interp_lds<true, false, true>(e, tab, cx, cy, cz); break;
+                case 6: val = fill_interp_lds<false, true, true>(e, tab, cx, cy, cz); break;
+                default: val = fill_interp_lds<true, true, true>(e, tab, cx, cy, cz); break;
+            }
+            u.p[v][r.dst] = val;
+        }
+    }
+}
+
 // -------------------------------------------------------------------------
 // Hamiltonian constraint (C_HAM of physical_constraints, physcon.cpp:12-89):
 // the generated constraint at every interior point of a chunk, from the
@@ -1638,7 +1724,9 @@ struct dgr_plan {
     bool fill_general = false;  // ... and from blocks of other sizes or levels (k_fill_general)
     FillRec *d_fill = nullptr;  // general fill records (built on first use)
     int64_t nfill = -1;         // records (-1: not built yet)
-    int64_t nfill_interp = 0;   // of which interpolations
+    int64_t nfill_interp = 0;   // of which interpolations (they follow the copies)
+    FillGroup *d_fill_groups = nullptr;  // the interpolations grouped by coarse element
+    int64_t nfill_groups = 0;
     double *d_istage = nullptr; // k_fused RK4: 2 * 24 * interior (u0 copy, accumulator)
     double *d_wstate = nullptr; // k_fused RK4: 24 * szpdof, the padded work state W
     int *d_err = nullptr;
@@ -2089,6 +2177,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     (void)hipFree(plan->d_nbr);
     (void)hipFree(plan->d_nbo);
     (void)hipFree(plan->d_fill);
+    (void)hipFree(plan->d_fill_groups);
     (void)hipFree(plan->d_halo_wg);
     (void)hipFree(plan->d_err);
     (void)hipFree(plan->d_norm);
@@ -2776,7 +2865,24 @@ int build_fill(dgr_plan *plan) {
     auto cls = [](const FillRec &r) {
         return (r.code & FILL_INTERP) ? 1 + (int)((r.code & 1u) | ((r.code >> 3) & 2u) | ((r.code >> 6) & 4u)) : 0;
     };
-    std::stable_sort(recs.begin(), recs.end(), [&](const FillRec &a, const FillRec &b) { return cls(a) < cls(b); });
+    // copies first; the interpolations by coarse element (first node), then
+    // by class, cut into groups of at most 64 that share the element
+    std::stable_sort(recs.begin(), recs.end(), [&](const FillRec &a, const FillRec &b) {
+        const bool ia = cls(a) > 0, ib = cls(b) > 0;
+        if (ia != ib) return ib;
+        if (!ia) return false;
+        if (a.src != b.src) return a.src < b.src;
+        return cls(a) < cls(b);
+    });
+    std::vector<FillGroup> groups;
+    for (size_t k = 0; k < recs.size(); k++) {
+        if (!cls(recs[k])) continue;
+        if (groups.empty() || groups.back().src != recs[k].src || groups.back().count == 64) {
+            groups.push_back(FillGroup{recs[k].src, recs[k].mx, recs[k].mxy, (uint32_t)k, 0});
+            if (k > UINT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many records");
+        }
+        groups.back().count++;
+    }
     if (plan->rk4_exec) {  // a captured step may hold the old records
         HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
         plan->rk4_exec = nullptr;
@@ -2790,6 +2896,16 @@ int build_fill(dgr_plan *plan) {
         if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "padding fill records");
         HIP_OK(hipMemcpy(plan->d_fill, recs.data(), sizeof(FillRec) * recs.size(), hipMemcpyHostToDevice));
     }
+    if (plan->d_fill_groups) {
+        HIP_OK(hipFree(plan->d_fill_groups));
+        plan->d_fill_groups = nullptr;
+    }
+    if (!groups.empty()) {
+        hipError_t e = hipMalloc(&plan->d_fill_groups, sizeof(FillGroup) * groups.size());
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "padding fill groups");
+        HIP_OK(hipMemcpy(plan->d_fill_groups, groups.data(), sizeof(FillGroup) * groups.size(), hipMemcpyHostToDevice));
+    }
+    plan->nfill_groups = (int64_t)groups.size();
     plan->nfill = (int64_t)recs.size();
     plan->nfill_interp = ninterp;
     return DGR_OK;
@@ -2817,11 +2933,15 @@ Lag6Table lag6_table() {
 
 int launch_fill_general(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
     if (plan->nfill <= 0) return DGR_OK;
-    const int64_t nwg = (plan->nfill + 255) / 256;
-    if (nwg > INT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many records");
     static const Lag6Table tab = lag6_table();
-    hipLaunchKernelGGL(k_fill_general, dim3((uint32_t)nwg, NV), dim3(256), 0, st, u, plan->d_fill, plan->nfill,
-                       tab);
+    const int64_t ncopy = plan->nfill - plan->nfill_interp;  // the copies come first
+    const int64_t nwg = (ncopy + 255) / 256;
+    if (nwg > INT32_MAX || plan->nfill_groups > INT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many records");
+    if (ncopy)
+        hipLaunchKernelGGL(k_fill_general, dim3((uint32_t)nwg, NV), dim3(256), 0, st, u, plan->d_fill, ncopy, tab);
+    if (plan->nfill_groups)
+        hipLaunchKernelGGL(k_fill_interp, dim3((uint32_t)plan->nfill_groups), dim3(64), 0, st, u, plan->d_fill,
+                           plan->d_fill_groups, tab);
     HIP_OK(hipGetLastError());
     return DGR_OK;
 }
