@@ -1896,6 +1896,14 @@ void group_tiles(std::vector<std::vector<Tile>> *per, std::vector<Tile> &out, in
         std::vector<size_t> bl;
         for (size_t b = 0; b < per[g].size(); b++)
             if (!per[g][b].empty()) bl.push_back(b);
+        // blocks with the same number of tiles together, most tiles first: a
+        // group of 8 then interleaves its tiles to the end, so every tile of
+        // a block lands on the same XCD (workgroups are dealt to the 8 XCDs
+        // round-robin) and its neighbours' halos are in that XCD's L2; in a
+        // group of mixed sizes the large block's remaining tiles would be
+        // dealt over all 8 XCDs
+        std::stable_sort(bl.begin(), bl.end(),
+                         [&](size_t a, size_t b) { return per[g][a].size() > per[g][b].size(); });
         for (size_t i0 = 0; i0 < bl.size(); i0 += 8) {
             const size_t i1 = std::min(bl.size(), i0 + 8);
             size_t mx = 0;
