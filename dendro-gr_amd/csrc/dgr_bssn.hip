@@ -693,18 +693,15 @@ struct FusedCtx {
         double *dst = pre[K % PD];
         const double *__restrict__ sb = io.src.p[v] + boff;
         if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = sb[corner];
-        if constexpr (HALO) {
+        // raw buffer loads: the base in a resource (scalar registers), the
+        // 32-bit byte offset as the only per-lane address operand (the base is
+        // the block's, or with HALO the variable's: the host runs the HALO
+        // kernels only for variables below 2^29 points, whose byte offsets fit)
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(sb), 0, HALO ? -1 : 0x7fffffff, 0x00020000);
 #pragma unroll
-            for (int c = 0; c < PREF; c++) dst[c] = sb[goff[c]];
-        } else {
-            // raw buffer loads: the block's base in a resource (scalar registers),
-            // the 32-bit byte offset as the only per-lane address operand
-            const __amdgpu_buffer_rsrc_t rs =
-                __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(sb), 0, 0x7fffffff, 0x00020000);
-#pragma unroll
-            for (int c = 0; c < PREF; c++)
-                dst[c] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, goff[c], 0, 0));
-        }
+        for (int c = 0; c < PREF; c++)
+            dst[c] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, goff[c], 0, 0));
     }
 
     // The type variable V is staged and differentiated in. The mixed path
@@ -1026,8 +1023,9 @@ __device__ __forceinline__ void tile_setup(FusedCtx<FACE, C, T, P, HALO> &ctx, c
     // elements outside the region (x or y beyond a short tile, z beyond ez+6)
     // load the block's first point and are never read
     if constexpr (HALO) {
-        // offsets are indices into the whole variable (below 2^32: the plan
-        // refuses larger variables); a padding point inside a linked
+        // offsets are byte offsets into the whole variable (below 2^32: the
+        // host runs this kernel only for variables below 2^29 points); a
+        // padding point inside a linked
         // same-level neighbour reads that neighbour's interior point at the
         // same coordinates (node index minus d (n - 2 PW - 1) per axis)
         ctx.boff = 0;
@@ -1047,7 +1045,7 @@ __device__ __forceinline__ void tile_setup(FusedCtx<FACE, C, T, P, HALO> &ctx, c
                 o >= 0 ? (uint64_t)o + (uint64_t)((gi - di * (nx - 2 * PW - 1)) +
                                                    nx * ((gj - dj * (ny - 2 * PW - 1)) + ny * (gk - dk * (nz - 2 * PW - 1))))
                        : B.offset + (uint64_t)(in_region ? gi + nx * (gj + ny * gk) : 0);
-            ctx.goff[c] = (uint32_t)a;
+            ctx.goff[c] = (uint32_t)(a * sizeof(double));
         }
     } else {
 #pragma unroll
@@ -2972,8 +2970,17 @@ int launch_padding_refresh(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
 
 // With the padding refresh on, FP64 stages read linked padding straight from
 // the neighbours (k_fused<..., HALO>); the mixed-precision kernels have no such
-// variant and get the padding copied in by k_halo_fill before each stage.
-bool halo_in_kernel(const dgr_plan *plan) { return plan->halo && plan->precision == DGR_PRECISION_FP64; }
+// variant, nor do plans whose variables hold 2^29 points or more, and they get
+// the padding copied in by k_halo_fill before each stage.
+bool halo_in_kernel(const dgr_plan *plan) {
+    // the HALO kernels address the whole variable with 32-bit byte offsets;
+    // DGR_HALO_IN_KERNEL=0 selects the copy for FP64 too (the path of larger plans)
+    static const bool allowed = [] {
+        const char *e = std::getenv("DGR_HALO_IN_KERNEL");
+        return !(e && std::string(e) == "0");
+    }();
+    return plan->halo && plan->precision == DGR_PRECISION_FP64 && plan->szpdof < (1ull << 29) && allowed;
+}
 
 // The four stage launches of one RK4 step on the fused path (rkBSSN.cpp:764-860):
 // stages read u, W, u, W and write the next state into W, u, W, u (see

@@ -235,8 +235,8 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
  * dgr_rk4_step(s) see its input state's padding refreshed, as the
  * reference's unzip between stages does (rkBSSN.cpp:764-860):
  * DGR_REFRESH_LINKED from same-level neighbours (FP64 stages read that
- * padding from the neighbours inside the stage kernel, mixed precision copies
- * it in before each stage), DGR_REFRESH_FULL also the rest (written into the
+ * padding from the neighbours inside the stage kernel when szpdof is below
+ * 2^29; mixed precision and larger plans copy it in before each stage), DGR_REFRESH_FULL also the rest (written into the
  * stage input before each stage). When the call returns, the caller's padding
  * is refreshed from the new interiors the same way. DGR_REFRESH_OFF (the
  * default) keeps every block's padding at its value at the start of the call.

@@ -284,3 +284,40 @@ def test_refined_octree_fill_and_rk4_match_oracle():
     assert e <= 1e-11
     assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref - st).max()
     plan.close()
+
+
+def test_copy_path_equals_in_kernel_refresh():
+    """The FP64 refresh copied into the stage input before each stage (the
+    path of plans of 2^29 points or more, forced here with
+    DGR_HALO_IN_KERNEL=0 in a child process) gives the same bits as the
+    in-kernel refresh, on the two-level lattice with the full refresh."""
+    import os
+    import subprocess
+    import sys
+    import torch
+    log("refresh: copy path vs in-kernel")
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    t = W.two_level_lattice()
+    st = smooth_state(t, amp=1e-2)
+    p = dgr.default_params(ko_sigma=0.4)
+    plan = dgr.Plan(t, st.shape[1])
+    plan.set_halo_refresh("full")
+    u = to_dev(st)
+    plan.rk4_steps(u, 0.02, p, nsteps=2)
+    torch.cuda.synchronize()
+    got = u.cpu().numpy()
+    plan.close()
+    code = ("import sys, numpy as np, torch; sys.path[:0] = {paths!r}\n"
+            "import dgr\nfrom dgr import workloads as W\nfrom test_oracle_halo import smooth_state\n"
+            "t = W.two_level_lattice(); st = smooth_state(t, amp=1e-2)\n"
+            "plan = dgr.Plan(t, st.shape[1]); plan.set_halo_refresh('full')\n"
+            "u = torch.from_numpy(st).cuda(); plan.rk4_steps(u, 0.02, dgr.default_params(ko_sigma=0.4), nsteps=2)\n"
+            "torch.cuda.synchronize(); np.save(sys.argv[1], u.cpu().numpy())\n").format(paths=[here, root, os.path.join(root, "oracle"), os.path.join(root, "dendro-gr_amd")])
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "copy.npy")
+        env = dict(os.environ, DGR_HALO_IN_KERNEL="0")
+        subprocess.run([sys.executable, "-c", code, out], check=True, env=env, timeout=300)
+        ref = np.load(out)
+    assert np.array_equal(got, ref)
