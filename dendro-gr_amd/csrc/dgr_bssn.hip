@@ -2680,7 +2680,7 @@ int link_neighbors(dgr_plan *plan) {
         for (size_t b = 0; b < nb; b++)
             if (c < nch[b]) wgs.push_back(make_int2((int)b, c));
     if (wgs.size() > (size_t)INT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many blocks x points");
-    if (plan->rk4_exec) {  // a captured step may hold the old table
+    if (plan->rk4_exec && plan->halo) {  // a captured step with the refresh holds the old table
         HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
         plan->rk4_exec = nullptr;
     }
@@ -2889,7 +2889,7 @@ int build_fill(dgr_plan *plan) {
         }
         groups.back().count++;
     }
-    if (plan->rk4_exec) {  // a captured step may hold the old records
+    if (plan->rk4_exec && plan->fill_general) {  // a captured step with the full refresh holds the old records
         HIP_OK(hipGraphExecDestroy(plan->rk4_exec));
         plan->rk4_exec = nullptr;
     }
