@@ -93,8 +93,9 @@ class Workload:
             self.key = "refined4-9"
             h = (self.blocks["pmax"][:, 0] - self.blocks["pmin"][:, 0]) / 12
             self.desc = (f"2:1-balanced octree of {len(self.blocks)} blocks of 13^3 on six levels refined toward "
-                         f"two punctures (Brill-Lindquist), padding of every RK4 stage refreshed across level "
-                         f"jumps (copy, injection, order-6 interpolation), one RK4 step per step")
+                         f"two punctures (Brill-Lindquist), domain-boundary faces flagged (radiative boundary), "
+                         f"padding of every RK4 stage refreshed across level jumps (copy, injection, order-6 "
+                         f"interpolation), one RK4 step per step")
             self.dt = 0.25 * float(h.min())
             self.params = dict(ko_sigma=0.4)
         elif name == "puncture":

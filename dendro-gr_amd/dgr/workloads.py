@@ -142,7 +142,7 @@ def mixed_size_lattice(h: float = 0.05, origin=(-0.43, -0.31, -0.19)) -> np.ndar
 
 
 def refined_octree(base: int = 4, max_level: int = 9, shell: float = 4.0, half_width: float = 16.0,
-                   punctures=((3.0, 0.0, 0.0), (-3.0, 0.0, 0.0))) -> np.ndarray:
+                   punctures=((3.0, 0.0, 0.0), (-3.0, 0.0, 0.0)), boundary_faces: bool = True) -> np.ndarray:
     """A 2:1-balanced octree over [-half_width, half_width]^3 refined toward
     the punctures, one 13^3 block per leaf octant (its 7^3 interior nodes are
     the octant's order-6 element; neighbours share their boundary nodes).
@@ -152,7 +152,10 @@ def refined_octree(base: int = 4, max_level: int = 9, shell: float = 4.0, half_w
     edge and corner neighbours are at most one level coarser (2:1 balance).
     Padding across a level jump lies in a coarser or finer neighbour: the
     workload of the full padding refresh. Blocks are in Morton order of their
-    octants (Dendro's block order along its space-filling curve)."""
+    octants (Dendro's block order along its space-filling curve). With
+    boundary_faces, a block's faces on the domain boundary are flagged
+    (OCT_DIR_* bits), as Dendro flags them, so the radiative boundary
+    condition applies there."""
     import itertools
     P = np.asarray(punctures, dtype=np.float64)
     size = lambda lev: 2.0 * half_width / 2 ** lev
@@ -194,13 +197,20 @@ def refined_octree(base: int = 4, max_level: int = 9, shell: float = 4.0, half_w
             split(key)
     top = max(q[0] for q in leaves)
     order = sorted(leaves, key=lambda q: _morton(q[1] << (top - q[0]), q[2] << (top - q[0]), q[3] << (top - q[0])))
-    sizes, pmins, hs = [], [], []
+    sizes, pmins, hs, flags = [], [], [], []
     for lev, i, j, k in order:
         h = size(lev) / 6.0
         sizes.append(13)
         hs.append(h)
         pmins.append(np.array([i, j, k], dtype=np.float64) * size(lev) - half_width - PW * h)
-    return _table(sizes, pmins, hs)
+        f = 0
+        if boundary_faces:
+            last = 2 ** lev - 1
+            for a, c in enumerate((i, j, k)):
+                f |= (1 << (2 * a)) if c == 0 else 0
+                f |= (1 << (2 * a + 1)) if c == last else 0
+        flags.append(f)
+    return _table(sizes, pmins, hs, flags)
 
 
 def total_padded(t: np.ndarray) -> int:
