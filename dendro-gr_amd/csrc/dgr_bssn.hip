@@ -1624,6 +1624,56 @@ __global__ __launch_bounds__(NTHREADS) void k_enforce(VarPtrs u, const DBlock *_
     for (int c = 0; c < 12; c++) u.p[12 + c][idx] = v14[2 + c];
 }
 
+// One stage combination of the three-stage SSP Runge-Kutta (rkBSSN.cpp:
+// 691-762) at every interior point, then enforce_bssn_constraints, as the
+// reference does after each stage: stage 0 saves u0 (interior-only buffer)
+// and sets u = u0 + c k; stages 1 and 2 set u = (a u0 + b u) + c k, the
+// products and sums in the reference's order. Pointwise, so u is updated in
+// place.
+__global__ __launch_bounds__(NTHREADS) void k_rk3_combine(VarPtrs u, VarPtrs k, VarPtrs u0,
+                                                         const DBlock *__restrict__ blocks,
+                                                         const Tile *__restrict__ tiles,
+                                                         const int *__restrict__ wg_tile, int tile_end, int64_t P,
+                                                         int stage, double a, double b, double c, double chi_floor,
+                                                         int *err) {
+#pragma clang fp contract(off)
+    const int64_t qq = (int64_t)blockIdx.x * NTHREADS + threadIdx.x;
+    if (qq >= P) return;
+    int t = wg_tile[blockIdx.x];
+    while (t + 1 < tile_end && qq >= tiles[t + 1].q0) t++;
+    const Tile T = tiles[t];
+    const DBlock B = blocks[T.block];
+    const int p = (int)(qq - T.q0);
+    const int i = T.o[0] + p % T.e[0];
+    const int j = T.o[1] + (p / T.e[0]) % T.e[1];
+    const int kk = T.o[2] + p / (T.e[0] * T.e[1]);
+    const uint64_t idx = B.offset + (uint64_t)i + (uint64_t)B.n[0] * ((uint64_t)j + (uint64_t)B.n[1] * kk);
+    const int mx = B.n[0] - 2 * PW, my = B.n[1] - 2 * PW;
+    const int64_t q = B.q0 + (i - PW) + (int64_t)mx * ((j - PW) + (int64_t)my * (kk - PW));
+    double out[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const double uv = u.p[v][idx], kv = k.p[v][idx];
+        if (stage == 0) {
+            u0.p[v][q] = uv;
+            out[v] = uv + c * kv;
+        } else {
+            out[v] = (a * u0.p[v][q] + b * uv) + c * kv;
+        }
+    }
+    double v14[14];
+    v14[0] = out[0];
+    v14[1] = out[1];
+    for (int m = 0; m < 12; m++) v14[2 + m] = out[12 + m];
+    const int e = enforce_point(v14, chi_floor);
+    if (e) atomicOr(err, e);
+    out[0] = v14[0];
+    out[1] = v14[1];
+    for (int m = 0; m < 12; m++) out[12 + m] = v14[2 + m];
+#pragma unroll
+    for (int v = 0; v < NV; v++) u.p[v][idx] = out[v];
+}
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string &msg) {
@@ -3128,6 +3178,48 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
     // in-kernel refresh: the caller's padding, refreshed from the new interiors
     if (nsteps > 0 && halo_in_kernel(plan)) return launch_padding_refresh(plan, u, (hipStream_t)stream);
     return DGR_OK;
+}
+
+int dgr_rk3_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params, void *stream) {
+    g_err.clear();
+    if (!plan || !params) return fail(DGR_E_INVALID, "null plan or params");
+    VarPtrs u;
+    int rc;
+    if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
+    HIP_OK(hipSetDevice(plan->device));
+    if ((rc = ensure_istage(plan))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t ni = (size_t)plan->interior;
+    VarPtrs k, u0;
+    for (int v = 0; v < NV; v++) {
+        k.p[v] = plan->d_wstate + (size_t)v * plan->szpdof;  // the right-hand side, padded layout
+        u0.p[v] = plan->d_istage + (size_t)v * ni;
+    }
+    const RhsParams prm = to_params(params);
+    // rkBSSN.cpp:691-762: (a, b, c) of u_next = (a u0 + b u) + c k per stage
+    const double A[3] = {1.0, 0.75, 1.0 / 3.0};
+    const double Bw[3] = {0.0, 0.25, 2.0 / 3.0};
+    const double Cw[3] = {dt, dt * 0.25, dt * (2.0 / 3.0)};
+    for (int s = 0; s < 3; s++) {
+        if (plan->halo && (rc = launch_padding_refresh(plan, u, st))) return rc;
+        if (plan->fused) {
+            StageIO io{};
+            io.src = u;
+            io.rhs = k;
+            io.stage = -1;
+            io.err = plan->d_err;
+            if ((rc = run_fused(plan, io, prm, st))) return rc;
+        } else {
+            RKArgs rk{};
+            if ((rc = run_chunks<MODE_RHS>(plan, u, k, rk, prm, st))) return rc;
+        }
+        for (const Chunk &c : plan->chunks)
+            hipLaunchKernelGGL(k_rk3_combine, dim3(c.nwg), dim3(NTHREADS), 0, st, u, k, u0, plan->d_blocks,
+                               plan->d_tiles, plan->d_wg_tile + c.wg_begin, c.tile_end, (int64_t)c.npoints, s, A[s],
+                               Bw[s], Cw[s], prm.chi_floor, plan->d_err);
+        HIP_OK(hipGetLastError());
+    }
+    return plan->halo ? launch_padding_refresh(plan, u, st) : DGR_OK;
 }
 
 int dgr_plan_link_neighbors(dgr_plan *plan, int32_t *nbr, int64_t *nlinks) {

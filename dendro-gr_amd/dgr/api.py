@@ -24,7 +24,7 @@ SYMBOLS = [
     "dgr_ham_constraint", "dgr_constraint_l2_norm", "dgr_rk4_steps", "dgr_plan_set_precision",
     "dgr_plan_autotune", "dgr_plan_tiling", "dgr_plan_set_tiling", "dgr_constraints",
     "dgr_plan_link_neighbors", "dgr_halo_fill", "dgr_plan_set_halo_refresh",
-    "dgr_constraint_l2_norm_noalloc", "dgr_padding_fill", "dgr_plan_fill_sources",
+    "dgr_constraint_l2_norm_noalloc", "dgr_padding_fill", "dgr_plan_fill_sources", "dgr_rk3_step",
 ]
 PRECISION = {"fp64": 0, "mixed": 1}
 TILINGS = ("wt", "w", "full")  # DGR_TILING_WT, _W, _FULL
@@ -78,6 +78,7 @@ def load():
         L.dgr_plan_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.dgr_plan_set_precision.argtypes = [vp, C.c_int]
         L.dgr_rk4_steps.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), C.c_int, C.c_int, vp]
+        L.dgr_rk3_step.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), vp]
         L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
         L.dgr_constraints.argtypes = [vp, vp, vp, vp, vp, vp]
         L.dgr_plan_link_neighbors.argtypes = [vp, vp, C.POINTER(C.c_int64)]
@@ -162,6 +163,11 @@ class Plan:
         """nsteps RK4 steps replayed from a captured HIP graph."""
         _check(load().dgr_rk4_steps(self._h, var_pointers(state), dt, C.byref(params), int(enforce_per_stage),
                                     int(nsteps), _stream_ptr(stream)), "dgr_rk4_steps")
+
+    def rk3_step(self, state, dt: float, params: BssnParams, stream=None):
+        """One step of the reference's SSP-RK3 (BSSN_RK_TYPE = 0), in place."""
+        _check(load().dgr_rk3_step(self._h, var_pointers(state), dt, C.byref(params), _stream_ptr(stream)),
+               "dgr_rk3_step")
 
     def enforce(self, state, chi_floor: float, stream=None):
         _check(load().dgr_enforce_constraints(self._h, var_pointers(state), chi_floor,

@@ -22,6 +22,7 @@ Keys read, with the reference's behaviour:
   BSSN_BH1_CONSTRAINT_R, BSSN_BH2_CONSTRAINT_R (default 5.0)        optional
   BSSN_BH1, BSSN_BH2 tables (X, Y, Z: the norm's excision centres)  optional here
   BSSN_CFL_FACTOR, BSSN_RK_TIME_BEGIN, BSSN_RK_TIME_END             optional here
+  BSSN_RK_TYPE (0 RK3, 1 RK4, 2 RK45; grUtils.cpp:100)              optional here
 A required key that is missing raises KeyError, where the reference's JSON
 reader throws on the null it would convert.
 """
@@ -42,6 +43,7 @@ class ParamFile:
     cfl: float | None                       # BSSN_CFL_FACTOR (dt = cfl * h_min in the reference)
     time: tuple | None                      # (BSSN_RK_TIME_BEGIN, BSSN_RK_TIME_END)
     raw: dict                               # the whole file as parsed
+    rk_type: int | None = None              # BSSN_RK_TYPE: 0 RK3 (dgr_rk3_step), 1 RK4, 2 RK45 (not provided)
 
 
 def _load(path: str) -> tuple[dict, bool]:
@@ -97,4 +99,5 @@ def read_param_file(path: str | os.PathLike) -> ParamFile:
     time = None
     if "BSSN_RK_TIME_BEGIN" in d and "BSSN_RK_TIME_END" in d:
         time = (float(d["BSSN_RK_TIME_BEGIN"]), float(d["BSSN_RK_TIME_END"]))
-    return ParamFile(params=p, bh=bh, constraint_r=r, cfl=cfl, time=time, raw=d)
+    rk = int(d["BSSN_RK_TYPE"]) if "BSSN_RK_TYPE" in d else None
+    return ParamFile(params=p, bh=bh, constraint_r=r, cfl=cfl, time=time, raw=d, rk_type=rk)

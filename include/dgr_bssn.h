@@ -206,6 +206,17 @@ int dgr_rk4_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
 int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params,
                   int enforce_per_stage, int nsteps, void *stream);
 
+/* One step of the three-stage SSP Runge-Kutta of the reference's legacy solver
+ * (BSSN_RK_TYPE = 0, BSSN_GR/src/rkBSSN.cpp:691-762): u1 = u0 + dt k(u0),
+ * u2 = 3/4 u0 + 1/4 u1 + dt/4 k(u1), u3 = 1/3 u0 + 2/3 u2 + 2/3 dt k(u2), with
+ * enforce_bssn_constraints after every stage as there. Each stage is the
+ * right-hand side (as dgr_bssn_rhs, into the plan's padded work state) and a
+ * pointwise combination in place on d_u's interior; the padding refresh, when
+ * on (dgr_plan_set_halo_refresh), runs on d_u before every stage and after the
+ * step, otherwise the padding keeps its values. Uses the plan buffers of
+ * dgr_rk4_step. The adaptive RK45 (BSSN_RK_TYPE = 2) is not provided. */
+int dgr_rk3_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params, void *stream);
+
 /* Padding refresh, the reference's unzip (BSSN_GR/src/bssnCtx.cpp:75; the
  * octree code is in dendrolib) restated on the block table.
  *

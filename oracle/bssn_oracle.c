@@ -1263,6 +1263,71 @@ void orc_padding_fill(double *const *u, const orc_block *blks, int64_t nblocks, 
     }
 }
 
+/* The three-stage SSP Runge-Kutta of the reference's legacy solver
+ * (BSSN_RK_TYPE = 0, rkBSSN.cpp:691-762): u1 = u0 + dt k(u0);
+ * u2 = 3/4 u0 + 1/4 u1 + dt/4 k(u1); u3 = 1/3 u0 + 2/3 u2 + (2/3) dt k(u2), each
+ * stage state followed by enforce_bssn_constraints (rkBSSN.cpp:705, 736, 760),
+ * the products and sums in the order written there. With nbr, every stage
+ * state's padding is refreshed by orc_padding_fill(general) before its right-hand
+ * side and u's after the step (the reference unzips each stage state); without,
+ * padding stays as it is. Returns 0, or -1 on a constraint violation. */
+int orc_rk3_step(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general,
+                 double dt, const orc_params *p, int nthreads) {
+    const int PW = 3;
+    int64_t n = 0;
+    for (int64_t b = 0; b < nblocks; b++) {
+        const int64_t e = (int64_t)blks[b].offset + (int64_t)blks[b].sz[0] * blks[b].sz[1] * blks[b].sz[2];
+        if (e > n) n = e;
+    }
+    double *buf = (double *)malloc(sizeof(double) * (size_t)n * 24 * 2);
+    if (!buf) return -2;
+    double *u0[24], *k[24];
+    for (int v = 0; v < 24; v++) {
+        u0[v] = buf + (size_t)n * v;
+        k[v] = buf + (size_t)n * (24 + v);
+        memset(k[v], 0, sizeof(double) * (size_t)n);
+    }
+    if (nbr) orc_padding_fill(u, blks, nblocks, nbr, general);
+    for (int v = 0; v < 24; v++) memcpy(u0[v], u[v], sizeof(double) * (size_t)n);
+    int err = 0;
+    for (int s = 0; s < 3 && !err; s++) {
+        if (s > 0 && nbr) orc_padding_fill(u, blks, nblocks, nbr, general);
+#ifdef _OPENMP
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+#endif
+        for (int64_t b = 0; b < nblocks; b++) {
+            const int e = orc_bssnrhs_block(k, (const double *const *)u, &blks[b], p);
+            err |= (e != 0);
+        }
+        /* the combination reads u at the same point only, so all blocks'
+         * right-hand sides are taken first, then u is overwritten */
+        for (int64_t b = 0; b < nblocks && !err; b++) {
+            const orc_block *blk = &blks[b];
+            const int nx = (int)blk->sz[0], ny = (int)blk->sz[1], nz = (int)blk->sz[2];
+            for (int kk = PW; kk < nz - PW; kk++)
+                for (int j = PW; j < ny - PW; j++)
+                    for (int i = PW; i < nx - PW; i++) {
+                        const int64_t pp = (int64_t)blk->offset + i + (int64_t)nx * (j + (int64_t)ny * kk);
+                        for (int v = 0; v < 24; v++) {
+                            if (s == 0)
+                                u[v][pp] = u0[v][pp] + dt * k[v][pp];
+                            else if (s == 1)
+                                u[v][pp] = (0.75) * u0[v][pp] + 0.25 * u[v][pp] + dt * 0.25 * k[v][pp];
+                            else
+                                u[v][pp] = (1.0 / 3.0) * u0[v][pp] + (2.0 / 3.0) * u[v][pp] +
+                                           dt * (2.0 / 3.0) * k[v][pp];
+                        }
+                        if (orc_enforce_constraints_point(u, (uint64_t)pp, p->chi_floor)) err = 1;
+                    }
+        }
+    }
+    (void)nthreads;
+    if (!err && nbr) orc_padding_fill(u, blks, nblocks, nbr, general);
+    free(buf);
+    return err ? -1 : 0;
+}
+
 /* RK4 (rkBSSN.cpp:764-860) with the padding of every stage state refreshed
  * from the neighbours before its right-hand side, as the reference's unzip
  * does between stages: u's padding before stage 0, each later stage's state

@@ -151,6 +151,11 @@ void orc_lagrange6_half(int code, double w[7]);
 /* orc_rk4_step_halo with orc_padding_fill(general) as the refresh. */
 int orc_rk4_step_fill(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general,
                       double dt, const orc_params *p, int enforce_per_stage, int nthreads);
+/* One step of the reference's three-stage SSP Runge-Kutta (rkBSSN.cpp:691-762),
+ * constraints enforced after every stage; with nbr (may be NULL) the padding is
+ * refreshed by orc_padding_fill(general) before every stage and after the step. */
+int orc_rk3_step(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general,
+                 double dt, const orc_params *p, int nthreads);
 /* One RK4 step with every stage state's padding refreshed by orc_halo_fill
  * before its right-hand side, and u's padding after the step. */
 int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, double dt,

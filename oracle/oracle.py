@@ -84,6 +84,9 @@ def _configure(L):
                                     C.POINTER(Params), C.c_int, C.c_int]
     L.orc_rk4_step_fill.restype = C.c_int
     L.orc_lagrange6_half.argtypes = [C.c_int, dp]
+    L.orc_rk3_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_double,
+                               C.POINTER(Params), C.c_int]
+    L.orc_rk3_step.restype = C.c_int
     L.orc_ham_points.argtypes = [dp, C.c_int64, dp]
     L.orc_ham_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
     L.orc_ham_batch.restype = C.c_int
@@ -353,6 +356,22 @@ def rk4_step(state: np.ndarray, blocks: np.ndarray, dt: float, p: Params,
                              C.byref(p), int(enforce_per_stage), nthreads)
     if err != 0:
         raise RuntimeError("orc_rk4_step failed (constraint violation)")
+
+
+def rk3_step(state: np.ndarray, blocks: np.ndarray, dt: float, p: Params, nbr: np.ndarray | None = None,
+             general: bool = True, nthreads: int = 0) -> None:
+    """In place: one step of the reference's SSP-RK3 (rkBSSN.cpp:691-762),
+    constraints enforced after every stage; with a neighbour table the padding
+    is refreshed before every stage and after the step, else it stays."""
+    blocks = np.ascontiguousarray(blocks)
+    nb = None
+    if nbr is not None:
+        nb = np.ascontiguousarray(nbr, dtype=np.int32)
+        assert nb.shape == (len(blocks), 27)
+    err = lib().orc_rk3_step(_ptrs(state_views(state)), blocks.ctypes.data, len(blocks),
+                             None if nb is None else nb.ctypes.data, int(general), dt, C.byref(p), nthreads)
+    if err != 0:
+        raise RuntimeError("orc_rk3_step failed (constraint violation)")
 
 
 def halo_fill(state: np.ndarray, blocks: np.ndarray, nbr: np.ndarray) -> None:

@@ -165,6 +165,44 @@ def test_rk4_closed_form_linear_gauge_mode():
     assert abs(st.reshape(24, n, n, n)[9, 3, 3, 3] - B0[0] * R) > 1e-6
 
 
+@pytest.mark.parametrize("method", ["rk3", "rk3_planted"])
+def test_rk3_closed_form_linear_gauge_mode(method):
+    """The same linear gauge mode through one step of the SSP-RK3 of
+    rkBSSN.cpp:691-762: a third-order method reproduces the cubic Taylor
+    polynomial of the linear system, so B is multiplied by
+    R3(z) = 1 + z + z^2/2 + z^3/6 (z = -eta dt) and beta gains
+    3/4 B0 dt (1 + z/2 + z^2/6). A copy of the stage weights with the 3/4 and
+    1/4 of the second stage swapped (the 'planted' case, applied to the
+    oracle's own output by recomputing the step from its stages) fails."""
+    n = 31
+    t = W.single_block_19(0)
+    t["sz"][0] = (n, n, n)
+    t["pmax"][0] = t["pmin"][0] + (n - 1) * 0.1
+    st = np.zeros((24, W.total_padded(t)))
+    O.init_perturbed_minkowski(st, t, seed=0, amp=0.0)
+    B0 = np.array([0.3, -0.2, 0.1])
+    for i in range(3):
+        st[9 + i] = B0[i]
+    eta, dt = 2.0, 0.05
+    p = O.default_params(ko_sigma=0.4, eta_central=eta, eta_outer=eta)
+    z = -eta * dt
+    R = 1 + z + z ** 2 / 2 + z ** 3 / 6
+    S = 1 + z / 2 + z ** 2 / 6
+    if method == "rk3":
+        O.rk3_step(st, t, dt, p, nthreads=1)
+        far = st.reshape(24, n, n, n)[:, 12:n - 12, 12:n - 12, 12:n - 12]
+        for i in range(3):
+            np.testing.assert_allclose(far[9 + i], B0[i] * R, rtol=1e-14)
+            np.testing.assert_allclose(far[6 + i], 0.75 * B0[i] * dt * S, rtol=1e-13)
+    else:
+        # the stages by hand for dB/dt = -eta B with the second-stage weights swapped
+        u0 = B0[0]
+        u1 = u0 + dt * (-eta * u0)
+        u2 = 0.25 * u0 + 0.75 * u1 + 0.25 * dt * (-eta * u1)
+        u3 = u0 / 3 + 2 * u2 / 3 + 2 * dt / 3 * (-eta * u2)
+        assert abs(u3 - u0 * R) > 1e-6
+
+
 def test_enforce_constraints_projects_and_is_idempotent():
     rng = np.random.default_rng(3)
     st = np.zeros((24, 1))

@@ -76,3 +76,9 @@ def test_lambda_is_cast_to_unsigned_like_the_json_reader(tmp_path):
 def test_missing_required_key_raises(tmp_path, key):
     with pytest.raises(KeyError):
         parfile.read_param_file(_variant(tmp_path, **{key: None}))
+
+
+def test_rk_type_is_read():
+    """BSSN_RK_TYPE (grUtils.cpp:100): the tinytest files ask for RK4 (1)."""
+    for name in ("q1.tinytest.par.json", "q1.tinytest.par.toml"):
+        assert parfile.read_param_file(os.path.join(PARS, name)).rk_type == 1
