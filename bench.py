@@ -443,7 +443,7 @@ def fill_rate(wl, plan, state, stream, calls=10):
     return {"linked_points": linked, "copied_points": ncopy, "interpolated_points": ninterp,
             "ms_per_call": round(ms, 4), "copy_equivalent_GBs": round(pts * 24 * 16 / ms / 1e6, 1),
             "use": ("RK4 stages read same-level padding inside k_fused (HALO variant) and get the rest "
-                    "written before each stage by k_fill_general; this full fill runs once per RK4 call")}
+                    "written before each stage by k_fill_copy and k_fill_interp; this full fill runs once per RK4 call")}
 
 
 def constraint_rates(wl, plan, state, stream, calls=5):

@@ -1284,146 +1284,167 @@ struct Lag6Table {
     double w[13][7];
 };
 
-// One interpolation record for variable array uv, odd (OX, OY, OZ) or even
-// code per axis. At a node of the coarse element (even code) the weights are
-// exactly 1 at that node and +-0 elsewhere, so only that tap is summed: the
-// partial sums are the same numbers the full 7-tap sums produce (adding a
-// signed zero leaves a finite sum unchanged), in the same order (x innermost,
-// then y, then z).
-template <bool OX, bool OY, bool OZ>
-__device__ __forceinline__ double fill_interp(const double *__restrict__ uv, const FillRec &r, const Lag6Table &tab,
-                                              int cx, int cy, int cz) {
-#pragma clang fp contract(off)
-    constexpr int NX = OX ? 7 : 1, NY = OY ? 7 : 1, NZ = OZ ? 7 : 1;
-    const double *base = uv + r.src + (OX ? 0 : (size_t)(cx >> 1)) + (OY ? 0 : (size_t)r.mx * (cy >> 1)) +
-                         (OZ ? 0 : (size_t)r.mxy * (cz >> 1));
-    double sz = 0.0;
-#pragma unroll
-    for (int c3 = 0; c3 < NZ; c3++) {
-        double sy = 0.0;
-#pragma unroll
-        for (int b2 = 0; b2 < NY; b2++) {
-            const double *row = base + (size_t)r.mx * b2 + (size_t)r.mxy * c3;
-            double sx = 0.0;
-#pragma unroll
-            for (int a1 = 0; a1 < NX; a1++) sx += (OX ? tab.w[cx][a1] : 1.0) * row[a1];
-            sy += (OY ? tab.w[cy][b2] : 1.0) * sx;
-        }
-        sz += (OZ ? tab.w[cz][c3] : 1.0) * sy;
-    }
-    return sz;
-}
-
-// One thread per (record, variable): blockIdx.y is the variable. Records are
-// sorted (copies, then interpolations by which axes interpolate), so a
-// wavefront takes one path.
-__global__ __launch_bounds__(256) void k_fill_general(VarPtrs u, const FillRec *__restrict__ recs, int64_t n,
-                                                      const Lag6Table tab) {
+// The copy records: one thread per (record, variable), blockIdx.y the
+// variable. They come first in the record list.
+__global__ __launch_bounds__(256) void k_fill_copy(VarPtrs u, const FillRec *__restrict__ recs, int64_t n) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     double *__restrict__ uv = u.p[blockIdx.y];
     const FillRec r = recs[q];
-    if (!(r.code & FILL_INTERP)) {
-        uv[r.dst] = uv[r.src];
-        return;
-    }
-    const int cx = (int)(r.code & 15u), cy = (int)((r.code >> 4) & 15u), cz = (int)((r.code >> 8) & 15u);
-    double val;
-    switch ((cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2)) {
-        case 0: val = fill_interp<false, false, false>(uv, r, tab, cx, cy, cz); break;
-        case 1: val = fill_interp<true, false, false>(uv, r, tab, cx, cy, cz); break;
-        case 2: val = fill_interp<false, true, false>(uv, r, tab, cx, cy, cz); break;
-        case 3: val = fill_interp<true, true, false>(uv, r, tab, cx, cy, cz); break;
-        case 4: val = fill_interp<false, false, true>(uv, r, tab, cx, cy, cz); break;
-        case 5: val = fill_interp<true, false, true>(uv, r, tab, cx, cy, cz); break;
-        case 6: val = fill_interp<false, true, true>(uv, r, tab, cx, cy, cz); break;
-        default: val = fill_interp<true, true, true>(uv, r, tab, cx, cy, cz); break;
-    }
-    uv[r.dst] = val;
+    uv[r.dst] = uv[r.src];
 }
 
-// Interpolation records that read one coarse element (at most 64, records
-// [first, first + count)), and the element's first node and strides.
+// Interpolation records that read one coarse element: records [first,
+// first + count), count <= FILL_GROUP_MAX, the element's first node and strides.
 struct FillGroup {
     uint64_t src;
     uint32_t mx, mxy;
     uint32_t first, count;
 };
 
-// The sums of fill_interp on the element staged in LDS (node (a, b, c) at
-// a + 7 b + 49 c): the same products and additions in the same order.
-template <bool OX, bool OY, bool OZ>
-__device__ __forceinline__ double fill_interp_lds(const double *s_e, const Lag6Table &tab, int cx, int cy, int cz) {
+// The value at element coordinates (cx, cy, cz)/2 (codes 0..12) is, per axis,
+// the Lagrange sum over the axis' 7 nodes (odd code) or the one node it sits
+// on (even code, weight 1), x innermost, then y, then z, each sum started at 0:
+//   sx(b, c) = sum_a wx[a] e(a, b, c),  sy(c) = sum_b wy[b] sx(b, c),  v = sum_c wz[c] sy(c).
+// k_fill_interp evaluates sx for all 13 x codes and sy for all 13 x 13 code
+// pairs once per element and variable, then each record's last sum: the
+// same products and additions, in the same order, as evaluating a record
+// alone. The x and y passes loop over codes uniformly across a wavefront, so
+// their weights are scalar operands. One group holds the records of one
+// element (up to FILL_GROUP_MAX), so the element is read once.
+constexpr int FILL_WG = 256;           // threads per workgroup
+constexpr int FILL_RPT = 4;            // records per thread
+constexpr int FILL_GROUP_MAX = FILL_WG * FILL_RPT;  // records per group
+constexpr int FILL_EPL = (343 + FILL_WG - 1) / FILL_WG;  // element nodes each thread stages
+constexpr int FILL_NVB = 2;  // variables per round of passes
+static_assert(NV % FILL_NVB == 0, "rounds of whole variables");
+
+__global__ __launch_bounds__(FILL_WG) void k_fill_interp(VarPtrs u, const FillRec *__restrict__ recs,
+                                                         const FillGroup *__restrict__ groups, const Lag6Table tab) {
 #pragma clang fp contract(off)
-    constexpr int NX = OX ? 7 : 1, NY = OY ? 7 : 1, NZ = OZ ? 7 : 1;
-    const double *base = s_e + (OX ? 0 : (cx >> 1)) + (OY ? 0 : 7 * (cy >> 1)) + (OZ ? 0 : 49 * (cz >> 1));
-    double sz = 0.0;
-#pragma unroll
-    for (int c3 = 0; c3 < NZ; c3++) {
-        double sy = 0.0;
-#pragma unroll
-        for (int b2 = 0; b2 < NY; b2++) {
-            const double *row = base + 7 * b2 + 49 * c3;
-            double sx = 0.0;
-#pragma unroll
-            for (int a1 = 0; a1 < NX; a1++) sx += (OX ? tab.w[cx][a1] : 1.0) * row[a1];
-            sy += (OY ? tab.w[cy][b2] : 1.0) * sx;
-        }
-        sz += (OZ ? tab.w[cz][c3] : 1.0) * sy;
-    }
-    return sz;
-}
-
-// One workgroup (one wave) per group of interpolation records sharing a
-// coarse element: the lanes hold their record for all 24 variables; per
-// variable the element's 343 nodes are staged in LDS (the next variable's
-// loads in flight while this one is evaluated), then every lane evaluates its
-// record from there. Records are sorted by class within a group, so lanes
-// mostly take one path.
-constexpr int FILL_EPL = (343 + 63) / 64;  // element nodes each lane stages
-
-__global__ __launch_bounds__(64) void k_fill_interp(VarPtrs u, const FillRec *__restrict__ recs,
-                                                    const FillGroup *__restrict__ groups, const Lag6Table tab) {
-    __shared__ double s_e[2][343];
+    __shared__ double s_e[FILL_NVB][343];       // the element, node (a, b, c) at a + 7 b + 49 c
+    __shared__ double s_x[FILL_NVB][13 * 49];   // sx(b, c) of x code cx at 49 cx + b + 7 c
+    __shared__ double s_y[FILL_NVB][13 * 91];   // sy(c) of codes (cx, cy) at 7 (13 cx + cy) + c
+    __shared__ double s_w[13 * 7];              // the weights, code-major
     const FillGroup g = groups[blockIdx.x];
-    const int tid = threadIdx.x;
-    const bool has = tid < (int)g.count;
-    FillRec r{};
-    if (has) r = recs[g.first + tid];
-    const int cx = (int)(r.code & 15u), cy = (int)((r.code >> 4) & 15u), cz = (int)((r.code >> 8) & 15u);
-    const int cls = (cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int k = tid; k < 13 * 7; k += FILL_WG) s_w[k] = (&tab.w[0][0])[k];
+    // this thread's records (a load inside the z pass would wait for the
+    // stores before it: loads and stores share one counter)
+    uint64_t rdst[FILL_RPT];
+    uint32_t rcode[FILL_RPT];
+#pragma unroll
+    for (int i = 0; i < FILL_RPT; i++) {
+        const int q = tid + FILL_WG * i;
+        rdst[i] = 0;
+        rcode[i] = 0;
+        if (q < (int)g.count) {
+            rdst[i] = recs[g.first + q].dst;
+            rcode[i] = recs[g.first + q].code;
+        }
+    }
     size_t off[FILL_EPL];
 #pragma unroll
     for (int k = 0; k < FILL_EPL; k++) {
-        const int e = min(tid + 64 * k, 342);
+        const int e = min(tid + FILL_WG * k, 342);
         off[k] = g.src + (size_t)(e % 7) + (size_t)g.mx * ((e / 7) % 7) + (size_t)g.mxy * (e / 49);
     }
-    double pre[FILL_EPL];
+    // every variable's nodes in flight at once; the empty asm makes each value
+    // opaque, so the compiler can neither sink a load to its variable's pass
+    // nor issue it again there (it did, and each pass then waited for its
+    // loads' full latency)
+    double pre[NV][FILL_EPL];
 #pragma unroll
-    for (int k = 0; k < FILL_EPL; k++) pre[k] = u.p[0][off[k]];
-    for (int v = 0; v < NV; v++) {
-        double *e = s_e[v & 1];
+    for (int v = 0; v < NV; v++)
 #pragma unroll
-        for (int k = 0; k < FILL_EPL; k++)
-            if (tid + 64 * k < 343) e[tid + 64 * k] = pre[k];
+        for (int k = 0; k < FILL_EPL; k++) pre[v][k] = tid + FILL_WG * k < 343 ? u.p[v][off[k]] : 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int k = 0; k < FILL_EPL; k++) asm volatile("" : "+v"(pre[v][k]));
+    // FILL_NVB variables per round of passes
+#pragma unroll
+    for (int v0 = 0; v0 < NV; v0 += FILL_NVB) {
+#pragma unroll
+        for (int m = 0; m < FILL_NVB; m++)
+#pragma unroll
+            for (int k = 0; k < FILL_EPL; k++)
+                if (tid + FILL_WG * k < 343) s_e[m][tid + FILL_WG * k] = pre[v0 + m][k];
         __syncthreads();
-        if (v + 1 < NV) {
+        // x pass: lane (b, c) < 49 of wave w, x codes w, w + 4, ...
+        if (lane < 49) {
+            double row[FILL_NVB][7];
 #pragma unroll
-            for (int k = 0; k < FILL_EPL; k++) pre[k] = u.p[v + 1][off[k]];
-        }
-        if (has) {
-            double val;
-            switch (cls) {
-                case 0: val = fill_interp_lds<false, false, false>(e, tab, cx, cy, cz); break;
-                case 1: val = fill_interp_lds<true, false, false>(e, tab, cx, cy, cz); break;
-                case 2: val = fill_interp_lds<false, true, false>(e, tab, cx, cy, cz); break;
-                case 3: val = fill_interp_lds<true, true, false>(e, tab, cx, cy, cz); break;
-                case 4: val = fill_interp_lds<false, false, true>(e, tab, cx, cy, cz); break;
-                case 5: val = fill_interp_lds<true, false, true>(e, tab, cx, cy, cz); break;
-                case 6: val = fill_interp_lds<false, true, true>(e, tab, cx, cy, cz); break;
-                default: val = fill_interp_lds<true, true, true>(e, tab, cx, cy, cz); break;
+            for (int m = 0; m < FILL_NVB; m++)
+#pragma unroll
+                for (int a1 = 0; a1 < 7; a1++) row[m][a1] = s_e[m][a1 + 7 * lane];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int x = wave + 4 * j;
+                if (x > 12) break;
+#pragma unroll
+                for (int m = 0; m < FILL_NVB; m++) {
+                    double sx = 0.0;
+                    if (x & 1) {
+#pragma unroll
+                        for (int a1 = 0; a1 < 7; a1++) sx += tab.w[x][a1] * row[m][a1];
+                    } else {
+                        sx += 1.0 * s_e[m][(x >> 1) + 7 * lane];  // (a register index would go to scratch)
+                    }
+                    s_x[m][49 * x + lane] = sx;
+                }
             }
-            u.p[v][r.dst] = val;
+        }
+        __syncthreads();
+        // y pass: lane + 64 (w & 1) = (x code, c) < 91, y codes (w >> 1), + 2, ...
+        {
+            const int pc = lane + 64 * (wave & 1);
+            if (pc < 91) {
+                const int x = pc / 7, c3 = pc - 7 * x;
+                double col[FILL_NVB][7];
+#pragma unroll
+                for (int m = 0; m < FILL_NVB; m++)
+#pragma unroll
+                    for (int b2 = 0; b2 < 7; b2++) col[m][b2] = s_x[m][49 * x + b2 + 7 * c3];
+#pragma unroll
+                for (int j = 0; j < 7; j++) {
+                    const int y = (wave >> 1) + 2 * j;
+                    if (y > 12) break;
+#pragma unroll
+                    for (int m = 0; m < FILL_NVB; m++) {
+                        double sy = 0.0;
+                        if (y & 1) {
+#pragma unroll
+                            for (int b2 = 0; b2 < 7; b2++) sy += tab.w[y][b2] * col[m][b2];
+                        } else {
+                            sy += 1.0 * s_x[m][49 * x + (y >> 1) + 7 * c3];
+                        }
+                        s_y[m][7 * (13 * x + y) + c3] = sy;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // z pass: the group's records, FILL_WG at a time
+#pragma unroll
+        for (int i = 0; i < FILL_RPT; i++) {
+            if (tid + FILL_WG * i >= (int)g.count) break;
+            const uint32_t cd = rcode[i];
+            const int cx = (int)(cd & 15u), cy = (int)((cd >> 4) & 15u), cz = (int)((cd >> 8) & 15u);
+            const double *w = s_w + 7 * cz;
+#pragma unroll
+            for (int m = 0; m < FILL_NVB; m++) {
+                const double *col = s_y[m] + 7 * (13 * cx + cy);
+                double sz = 0.0;
+                if (cz & 1) {
+#pragma unroll
+                    for (int c3 = 0; c3 < 7; c3++) sz += w[c3] * col[c3];
+                } else {
+                    sz += 1.0 * col[cz >> 1];
+                }
+                u.p[v0 + m][rdst[i]] = sz;
+            }
         }
     }
 }
@@ -1769,7 +1790,7 @@ struct dgr_plan {
     int2 *d_halo_wg = nullptr;  // k_halo_fill's workgroups: (block, chunk) of the linked blocks
     int64_t nhalo_wg = 0;
     bool halo = false;          // dgr_rk4_step(s): refresh padding from neighbours before each stage
-    bool fill_general = false;  // ... and from blocks of other sizes or levels (k_fill_general)
+    bool fill_general = false;  // ... and from blocks of other sizes or levels (k_fill_copy, k_fill_interp)
     FillRec *d_fill = nullptr;  // general fill records (built on first use)
     int64_t nfill = -1;         // records (-1: not built yet)
     int64_t nfill_interp = 0;   // of which interpolations (they follow the copies)
@@ -2915,27 +2936,26 @@ int build_fill(dgr_plan *plan) {
                     }
         }
     }
-    // copies first, then the interpolations grouped by which axes interpolate
-    // (the parity of each axis' code): a wavefront then takes one path of
-    // k_fill_general; within a group the records keep their block order
-    auto cls = [](const FillRec &r) {
-        return (r.code & FILL_INTERP) ? 1 + (int)((r.code & 1u) | ((r.code >> 3) & 2u) | ((r.code >> 6) & 4u)) : 0;
-    };
-    // copies first; the interpolations by coarse element (first node), then
-    // by class, cut into groups of at most 64 that share the element
+    // copies first; the interpolations by coarse element (first node), then by
+    // the point they write (neighbouring threads write neighbouring points),
+    // cut into groups of at most FILL_GROUP_MAX that share the element
+    auto interp = [](const FillRec &r) { return (r.code & FILL_INTERP) != 0; };
     std::stable_sort(recs.begin(), recs.end(), [&](const FillRec &a, const FillRec &b) {
-        const bool ia = cls(a) > 0, ib = cls(b) > 0;
+        const bool ia = interp(a), ib = interp(b);
         if (ia != ib) return ib;
         if (!ia) return false;
         if (a.src != b.src) return a.src < b.src;
-        return cls(a) < cls(b);
+        return a.dst < b.dst;
     });
     std::vector<FillGroup> groups;
     for (size_t k = 0; k < recs.size(); k++) {
-        if (!cls(recs[k])) continue;
-        if (groups.empty() || groups.back().src != recs[k].src || groups.back().count == 64) {
-            groups.push_back(FillGroup{recs[k].src, recs[k].mx, recs[k].mxy, (uint32_t)k, 0});
+        if (!interp(recs[k])) continue;
+        const uint32_t cd = recs[k].code;
+        if ((cd & 15u) > 12 || ((cd >> 4) & 15u) > 12 || ((cd >> 8) & 15u) > 12)
+            return fail(DGR_E_INVALID, "padding fill: element coordinate out of range");
+        if (groups.empty() || groups.back().src != recs[k].src || groups.back().count == FILL_GROUP_MAX) {
             if (k > UINT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many records");
+            groups.push_back(FillGroup{recs[k].src, recs[k].mx, recs[k].mxy, (uint32_t)k, 0});
         }
         groups.back().count++;
     }
@@ -2994,9 +3014,9 @@ int launch_fill_general(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
     const int64_t nwg = (ncopy + 255) / 256;
     if (nwg > INT32_MAX || plan->nfill_groups > INT32_MAX) return fail(DGR_E_INVALID, "padding fill: too many records");
     if (ncopy)
-        hipLaunchKernelGGL(k_fill_general, dim3((uint32_t)nwg, NV), dim3(256), 0, st, u, plan->d_fill, ncopy, tab);
+        hipLaunchKernelGGL(k_fill_copy, dim3((uint32_t)nwg, NV), dim3(256), 0, st, u, plan->d_fill, ncopy);
     if (plan->nfill_groups)
-        hipLaunchKernelGGL(k_fill_interp, dim3((uint32_t)plan->nfill_groups), dim3(64), 0, st, u, plan->d_fill,
+        hipLaunchKernelGGL(k_fill_interp, dim3((uint32_t)plan->nfill_groups), dim3(FILL_WG), 0, st, u, plan->d_fill,
                            plan->d_fill_groups, tab);
     HIP_OK(hipGetLastError());
     return DGR_OK;
