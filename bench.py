@@ -570,6 +570,34 @@ def run_fp64(wl, args, stream, dev, precision="fp64", keep_state0=True):
     return res, state0
 
 
+def refresh_rate(wl, args, stream, dev, precision):
+    """RK4 steps (graph replay) of a workload with the full padding refresh in
+    one precision: the rate alone."""
+    import torch
+    import dgr
+    state = wl.device_state(dev)
+    plan = dgr.Plan(wl.blocks, wl.n_pad)
+    plan.set_precision(precision)
+    plan.set_halo_refresh("full")
+    params = dgr.default_params(**wl.params)
+    torch.cuda.empty_cache()
+    plan.rk4_steps(state, wl.dt, params, nsteps=max(args.warmup, 1), stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.rk4_steps(state, wl.dt, params, nsteps=args.steps, stream=stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if plan.status(stream) != 0:
+        raise RuntimeError(f"constraint violation during the {precision} steps")
+    plan.close()
+    del state
+    res = {"precision": precision, "value": round(wl.npts * 4 * args.steps / elapsed / 1e6, 2),
+           "unit": "Mpoints/s", "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+           "padding": "refreshed in the stage kernels from same-level neighbours, the rest written before each stage"}
+    log(f"{wl.key} {precision}: {res['value']} Mpoints/s")
+    return res
+
+
 def large_octree(args, stream, dev):
     """The 120 M-point octree (the puncture distribution with 4x the block
     counts): `--large-steps` graph-replayed RK4 steps in FP64, then in mixed
@@ -709,6 +737,9 @@ def main():
                    "frozen_padding_mpoints_s": rr["frozen_padding_mpoints_s"],
                    "padding_fill": rr["padding_fill"], "roofline": rr["roofline"]}
         torch.cuda.empty_cache()
+        if not args.no_mixed:
+            refined["mixed_precision"] = refresh_rate(wr, args, stream, dev, "mixed")
+            torch.cuda.empty_cache()
         if not args.no_large:
             free, _ = torch.cuda.mem_get_info()
             need = 24 * 8 * (2 * Workload("puncture", 0, args.large_scale).n_pad) * 1.3

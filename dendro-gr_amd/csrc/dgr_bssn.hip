@@ -635,6 +635,7 @@ struct FusedCtx {
     bool lox, hix, loy, hiy, loz, hiz;
     uint64_t boff;
     uint32_t corner;                // padded offset (within the block) of the region's first point
+    uint64_t corner_src;            // HALO: where that point's value is read (a neighbour's interior)
     double pmin[3], h[3];
     // this thread
     int tid, ax, ay, az, cofs, dxo, dyo;
@@ -692,7 +693,7 @@ struct FusedCtx {
         constexpr int v = P::var(K);
         double *dst = pre[K % PD];
         const double *__restrict__ sb = io.src.p[v] + boff;
-        if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = sb[corner];
+        if constexpr (sizeof(T) < sizeof(double)) poff[K % PD] = HALO ? sb[corner_src] : sb[corner];
         // raw buffer loads: the base in a resource (scalar registers), the
         // 32-bit byte offset as the only per-lane address operand (the base is
         // the block's, or with HALO the variable's: the host runs the HALO
@@ -1046,6 +1047,17 @@ __device__ __forceinline__ void tile_setup(FusedCtx<FACE, C, T, P, HALO> &ctx, c
                                                    nx * ((gj - dj * (ny - 2 * PW - 1)) + ny * (gk - dk * (nz - 2 * PW - 1))))
                        : B.offset + (uint64_t)(in_region ? gi + nx * (gj + ny * gk) : 0);
             ctx.goff[c] = (uint32_t)(a * sizeof(double));
+        }
+        if constexpr (sizeof(T) < sizeof(double)) {
+            // the mixed-precision staging offset: the value at the region's
+            // first point, read where the copy path would have copied it from
+            const int gi = ctx.ox - PW, gj = ctx.oy - PW, gk = ctx.oz - PW;
+            const int di = gi < PW ? -1 : 0, dj = gj < PW ? -1 : 0, dk = gk < PW ? -1 : 0;
+            const int64_t o = (di | dj | dk) ? nbo[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)] : -1;
+            ctx.corner_src = o >= 0 ? (uint64_t)o + (uint64_t)((gi - di * (nx - 2 * PW - 1)) +
+                                                               nx * ((gj - dj * (ny - 2 * PW - 1)) +
+                                                                     ny * (gk - dk * (nz - 2 * PW - 1))))
+                                    : B.offset + (uint64_t)(gi + nx * (gj + ny * gk));
         }
     } else {
 #pragma unroll
@@ -1722,6 +1734,10 @@ extern "C" __attribute__((visibility("hidden"))) void dgr_internal_fused_f32_ker
     k[1] = reinterpret_cast<const void *>(&k_fused<false, CfgOf<float>::Thin, float>);
     k[2] = reinterpret_cast<const void *>(&k_fused<false, CfgFull, float>);
     k[3] = reinterpret_cast<const void *>(&k_fused<true, CfgOf<float>::Wide, float>);
+    k[4] = reinterpret_cast<const void *>(&k_fused<false, CfgOf<float>::Wide, float, true>);
+    k[5] = reinterpret_cast<const void *>(&k_fused<false, CfgOf<float>::Thin, float, true>);
+    k[6] = reinterpret_cast<const void *>(&k_fused<false, CfgFull, float, true>);
+    k[7] = reinterpret_cast<const void *>(&k_fused<true, CfgOf<float>::Wide, float, true>);
 }
 #else
 extern "C" void dgr_internal_fused_f32_kernels(const void **k);
@@ -2421,16 +2437,16 @@ const void *fused_kernel() {
     if constexpr (sizeof(T) == sizeof(double)) {
         return reinterpret_cast<const void *>(&k_fused<FACE, C, T, HALO>);
     } else {
-        static_assert(!HALO, "the in-kernel padding refresh is FP64 only");
         static const void *const *tab = [] {
-            static const void *k[4];
+            static const void *k[8];
             dgr_internal_fused_f32_kernels(k);
             return k;
         }();
-        if constexpr (FACE) return tab[3];
-        else if constexpr (C::FT == CfgFull::FT) return tab[2];
-        else if constexpr (C::FT == CfgThin::FT) return tab[1];
-        else return tab[0];
+        constexpr int h = HALO ? 4 : 0;  // the in-kernel refresh variants follow the others
+        if constexpr (FACE) return tab[h + 3];
+        else if constexpr (C::FT == CfgFull::FT) return tab[h + 2];
+        else if constexpr (C::FT == CfgThin::FT) return tab[h + 1];
+        else return tab[h];
     }
 }
 
@@ -2457,13 +2473,13 @@ void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, i
 template <typename T>
 void launch_fused_all(dgr_plan *plan, const Tile *tiles, const int *fg, const StageIO &io, const RhsParams &prm,
                       hipStream_t st) {
-    if constexpr (sizeof(T) == sizeof(double)) {
-        if (io.nbo) {  // the padding refresh inside the stage kernel
-            launch_fused<false, CfgWide, T, true>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
-            launch_fused<false, CfgThin, T, true>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
-            launch_fused<true, CfgWide, T, true>(plan, tiles, fg, FG_FACE, plan->dtab_wide, io, prm, st);
-            return;
-        }
+    if (io.nbo) {  // the padding refresh inside the stage kernel
+        launch_fused<false, typename CfgOf<T>::Wide, T, true>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
+        launch_fused<false, typename CfgOf<T>::Thin, T, true>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
+        if constexpr (sizeof(T) < sizeof(double))
+            launch_fused<false, CfgFull, T, true>(plan, tiles, fg, FG_FULL, plan->dtab_full, io, prm, st);
+        launch_fused<true, typename CfgOf<T>::Wide, T, true>(plan, tiles, fg, FG_FACE, plan->dtab_wide, io, prm, st);
+        return;
     }
     launch_fused<false, typename CfgOf<T>::Wide, T>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
     launch_fused<false, typename CfgOf<T>::Thin, T>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
@@ -3038,18 +3054,17 @@ int launch_padding_refresh(dgr_plan *plan, const VarPtrs &u, hipStream_t st) {
     return rc;
 }
 
-// With the padding refresh on, FP64 stages read linked padding straight from
-// the neighbours (k_fused<..., HALO>); the mixed-precision kernels have no such
-// variant, nor do plans whose variables hold 2^29 points or more, and they get
-// the padding copied in by k_halo_fill before each stage.
+// With the padding refresh on, stages read linked padding straight from the
+// neighbours (k_fused<..., HALO>); plans whose variables hold 2^29 points or
+// more get the padding copied in by k_halo_fill before each stage instead.
 bool halo_in_kernel(const dgr_plan *plan) {
     // the HALO kernels address the whole variable with 32-bit byte offsets;
-    // DGR_HALO_IN_KERNEL=0 selects the copy for FP64 too (the path of larger plans)
+    // DGR_HALO_IN_KERNEL=0 selects the copy for any plan (the path of larger plans)
     static const bool allowed = [] {
         const char *e = std::getenv("DGR_HALO_IN_KERNEL");
         return !(e && std::string(e) == "0");
     }();
-    return plan->halo && plan->precision == DGR_PRECISION_FP64 && plan->szpdof < (1ull << 29) && allowed;
+    return plan->halo && plan->szpdof < (1ull << 29) && allowed;
 }
 
 // The four stage launches of one RK4 step on the fused path (rkBSSN.cpp:764-860):
@@ -3077,10 +3092,10 @@ int rk4_fused_launches(dgr_plan *plan, const RkBuffers &buf, const VarPtrs &u, d
         io.err = plan->d_err;
         int rc;
         if (halo_in_kernel(plan)) {
-            io.nbo = plan->d_nbo;  // FP64: the stage kernel reads the neighbours itself
+            io.nbo = plan->d_nbo;  // the stage kernel reads the neighbours itself
         } else if (plan->halo && s > 0 && (rc = launch_halo_fill(plan, io.src, st))) {
-            // mixed precision: copy first; stage 0 reads u, whose padding the
-            // entry point (before the first step) or the previous step has refreshed
+            // copy first; stage 0 reads u, whose padding the entry point
+            // (before the first step) or the previous step has refreshed
             return rc;
         }
         // padding across level jumps and from blocks of other sizes: written

@@ -245,10 +245,10 @@ int dgr_rk3_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
  * dgr_plan_set_halo_refresh(plan, mode) makes every stage of
  * dgr_rk4_step(s) see its input state's padding refreshed, as the
  * reference's unzip between stages does (rkBSSN.cpp:764-860):
- * DGR_REFRESH_LINKED from same-level neighbours (FP64 stages read that
- * padding from the neighbours inside the stage kernel when szpdof is below
- * 2^29; mixed precision and larger plans copy it in before each stage), DGR_REFRESH_FULL also the rest (written into the
- * stage input before each stage). When the call returns, the caller's padding
+ * DGR_REFRESH_LINKED from same-level neighbours (the stage kernels of both
+ * precisions read that padding from the neighbours when szpdof is below 2^29;
+ * larger plans copy it in before each stage), DGR_REFRESH_FULL also the rest
+ * (written into the stage input before each stage). When the call returns, the caller's padding
  * is refreshed from the new interiors the same way. DGR_REFRESH_OFF (the
  * default) keeps every block's padding at its value at the start of the call.
  * Needs the fused path. */

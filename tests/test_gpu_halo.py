@@ -113,10 +113,11 @@ def test_refresh_off_keeps_padding_frozen():
 
 
 def test_mixed_precision_rk4_with_refresh_stays_within_error():
-    """Mixed precision has no in-kernel refresh: the padding is copied in
-    before each stage. Two steps against the oracle's FP64 stage-by-stage
-    step: the increments within 2e-5 normwise (measured 7.1e-6; FP32
-    derivatives and equations; the FP64 path meets 1e-11)."""
+    """Mixed precision with the in-kernel refresh (the FP32 stage kernels read
+    same-level padding from the neighbours as the FP64 ones do). Two steps
+    against the oracle's FP64 stage-by-stage step: the increments within 2e-5
+    normwise (measured 7.1e-6; FP32 derivatives and equations; the FP64 path
+    meets 1e-11)."""
     import torch
     log("halo: mixed rk4")
     t = lattice()
@@ -286,34 +287,39 @@ def test_refined_octree_fill_and_rk4_match_oracle():
     plan.close()
 
 
-def test_copy_path_equals_in_kernel_refresh():
-    """The FP64 refresh copied into the stage input before each stage (the
-    path of plans of 2^29 points or more, forced here with
-    DGR_HALO_IN_KERNEL=0 in a child process) gives the same bits as the
-    in-kernel refresh, on the two-level lattice with the full refresh."""
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_copy_path_equals_in_kernel_refresh(prec):
+    """The refresh copied into the stage input before each stage (the path of
+    plans of 2^29 points or more, forced here with DGR_HALO_IN_KERNEL=0 in a
+    child process) gives the same bits as the in-kernel refresh, on the
+    two-level lattice with the full refresh, in both precisions (in mixed
+    precision the staging offset of a tile whose first point is linked
+    padding is read from the neighbour, where the copy would come from)."""
     import os
     import subprocess
     import sys
     import torch
-    log("refresh: copy path vs in-kernel")
+    log(f"refresh: copy path vs in-kernel, {prec}")
     here = os.path.dirname(os.path.abspath(__file__))
     root = os.path.dirname(here)
     t = W.two_level_lattice()
     st = smooth_state(t, amp=1e-2)
     p = dgr.default_params(ko_sigma=0.4)
     plan = dgr.Plan(t, st.shape[1])
+    plan.set_precision(prec)
     plan.set_halo_refresh("full")
     u = to_dev(st)
     plan.rk4_steps(u, 0.02, p, nsteps=2)
     torch.cuda.synchronize()
+    assert plan.status() == 0
     got = u.cpu().numpy()
     plan.close()
     code = ("import sys, numpy as np, torch; sys.path[:0] = {paths!r}\n"
             "import dgr\nfrom dgr import workloads as W\nfrom test_oracle_halo import smooth_state\n"
             "t = W.two_level_lattice(); st = smooth_state(t, amp=1e-2)\n"
-            "plan = dgr.Plan(t, st.shape[1]); plan.set_halo_refresh('full')\n"
+            "plan = dgr.Plan(t, st.shape[1]); plan.set_precision({prec!r}); plan.set_halo_refresh('full')\n"
             "u = torch.from_numpy(st).cuda(); plan.rk4_steps(u, 0.02, dgr.default_params(ko_sigma=0.4), nsteps=2)\n"
-            "torch.cuda.synchronize(); np.save(sys.argv[1], u.cpu().numpy())\n").format(paths=[here, root, os.path.join(root, "oracle"), os.path.join(root, "dendro-gr_amd")])
+            "torch.cuda.synchronize(); np.save(sys.argv[1], u.cpu().numpy())\n").format(prec=prec, paths=[here, root, os.path.join(root, "oracle"), os.path.join(root, "dendro-gr_amd")])
     import tempfile
     with tempfile.TemporaryDirectory() as d:
         out = os.path.join(d, "copy.npy")
