@@ -1707,6 +1707,126 @@ __global__ __launch_bounds__(NTHREADS) void k_rk3_combine(VarPtrs u, VarPtrs k, 
     for (int v = 0; v < NV; v++) u.p[v][idx] = out[v];
 }
 
+// The tableau of the embedded Runge-Kutta 4(5) times the step size, per
+// stage: a[m] = U[s+1][m] h for stages 0..4; c4 and c5 the weights of the
+// fourth- and fifth-order solutions times h for stage 5 (rkBSSN.h:57-71, with
+// the reference's values; each product formed in FP64 as the reference forms
+// U h before multiplying by k).
+struct Rk45Coef {
+    double a[6];
+    double c4[5];
+    double c5[6];
+};
+
+// One stage of the embedded Runge-Kutta 4(5) (rkBSSN.cpp:906-1060) at every
+// interior point: stage 0 saves u0 (interior-only); the stage's right-hand
+// side r (padded) is kept in k_s (interior-only, stage-major); stages 0..4 set
+// u = u0 + sum_m a[m] k_m and enforce the constraints; stage 5 forms y4 and
+// y5 (stage 1 has weight 0 in both and is skipped, as there), enforces both,
+// leaves y5 in u and takes max |y4 - y5| over the variables into *emax (the
+// bits of a non-negative double, so unsigned max orders them). Sums in the
+// reference's order with contraction off.
+__global__ __launch_bounds__(NTHREADS) void k_rk45_stage(VarPtrs u, VarPtrs r, VarPtrs u0, double *__restrict__ kb,
+                                                        int64_t ni, const DBlock *__restrict__ blocks,
+                                                        const Tile *__restrict__ tiles,
+                                                        const int *__restrict__ wg_tile, int tile_end, int64_t P,
+                                                        int stage, Rk45Coef cf, double chi_floor, int *err,
+                                                        unsigned long long *emax) {
+#pragma clang fp contract(off)
+    __shared__ double s_max[NTHREADS / 64];
+    const int64_t qq = (int64_t)blockIdx.x * NTHREADS + threadIdx.x;
+    double dmax = 0.0;
+    if (qq < P) {
+        int t = wg_tile[blockIdx.x];
+        while (t + 1 < tile_end && qq >= tiles[t + 1].q0) t++;
+        const Tile T = tiles[t];
+        const DBlock B = blocks[T.block];
+        const int p = (int)(qq - T.q0);
+        const int i = T.o[0] + p % T.e[0];
+        const int j = T.o[1] + (p / T.e[0]) % T.e[1];
+        const int kk = T.o[2] + p / (T.e[0] * T.e[1]);
+        const uint64_t idx = B.offset + (uint64_t)i + (uint64_t)B.n[0] * ((uint64_t)j + (uint64_t)B.n[1] * kk);
+        const int mx = B.n[0] - 2 * PW, my = B.n[1] - 2 * PW;
+        const int64_t q = B.q0 + (i - PW) + (int64_t)mx * ((j - PW) + (int64_t)my * (kk - PW));
+        auto kv = [&](int m, int v) -> double & { return kb[((size_t)m * NV + v) * (size_t)ni + (size_t)q]; };
+        double out[NV];
+        for (int v = 0; v < NV; v++) {
+            if (stage == 0) u0.p[v][q] = u.p[v][idx];
+            kv(stage, v) = r.p[v][idx];
+        }
+        auto enforce = [&](double *o) {
+            double v14[14];
+            v14[0] = o[0];
+            v14[1] = o[1];
+            for (int m = 0; m < 12; m++) v14[2 + m] = o[12 + m];
+            const int e = enforce_point(v14, chi_floor);
+            if (e) atomicOr(err, e);
+            o[0] = v14[0];
+            o[1] = v14[1];
+            for (int m = 0; m < 12; m++) o[12 + m] = v14[2 + m];
+        };
+        if (stage < 5) {
+            for (int v = 0; v < NV; v++) {
+                double a = u0.p[v][q];
+                for (int m = 0; m <= stage; m++) a += cf.a[m] * kv(m, v);
+                out[v] = a;
+            }
+            enforce(out);
+        } else {
+            double y4[NV];
+            for (int v = 0; v < NV; v++) {
+                double a = u0.p[v][q];
+                for (int m = 0; m < 5; m++)
+                    if (m != 1) a += cf.c4[m] * kv(m, v);
+                y4[v] = a;
+            }
+            enforce(y4);
+            for (int v = 0; v < NV; v++) {
+                double a = u0.p[v][q];
+                for (int m = 0; m < 6; m++)
+                    if (m != 1) a += cf.c5[m] * kv(m, v);
+                out[v] = a;
+            }
+            enforce(out);
+            for (int v = 0; v < NV; v++) dmax = fmax(dmax, fabs(y4[v] - out[v]));
+        }
+        for (int v = 0; v < NV; v++) u.p[v][idx] = out[v];
+    }
+    if (stage == 5) {
+        // workgroup maximum, then one atomic per workgroup
+        for (int o = 32; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o));
+        if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = dmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = s_max[0];
+            for (int w = 1; w < NTHREADS / 64; w++) m = fmax(m, s_max[w]);
+            atomicMax(emax, (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
+// A rejected try of the embedded Runge-Kutta 4(5): u's interior back to u0.
+__global__ __launch_bounds__(NTHREADS) void k_rk45_restore(VarPtrs u, VarPtrs u0, const DBlock *__restrict__ blocks,
+                                                          const Tile *__restrict__ tiles,
+                                                          const int *__restrict__ wg_tile, int tile_end, int64_t P) {
+    const int64_t qq = (int64_t)blockIdx.x * NTHREADS + threadIdx.x;
+    if (qq >= P) return;
+    int t = wg_tile[blockIdx.x];
+    while (t + 1 < tile_end && qq >= tiles[t + 1].q0) t++;
+    const Tile T = tiles[t];
+    const DBlock B = blocks[T.block];
+    const int p = (int)(qq - T.q0);
+    const int i = T.o[0] + p % T.e[0];
+    const int j = T.o[1] + (p / T.e[0]) % T.e[1];
+    const int kk = T.o[2] + p / (T.e[0] * T.e[1]);
+    const uint64_t idx = B.offset + (uint64_t)i + (uint64_t)B.n[0] * ((uint64_t)j + (uint64_t)B.n[1] * kk);
+    const int mx = B.n[0] - 2 * PW, my = B.n[1] - 2 * PW;
+    const int64_t q = B.q0 + (i - PW) + (int64_t)mx * ((j - PW) + (int64_t)my * (kk - PW));
+    for (int v = 0; v < NV; v++) u.p[v][idx] = u0.p[v][q];
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string &msg) {
@@ -1813,6 +1933,8 @@ struct dgr_plan {
     FillGroup *d_fill_groups = nullptr;  // the interpolations grouped by coarse element
     int64_t nfill_groups = 0;
     double *d_istage = nullptr; // k_fused RK4: 2 * 24 * interior (u0 copy, accumulator)
+    double *d_rk45 = nullptr;   // dgr_rk45_step: the six stages' right-hand sides, 6 * 24 * interior
+    unsigned long long *d_rk45_emax = nullptr;  // dgr_rk45_step: the error estimate's bits
     double *d_wstate = nullptr; // k_fused RK4: 24 * szpdof, the padded work state W
     int *d_err = nullptr;
     // dgr_rk4_steps: one RK4 step captured as a graph, valid for the arguments in rk4_key
@@ -2266,6 +2388,8 @@ int dgr_plan_destroy(dgr_plan *plan) {
     (void)hipFree(plan->d_bconst);
     (void)hipFree(plan->d_tabs);
     (void)hipFree(plan->d_istage);
+    (void)hipFree(plan->d_rk45);
+    (void)hipFree(plan->d_rk45_emax);
     (void)hipFree(plan->d_wstate);
     (void)hipFree(plan->d_nbr);
     (void)hipFree(plan->d_nbo);
@@ -3255,6 +3379,97 @@ int dgr_rk3_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_p
         HIP_OK(hipGetLastError());
     }
     return plan->halo ? launch_padding_refresh(plan, u, st) : DGR_OK;
+}
+
+int dgr_rk45_step(dgr_plan *plan, double *const *d_u, double *dt, double tol, const dgr_bssn_params *params,
+                  int max_attempts, double *dt_used, double *err_norm, int *attempts, void *stream) {
+    g_err.clear();
+    if (!plan || !params || !dt) return fail(DGR_E_INVALID, "null plan, params or dt");
+    if (!(*dt > 0.0) || !(tol > 0.0) || max_attempts < 1)
+        return fail(DGR_E_INVALID, "dgr_rk45_step needs dt > 0, tol > 0 and max_attempts >= 1");
+    VarPtrs u;
+    int rc;
+    if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
+    HIP_OK(hipSetDevice(plan->device));
+    if ((rc = ensure_istage(plan))) return rc;
+    const size_t ni = (size_t)plan->interior;
+    if (!plan->d_rk45 && ni) {
+        hipError_t e = hipMalloc(&plan->d_rk45, sizeof(double) * 6 * NV * ni);
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "RK45 stage buffers");
+    }
+    if (!plan->d_rk45_emax) {
+        hipError_t e = hipMalloc(&plan->d_rk45_emax, sizeof(unsigned long long));
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "RK45 error estimate");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    VarPtrs r, u0;
+    for (int v = 0; v < NV; v++) {
+        r.p[v] = plan->d_wstate + (size_t)v * plan->szpdof;  // the right-hand side, padded layout
+        u0.p[v] = plan->d_istage + (size_t)v * ni;
+    }
+    const RhsParams prm = to_params(params);
+    // rkBSSN.h:57-71 as the reference has them (its fourth-order weight of
+    // stage 3 is 2197/4101, Fehlberg's 2197/4104)
+    static const double U[6][6] = {
+        {0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {1.0 / 4.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {3.0 / 32.0, 9.0 / 32.0, 0.0, 0.0, 0.0, 0.0},
+        {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0.0, 0.0, 0.0},
+        {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0, 0.0, 0.0},
+        {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0, 0.0}};
+    static const double C5[6] = {16.0 / 135.0, 0.0, 6656.0 / 12825.0, 28561.0 / 56430.0, -9.0 / 50.0, 2.0 / 55.0};
+    static const double C4[5] = {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4101.0, -1.0 / 5.0};
+    double h = *dt, en = 0.0;
+    int tries = 0;
+    for (;;) {
+        tries++;
+        for (int s = 0; s < 6; s++) {
+            if (plan->halo && (rc = launch_padding_refresh(plan, u, st))) return rc;
+            if (plan->fused) {
+                StageIO io{};
+                io.src = u;
+                io.rhs = r;
+                io.stage = -1;
+                io.err = plan->d_err;
+                if ((rc = run_fused(plan, io, prm, st))) return rc;
+            } else {
+                RKArgs rk{};
+                if ((rc = run_chunks<MODE_RHS>(plan, u, r, rk, prm, st))) return rc;
+            }
+            Rk45Coef cf{};
+            for (int m = 0; m < 6; m++) cf.a[m] = s < 5 ? U[s + 1][m] * h : 0.0;
+            for (int m = 0; m < 5; m++) cf.c4[m] = C4[m] * h;
+            for (int m = 0; m < 6; m++) cf.c5[m] = C5[m] * h;
+            if (s == 5) hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, plan->d_rk45_emax, 0ull);
+            for (const Chunk &c : plan->chunks)
+                hipLaunchKernelGGL(k_rk45_stage, dim3(c.nwg), dim3(NTHREADS), 0, st, u, r, u0, plan->d_rk45,
+                                   (int64_t)ni, plan->d_blocks, plan->d_tiles, plan->d_wg_tile + c.wg_begin,
+                                   c.tile_end, (int64_t)c.npoints, s, cf, prm.chi_floor, plan->d_err,
+                                   plan->d_rk45_emax);
+            HIP_OK(hipGetLastError());
+        }
+        unsigned long long bits = 0;
+        HIP_OK(hipMemcpyAsync(&bits, plan->d_rk45_emax, sizeof(bits), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        std::memcpy(&en, &bits, sizeof(en));
+        // rkBSSN.cpp:1062-1080: retry with a smaller step above the tolerance
+        if (en > tol && tries < max_attempts) {
+            h = 0.8 * h * std::pow(std::fabs(tol / en), 0.25);
+            for (const Chunk &c : plan->chunks)
+                hipLaunchKernelGGL(k_rk45_restore, dim3(c.nwg), dim3(NTHREADS), 0, st, u, u0, plan->d_blocks,
+                                   plan->d_tiles, plan->d_wg_tile + c.wg_begin, c.tile_end, (int64_t)c.npoints);
+            HIP_OK(hipGetLastError());
+            continue;
+        }
+        break;
+    }
+    if (plan->halo && (rc = launch_padding_refresh(plan, u, st))) return rc;
+    if (dt_used) *dt_used = h;
+    if (err_norm) *err_norm = en;
+    if (attempts) *attempts = tries;
+    *dt = en > tol ? 0.8 * h * std::pow(std::fabs(tol / en), 0.25) : 0.8 * h * std::pow(std::fabs(tol / en), 0.20);
+    if (en > tol) return fail(DGR_E_STEP, "dgr_rk45_step: the error estimate exceeds tol after max_attempts tries");
+    return DGR_OK;
 }
 
 int dgr_plan_link_neighbors(dgr_plan *plan, int32_t *nbr, int64_t *nlinks) {

@@ -25,6 +25,7 @@ SYMBOLS = [
     "dgr_plan_autotune", "dgr_plan_tiling", "dgr_plan_set_tiling", "dgr_constraints",
     "dgr_plan_link_neighbors", "dgr_halo_fill", "dgr_plan_set_halo_refresh",
     "dgr_constraint_l2_norm_noalloc", "dgr_padding_fill", "dgr_plan_fill_sources", "dgr_rk3_step",
+    "dgr_rk45_step",
 ]
 PRECISION = {"fp64": 0, "mixed": 1}
 TILINGS = ("wt", "w", "full")  # DGR_TILING_WT, _W, _FULL
@@ -79,6 +80,8 @@ def load():
         L.dgr_plan_set_precision.argtypes = [vp, C.c_int]
         L.dgr_rk4_steps.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), C.c_int, C.c_int, vp]
         L.dgr_rk3_step.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), vp]
+        L.dgr_rk45_step.argtypes = [vp, vp, C.POINTER(C.c_double), C.c_double, C.POINTER(BssnParams), C.c_int,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), vp]
         L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
         L.dgr_constraints.argtypes = [vp, vp, vp, vp, vp, vp]
         L.dgr_plan_link_neighbors.argtypes = [vp, vp, C.POINTER(C.c_int64)]
@@ -168,6 +171,20 @@ class Plan:
         """One step of the reference's SSP-RK3 (BSSN_RK_TYPE = 0), in place."""
         _check(load().dgr_rk3_step(self._h, var_pointers(state), dt, C.byref(params), _stream_ptr(stream)),
                "dgr_rk3_step")
+
+    def rk45_step(self, state, dt: float, tol: float, params: BssnParams, max_attempts: int = 50, stream=None):
+        """One step of the reference's embedded Runge-Kutta 4(5) with step-size
+        control (BSSN_RK_TYPE = 2), in place. Returns the step's h_used,
+        h_next, err_norm (max |y4 - y5|), attempts and accepted (False when
+        max_attempts tries all exceeded tol; the state then holds the last)."""
+        h = C.c_double(dt)
+        hu, en, att = C.c_double(0.0), C.c_double(0.0), C.c_int(0)
+        rc = load().dgr_rk45_step(self._h, var_pointers(state), C.byref(h), tol, C.byref(params), max_attempts,
+                                  C.byref(hu), C.byref(en), C.byref(att), _stream_ptr(stream))
+        if rc not in (0, -5):  # DGR_E_STEP: not accepted, reported below
+            _check(rc, "dgr_rk45_step")
+        return {"h_used": hu.value, "h_next": h.value, "err_norm": en.value, "attempts": att.value,
+                "accepted": rc == 0}
 
     def enforce(self, state, chi_floor: float, stream=None):
         _check(load().dgr_enforce_constraints(self._h, var_pointers(state), chi_floor,

@@ -23,6 +23,8 @@ Keys read, with the reference's behaviour:
   BSSN_BH1, BSSN_BH2 tables (X, Y, Z: the norm's excision centres)  optional here
   BSSN_CFL_FACTOR, BSSN_RK_TIME_BEGIN, BSSN_RK_TIME_END             optional here
   BSSN_RK_TYPE (0 RK3, 1 RK4, 2 RK45; grUtils.cpp:100)              optional here
+  BSSN_RK45_TIME_STEP_SIZE, BSSN_RK45_DESIRED_TOL (parameters.cpp:
+  220-223: the first step size and the tolerance of dgr_rk45_step)   optional here
 A required key that is missing raises KeyError, where the reference's JSON
 reader throws on the null it would convert.
 """
@@ -43,7 +45,8 @@ class ParamFile:
     cfl: float | None                       # BSSN_CFL_FACTOR (dt = cfl * h_min in the reference)
     time: tuple | None                      # (BSSN_RK_TIME_BEGIN, BSSN_RK_TIME_END)
     raw: dict                               # the whole file as parsed
-    rk_type: int | None = None              # BSSN_RK_TYPE: 0 RK3 (dgr_rk3_step), 1 RK4, 2 RK45 (not provided)
+    rk_type: int | None = None              # BSSN_RK_TYPE: 0 RK3 (dgr_rk3_step), 1 RK4, 2 RK45 (dgr_rk45_step)
+    rk45: tuple | None = None               # (BSSN_RK45_TIME_STEP_SIZE, BSSN_RK45_DESIRED_TOL)
 
 
 def _load(path: str) -> tuple[dict, bool]:
@@ -100,4 +103,7 @@ def read_param_file(path: str | os.PathLike) -> ParamFile:
     if "BSSN_RK_TIME_BEGIN" in d and "BSSN_RK_TIME_END" in d:
         time = (float(d["BSSN_RK_TIME_BEGIN"]), float(d["BSSN_RK_TIME_END"]))
     rk = int(d["BSSN_RK_TYPE"]) if "BSSN_RK_TYPE" in d else None
-    return ParamFile(params=p, bh=bh, constraint_r=r, cfl=cfl, time=time, raw=d, rk_type=rk)
+    rk45 = None
+    if "BSSN_RK45_TIME_STEP_SIZE" in d and "BSSN_RK45_DESIRED_TOL" in d:
+        rk45 = (float(d["BSSN_RK45_TIME_STEP_SIZE"]), float(d["BSSN_RK45_DESIRED_TOL"]))
+    return ParamFile(params=p, bh=bh, constraint_r=r, cfl=cfl, time=time, raw=d, rk_type=rk, rk45=rk45)

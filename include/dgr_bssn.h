@@ -31,7 +31,8 @@ enum {
     DGR_E_INVALID = -1,  /* bad argument (sizes, offsets, null pointers) */
     DGR_E_HIP = -2,      /* HIP runtime error */
     DGR_E_NOMEM = -3,    /* device allocation failed */
-    DGR_E_CONSTRAINT = -4 /* det(gt) <= 0 met while enforcing constraints */
+    DGR_E_CONSTRAINT = -4, /* det(gt) <= 0 met while enforcing constraints */
+    DGR_E_STEP = -5        /* dgr_rk45_step: tolerance not met within max_attempts tries */
 };
 
 /* Face bits of bflag: OCT_DIR_LEFT/RIGHT (-x/+x) = bits 0/1, DOWN/UP (-y/+y) =
@@ -214,8 +215,27 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
  * pointwise combination in place on d_u's interior; the padding refresh, when
  * on (dgr_plan_set_halo_refresh), runs on d_u before every stage and after the
  * step, otherwise the padding keeps its values. Uses the plan buffers of
- * dgr_rk4_step. The adaptive RK45 (BSSN_RK_TYPE = 2) is not provided. */
+ * dgr_rk4_step. */
 int dgr_rk3_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params, void *stream);
+
+/* One step of the embedded Runge-Kutta 4(5) with step-size control of the
+ * reference's legacy solver (BSSN_RK_TYPE = 2, BSSN_GR/src/rkBSSN.cpp:861-1085,
+ * tableau rkBSSN.h:57-71 as the reference has it, including its fourth-order
+ * weight 2197/4101 of stage 3). Six stages, each the right-hand side (as
+ * dgr_bssn_rhs) and a pointwise combination u = u0 + sum U[s+1][m] h k_m with
+ * enforce_bssn_constraints; then the fourth- and fifth-order solutions, both
+ * enforced, and err = max |y4 - y5| over every variable and interior point.
+ * While err > tol the step is repeated from the step's start with
+ * h = 0.8 h (tol/err)^(1/4) (at most max_attempts tries; the reference has no
+ * limit). On return d_u holds y5 of the last try, *dt_used the h it took,
+ * *err_norm its err, *attempts the tries (each may be NULL), and *dt the next
+ * step size, 0.8 h (tol/err)^(1/5) (or the retry size when the limit was hit,
+ * which returns DGR_E_STEP). The padding refresh, when on, runs before every
+ * stage and after the step. Synchronizes `stream` once per try (the accept
+ * test is on the host, as the reference's is after its all-reduce). Uses the
+ * plan buffers of dgr_rk4_step and 6 x 24 x interior doubles more. */
+int dgr_rk45_step(dgr_plan *plan, double *const *d_u, double *dt, double tol, const dgr_bssn_params *params,
+                  int max_attempts, double *dt_used, double *err_norm, int *attempts, void *stream);
 
 /* Padding refresh, the reference's unzip (BSSN_GR/src/bssnCtx.cpp:75; the
  * octree code is in dendrolib) restated on the block table.

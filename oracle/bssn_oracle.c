@@ -1328,6 +1328,123 @@ int orc_rk3_step(double *const *u, const orc_block *blks, int64_t nblocks, const
     return err ? -1 : 0;
 }
 
+/* The embedded Runge-Kutta 4(5) of the reference (rkBSSN.cpp:861-1085), with
+ * its coefficients (rkBSSN.h:57-71) as the reference writes them; note that
+ * its fourth-order weight of stage 3 is 2197/4101 where Fehlberg's is
+ * 2197/4104, so its fourth-order solution is not consistent and the error
+ * estimate carries a term of order dt. */
+static const double RK45_U[6][6] = {
+    {0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {1.0 / 4.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {3.0 / 32.0, 9.0 / 32.0, 0.0, 0.0, 0.0, 0.0},
+    {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0.0, 0.0, 0.0},
+    {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0, 0.0, 0.0},
+    {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0, 0.0}};
+static const double RK45_C5[6] = {16.0 / 135.0, 0.0, 6656.0 / 12825.0, 28561.0 / 56430.0, -9.0 / 50.0, 2.0 / 55.0};
+static const double RK45_C4[5] = {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4101.0, -1.0 / 5.0};
+
+/* One accepted step: stages 0..4 set u = u0 + sum_j (U[s+1][j] h) k_j and
+ * enforce the constraints; stage 5 forms the fourth- and fifth-order
+ * solutions (stage 1 skipped, its weights being 0), enforces both, and takes
+ * the largest |y4 - y5| over every variable and interior point. Above tol the
+ * step is repeated from u0 with h = 0.8 h (tol/err)^(1/4); else u = y5 and the
+ * next step size is 0.8 h (tol/err)^(1/5) (the reference's BSSN_SAFETY_FAC
+ * and exponents). With nbr (may be NULL) the padding is refreshed by
+ * orc_padding_fill(general) before every stage and after the step. */
+int orc_rk45_step(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general,
+                  double h, double tol, const orc_params *p, int max_attempts, int nthreads, double *h_used,
+                  double *h_next, double *err_norm, int *attempts) {
+    const int PW = 3;
+    int64_t n = 0;
+    for (int64_t b = 0; b < nblocks; b++) {
+        const int64_t e = (int64_t)blks[b].offset + (int64_t)blks[b].sz[0] * blks[b].sz[1] * blks[b].sz[2];
+        if (e > n) n = e;
+    }
+    double *buf = (double *)malloc(sizeof(double) * (size_t)n * 24 * 7);
+    if (!buf) return -2;
+    double *u0[24], *k[6][24];
+    for (int v = 0; v < 24; v++) {
+        u0[v] = buf + (size_t)n * v;
+        for (int s = 0; s < 6; s++) {
+            k[s][v] = buf + (size_t)n * (24 * (s + 1) + v);
+            memset(k[s][v], 0, sizeof(double) * (size_t)n);
+        }
+    }
+    if (nbr) orc_padding_fill(u, blks, nblocks, nbr, general);
+    for (int v = 0; v < 24; v++) memcpy(u0[v], u[v], sizeof(double) * (size_t)n);
+    int err = 0, tries = 0;
+    double en = 0.0;
+    for (;;) {
+        tries++;
+        en = 0.0;
+        for (int s = 0; s < 6 && !err; s++) {
+            if (s > 0 && nbr) orc_padding_fill(u, blks, nblocks, nbr, general);
+#ifdef _OPENMP
+            if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+#endif
+            for (int64_t b = 0; b < nblocks; b++) {
+                const int e = orc_bssnrhs_block(k[s], (const double *const *)u, &blks[b], p);
+                err |= (e != 0);
+            }
+            for (int64_t b = 0; b < nblocks && !err; b++) {
+                const orc_block *blk = &blks[b];
+                const int nx = (int)blk->sz[0], ny = (int)blk->sz[1], nz = (int)blk->sz[2];
+                for (int kk = PW; kk < nz - PW; kk++)
+                    for (int j = PW; j < ny - PW; j++)
+                        for (int i = PW; i < nx - PW; i++) {
+                            const int64_t pp = (int64_t)blk->offset + i + (int64_t)nx * (j + (int64_t)ny * kk);
+                            if (s < 5) {
+                                for (int v = 0; v < 24; v++) {
+                                    double a = u0[v][pp];
+                                    for (int m = 0; m <= s; m++) a += (RK45_U[s + 1][m] * h) * k[m][v][pp];
+                                    u[v][pp] = a;
+                                }
+                                if (orc_enforce_constraints_point(u, (uint64_t)pp, p->chi_floor)) err = 1;
+                                continue;
+                            }
+                            double y4[24];
+                            for (int v = 0; v < 24; v++) {
+                                double a = u0[v][pp];
+                                for (int m = 0; m < 5; m++)
+                                    if (m != 1) a += (RK45_C4[m] * h) * k[m][v][pp];
+                                u[v][pp] = a;
+                            }
+                            if (orc_enforce_constraints_point(u, (uint64_t)pp, p->chi_floor)) err = 1;
+                            for (int v = 0; v < 24; v++) y4[v] = u[v][pp];
+                            for (int v = 0; v < 24; v++) {
+                                double a = u0[v][pp];
+                                for (int m = 0; m < 6; m++)
+                                    if (m != 1) a += (RK45_C5[m] * h) * k[m][v][pp];
+                                u[v][pp] = a;
+                            }
+                            if (orc_enforce_constraints_point(u, (uint64_t)pp, p->chi_floor)) err = 1;
+                            for (int v = 0; v < 24; v++) {
+                                const double d = fabs(y4[v] - u[v][pp]);
+                                if (d > en) en = d;
+                            }
+                        }
+            }
+        }
+        if (err) break;
+        if (en > tol && tries < max_attempts) {
+            h = 0.8 * h * pow(fabs(tol / en), 0.25);
+            for (int v = 0; v < 24; v++) memcpy(u[v], u0[v], sizeof(double) * (size_t)n);
+            continue;
+        }
+        break;
+    }
+    (void)nthreads;
+    if (!err && nbr) orc_padding_fill(u, blks, nblocks, nbr, general);
+    free(buf);
+    if (err) return -1;
+    if (h_used) *h_used = h;
+    if (h_next) *h_next = en > tol ? 0.8 * h * pow(fabs(tol / en), 0.25) : 0.8 * h * pow(fabs(tol / en), 0.20);
+    if (err_norm) *err_norm = en;
+    if (attempts) *attempts = tries;
+    return en > tol ? 1 : 0;
+}
+
 /* RK4 (rkBSSN.cpp:764-860) with the padding of every stage state refreshed
  * from the neighbours before its right-hand side, as the reference's unzip
  * does between stages: u's padding before stage 0, each later stage's state

@@ -156,6 +156,15 @@ int orc_rk4_step_fill(double *const *u, const orc_block *blks, int64_t nblocks, 
  * refreshed by orc_padding_fill(general) before every stage and after the step. */
 int orc_rk3_step(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general,
                  double dt, const orc_params *p, int nthreads);
+/* One step of the reference's embedded Runge-Kutta 4(5) with step-size control
+ * (rkBSSN.cpp:861-1085): repeated from the step's start with a smaller h while
+ * the largest |y4 - y5| exceeds tol (at most max_attempts tries); u = y5 of the
+ * accepted try. Returns 0 when accepted, 1 when max_attempts tries all
+ * exceeded tol (u then holds the last try), -1 on a constraint violation;
+ * h_used, h_next, err_norm and attempts report the step. */
+int orc_rk45_step(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, int general,
+                  double h, double tol, const orc_params *p, int max_attempts, int nthreads, double *h_used,
+                  double *h_next, double *err_norm, int *attempts);
 /* One RK4 step with every stage state's padding refreshed by orc_halo_fill
  * before its right-hand side, and u's padding after the step. */
 int orc_rk4_step_halo(double *const *u, const orc_block *blks, int64_t nblocks, const int32_t *nbr, double dt,

@@ -87,6 +87,9 @@ def _configure(L):
     L.orc_rk3_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_double,
                                C.POINTER(Params), C.c_int]
     L.orc_rk3_step.restype = C.c_int
+    L.orc_rk45_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                C.POINTER(Params), C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int)]
+    L.orc_rk45_step.restype = C.c_int
     L.orc_ham_points.argtypes = [dp, C.c_int64, dp]
     L.orc_ham_batch.argtypes = [dp, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
     L.orc_ham_batch.restype = C.c_int
@@ -372,6 +375,29 @@ def rk3_step(state: np.ndarray, blocks: np.ndarray, dt: float, p: Params, nbr: n
                              None if nb is None else nb.ctypes.data, int(general), dt, C.byref(p), nthreads)
     if err != 0:
         raise RuntimeError("orc_rk3_step failed (constraint violation)")
+
+
+def rk45_step(state: np.ndarray, blocks: np.ndarray, h: float, tol: float, p: Params,
+              nbr: np.ndarray | None = None, general: bool = True, max_attempts: int = 50,
+              nthreads: int = 0) -> dict:
+    """In place: one step of the reference's embedded Runge-Kutta 4(5) with
+    step-size control (rkBSSN.cpp:861-1085). Returns the step's h_used,
+    h_next, err_norm (largest |y4 - y5|), attempts and accepted."""
+    blocks = np.ascontiguousarray(blocks)
+    nb = None
+    if nbr is not None:
+        nb = np.ascontiguousarray(nbr, dtype=np.int32)
+        assert nb.shape == (len(blocks), 27)
+    hu, hn, en = np.zeros(1), np.zeros(1), np.zeros(1)
+    att = C.c_int(0)
+    rc = lib().orc_rk45_step(_ptrs(state_views(state)), blocks.ctypes.data, len(blocks),
+                             None if nb is None else nb.ctypes.data, int(general), h, tol, C.byref(p),
+                             max_attempts, nthreads, hu.ctypes.data_as(C.POINTER(C.c_double)), hn.ctypes.data_as(C.POINTER(C.c_double)),
+                             en.ctypes.data_as(C.POINTER(C.c_double)), C.byref(att))
+    if rc < 0:
+        raise RuntimeError("orc_rk45_step failed (constraint violation)")
+    return {"h_used": float(hu[0]), "h_next": float(hn[0]), "err_norm": float(en[0]), "attempts": att.value,
+            "accepted": rc == 0}
 
 
 def halo_fill(state: np.ndarray, blocks: np.ndarray, nbr: np.ndarray) -> None:

@@ -203,6 +203,96 @@ def test_rk3_closed_form_linear_gauge_mode(method):
         assert abs(u3 - u0 * R) > 1e-6
 
 
+def _uniform_gauge_block(n=13):
+    """One n^3 block holding flat data with a uniform B, and a neighbour table
+    that links the block to itself in every direction (its padding refreshed
+    from its own interior): the fields stay uniform, every derivative is 0, and
+    the right-hand side is the ODE B' = -eta B, beta' = 3/4 B."""
+    t = W.single_block_19(0)
+    t["sz"][0] = (n, n, n)
+    t["pmax"][0] = t["pmin"][0] + (n - 1) * 0.1
+    st = np.zeros((24, W.total_padded(t)))
+    O.init_perturbed_minkowski(st, t, seed=0, amp=0.0)
+    B0 = np.array([0.3, -0.2, 0.1])
+    for i in range(3):
+        st[9 + i] = B0[i]
+    nbr = np.zeros((1, 27), np.int32)
+    return t, st, B0, nbr
+
+
+# Fehlberg's embedded pair: the fifth-order solution's stability function is
+# the degree-5 Taylor polynomial of e^z plus z^6/2080 (Hairer, Norsett and
+# Wanner, Solving ODEs I, the RKF45 tableau)
+def _r5(z):
+    return 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24 + z ** 5 / 120 + z ** 6 / 2080
+
+
+@pytest.mark.parametrize("method", ["rk45", "rk45_planted"])
+def test_rk45_closed_form_linear_gauge_mode(method):
+    """One accepted step of the reference's Runge-Kutta 4(5) (rkBSSN.cpp:
+    861-1085) on the uniform gauge mode: B is multiplied by R5(z), z = -eta h,
+    and beta gains 3/4 B0 h (R5(z) - 1) / z (the integral of the same stages).
+    The 'planted' case recomputes the scalar stages by hand with one tableau
+    entry changed (-845/4101 for -845/4104) and shows the check fails."""
+    t, st, B0, nbr = _uniform_gauge_block()
+    eta, h = 2.0, 0.05
+    p = O.default_params(ko_sigma=0.4, eta_central=eta, eta_outer=eta)
+    z = -eta * h
+    if method == "rk45":
+        r = O.rk45_step(st, t, h, 1.0, p, nbr=nbr, general=False, nthreads=1)
+        assert r["accepted"] and r["attempts"] == 1 and r["h_used"] == h
+        inner = st.reshape(24, 13, 13, 13)[:, 3:10, 3:10, 3:10]
+        for i in range(3):
+            np.testing.assert_allclose(inner[9 + i], B0[i] * _r5(z), rtol=1e-14)
+            np.testing.assert_allclose(inner[6 + i], 0.75 * B0[i] * h * (_r5(z) - 1) / z, rtol=1e-13)
+    else:
+        from fractions import Fraction as F
+        U = [[], [F(1, 4)], [F(3, 32), F(9, 32)], [F(1932, 2197), F(-7200, 2197), F(7296, 2197)],
+             [F(439, 216), F(-8), F(3680, 513), F(-845, 4101)],
+             [F(-8, 27), F(2), F(-3544, 2565), F(1859, 4104), F(-11, 40)]]
+        C5 = [F(16, 135), 0, F(6656, 12825), F(28561, 56430), F(-9, 50), F(2, 55)]
+        k = []
+        for j in range(6):
+            y = B0[0] + h * sum(float(U[j][m]) * k[m] for m in range(j))
+            k.append(-eta * y)
+        y5 = B0[0] + h * sum(float(C5[j]) * k[j] for j in range(6))
+        assert abs(y5 - B0[0] * _r5(z)) > 1e-9
+
+
+def test_rk45_error_estimate_and_step_control():
+    """The error estimate of the reference's pair: with its fourth-order
+    weight of stage 3 at 2197/4101 (rkBSSN.h:59; Fehlberg's is 2197/4104) the
+    two solutions differ at order h, |B4 - B5| = |B0| |z| d (1 + c3 z + O(z^2))
+    with d = 2197/4101 - 2197/4104 and c3 = 12/13 the stage's node (the
+    Fehlberg difference itself is O(z^5)). Step control (rkBSSN.cpp:1062-1080):
+    with that estimate proportional to h, a tolerance below it is met after
+    the retries h <- 0.8 h (tol/err)^(1/4) predicts, and the next step is
+    0.8 h (tol/err)^(1/5)."""
+    eta, h = 2.0, 0.01
+    p = O.default_params(ko_sigma=0.4, eta_central=eta, eta_outer=eta)
+    t, st, B0, nbr = _uniform_gauge_block()
+    r = O.rk45_step(st, t, h, 1.0, p, nbr=nbr, general=False, nthreads=1)
+    z, d = -eta * h, 2197.0 / 4101.0 - 2197.0 / 4104.0
+    expect = abs(B0).max() * abs(z) * d * (1 + 12.0 / 13.0 * z)
+    assert r["err_norm"] == pytest.approx(expect, rel=1e-3)
+    # the same model of the estimate predicts the retries and the accepted step
+    def est(hh):
+        return abs(B0).max() * eta * hh * d * (1 - 12.0 / 13.0 * eta * hh)
+    tol = r["err_norm"] / 20
+    hh, tries = h, 1
+    while est(hh) > tol:
+        hh = 0.8 * hh * (tol / est(hh)) ** 0.25
+        tries += 1
+    t, st, B0, nbr = _uniform_gauge_block()
+    r2 = O.rk45_step(st, t, h, tol, p, nbr=nbr, general=False, nthreads=1)
+    assert r2["accepted"] and r2["attempts"] == tries > 1
+    assert r2["h_used"] == pytest.approx(hh, rel=1e-3)
+    assert r2["err_norm"] <= tol
+    assert r2["h_next"] == pytest.approx(0.8 * r2["h_used"] * (tol / r2["err_norm"]) ** 0.2, rel=1e-12)
+    inner = st.reshape(24, 13, 13, 13)[:, 3:10, 3:10, 3:10]
+    np.testing.assert_allclose(inner[9], B0[0] * _r5(-eta * r2["h_used"]), rtol=1e-14)
+
+
 def test_enforce_constraints_projects_and_is_idempotent():
     rng = np.random.default_rng(3)
     st = np.zeros((24, 1))

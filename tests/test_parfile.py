@@ -82,3 +82,10 @@ def test_rk_type_is_read():
     """BSSN_RK_TYPE (grUtils.cpp:100): the tinytest files ask for RK4 (1)."""
     for name in ("q1.tinytest.par.json", "q1.tinytest.par.toml"):
         assert parfile.read_param_file(os.path.join(PARS, name)).rk_type == 1
+
+
+def test_rk45_keys_are_read():
+    """BSSN_RK45_TIME_STEP_SIZE and BSSN_RK45_DESIRED_TOL (parameters.cpp:
+    220-223) as the reference's files hold them: 0.01 and 1e-3."""
+    for name in ("q1.tinytest.par.json", "q1.tinytest.par.toml", "q2.par.toml"):
+        assert parfile.read_param_file(os.path.join(PARS, name)).rk45 == (0.01, 1e-3)
