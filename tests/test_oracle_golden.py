@@ -322,6 +322,29 @@ def test_enforce_constraints_rejects_negative_determinant():
     assert O.enforce_point(st, 0, 0.1) == -1
 
 
+@pytest.mark.parametrize("bflag", [0, 63])
+def test_harness_recipe_oracle_matches_reference_equations(bflag):
+    """The reference GPU test harness's case (cuda/src/rhs_test.cu: 13^3 blocks
+    of spacing 1e-2/13/(b % 3 + 1), fake_initial_data, KO sigma 0.4): the
+    oracle's block right-hand side equals the one evaluated with the
+    reference's own generated equations to 1e-12 normwise per variable."""
+    if O.ref_lib() is None:
+        pytest.skip("reference checkout not present")
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from gpu_util import fake_initial_data, harness_blocks
+    t = harness_blocks(6, bflag=bflag)
+    st = fake_initial_data(t)
+    p = O.default_params(ko_sigma=0.4)
+    a = O.bssnrhs_batch(st, t, p, nthreads=1)
+    with O.reference_block_algebra():
+        b = O.bssnrhs_batch(st, t, p, nthreads=1)
+    m = O.interior_mask(t, st.shape[1])
+    assert np.isfinite(a[:, m]).all()
+    err = normwise(b[:, m], a[:, m], axis=1)
+    assert err.max() < 1e-12, err
+
+
 def test_reference_block_algebra_matches_oracle_block():
     """The reference's generated equations run over whole blocks (the CPU
     baseline's configuration) give the oracle's block right-hand side, also
