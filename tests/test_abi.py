@@ -60,6 +60,25 @@ def test_plan_rejects_invalid_blocks_without_gpu(mutate, msg):
         dgr.Plan(t, W.total_padded(W.single_block_19()))
 
 
+def test_entry_points_reject_null_plans_without_gpu():
+    """Entry points return DGR_E_INVALID (-1) with a message for a null plan or
+    missing arguments, before any HIP call (so this runs without a GPU)."""
+    L = dgr.load()
+    p = dgr.default_params()
+    dt = C.c_double(0.01)
+    calls = {
+        "dgr_rk45_step": lambda: L.dgr_rk45_step(None, None, C.byref(dt), 1e-3, C.byref(p), 5, None, None, None, None),
+        "dgr_rk3_step": lambda: L.dgr_rk3_step(None, None, 0.01, C.byref(p), None),
+        "dgr_rk4_step": lambda: L.dgr_rk4_step(None, None, 0.01, C.byref(p), 1, None),
+        "dgr_bssn_rhs": lambda: L.dgr_bssn_rhs(None, None, None, 0.0, C.byref(p), None),
+        "dgr_padding_fill": lambda: L.dgr_padding_fill(None, None, None),
+        "dgr_halo_fill": lambda: L.dgr_halo_fill(None, None, None),
+    }
+    for name, call in calls.items():
+        assert call() == -1, name
+        assert L.dgr_last_error().decode(), name
+
+
 def _host_lib():
     return C.CDLL(os.path.join(ROOT, "dendro-gr_amd", "lib", "libdgr_algebra_host.so"))
 
