@@ -273,7 +273,9 @@ __device__ int enforce_point(double v14[14], double chi_floor) {
     v14[2] = g00; v14[3] = g01; v14[4] = g02; v14[5] = g11; v14[6] = g12; v14[7] = g22;
     v14[8] = A00; v14[9] = A01; v14[10] = A02; v14[11] = A11; v14[12] = A12; v14[13] = A22;
     if (v14[1] < chi_floor) v14[1] = chi_floor;
-    v14[0] = fmax(v14[0], chi_floor);
+    // std::max(alpha, CHI_FLOOR) (bssn_constraints.h:171): a NaN stays a NaN
+    // (fmax would replace it with the floor and hide it from the NaN check)
+    if (v14[0] < chi_floor) v14[0] = chi_floor;
     return err;
 }
 
@@ -1827,6 +1829,71 @@ __global__ __launch_bounds__(NTHREADS) void k_rk45_restore(VarPtrs u, VarPtrs u0
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+// The range of one variable over every interior point and the number of
+// non-finite values there (the reference's check after each step,
+// BSSNCtx::terminal_output, bssnCtx.cpp:1240-1262, takes the min and max of
+// alpha over the nodes and stops on a NaN). Finite values only enter the
+// range; they are reduced as order-preserving 64-bit keys (sign bit set for
+// non-negative numbers, all bits flipped for negative ones) with one atomic
+// per workgroup. r[0] = min key, r[1] = max key, r[2] = non-finite count.
+__device__ __forceinline__ unsigned long long order_key(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_var_range(const double *__restrict__ v,
+                                                       const DBlock *__restrict__ blocks,
+                                                       const Tile *__restrict__ tiles,
+                                                       const int *__restrict__ wg_tile, int tile_end, int64_t P,
+                                                       unsigned long long *r) {
+    __shared__ unsigned long long s_lo[NTHREADS / 64], s_hi[NTHREADS / 64], s_bad[NTHREADS / 64];
+    const int64_t qq = (int64_t)blockIdx.x * NTHREADS + threadIdx.x;
+    unsigned long long lo = ~0ull, hi = 0ull, bad = 0ull;
+    if (qq < P) {
+        int t = wg_tile[blockIdx.x];
+        while (t + 1 < tile_end && qq >= tiles[t + 1].q0) t++;
+        const Tile T = tiles[t];
+        const DBlock B = blocks[T.block];
+        const int p = (int)(qq - T.q0);
+        const int i = T.o[0] + p % T.e[0];
+        const int j = T.o[1] + (p / T.e[0]) % T.e[1];
+        const int kk = T.o[2] + p / (T.e[0] * T.e[1]);
+        const double x = v[B.offset + (uint64_t)i + (uint64_t)B.n[0] * ((uint64_t)j + (uint64_t)B.n[1] * kk)];
+        if (isfinite(x)) {
+            lo = hi = order_key(x);
+        } else {
+            bad = 1;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (unsigned long long)__shfl_xor(lo, o));
+        hi = max(hi, (unsigned long long)__shfl_xor(hi, o));
+        bad += (unsigned long long)__shfl_xor(bad, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+        s_bad[threadIdx.x >> 6] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NTHREADS / 64; w++) {
+            lo = min(lo, s_lo[w]);
+            hi = max(hi, s_hi[w]);
+            bad += s_bad[w];
+        }
+        atomicMin(&r[0], lo);
+        atomicMax(&r[1], hi);
+        if (bad) atomicAdd(&r[2], bad);
+    }
+}
+
+__global__ void k_var_range_init(unsigned long long *r) {
+    r[0] = ~0ull;
+    r[1] = 0ull;
+    r[2] = 0ull;
+}
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string &msg) {
@@ -1935,6 +2002,7 @@ struct dgr_plan {
     double *d_istage = nullptr; // k_fused RK4: 2 * 24 * interior (u0 copy, accumulator)
     double *d_rk45 = nullptr;   // dgr_rk45_step: the six stages' right-hand sides, 6 * 24 * interior
     unsigned long long *d_rk45_emax = nullptr;  // dgr_rk45_step: the error estimate's bits
+    unsigned long long *d_range = nullptr;      // dgr_variable_range: min key, max key, non-finite count
     double *d_wstate = nullptr; // k_fused RK4: 24 * szpdof, the padded work state W
     int *d_err = nullptr;
     // dgr_rk4_steps: one RK4 step captured as a graph, valid for the arguments in rk4_key
@@ -2390,6 +2458,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     (void)hipFree(plan->d_istage);
     (void)hipFree(plan->d_rk45);
     (void)hipFree(plan->d_rk45_emax);
+    (void)hipFree(plan->d_range);
     (void)hipFree(plan->d_wstate);
     (void)hipFree(plan->d_nbr);
     (void)hipFree(plan->d_nbo);
@@ -3336,6 +3405,37 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
     for (int i = 0; i < nsteps; i++) HIP_OK(hipGraphLaunch(plan->rk4_exec, (hipStream_t)stream));
     // in-kernel refresh: the caller's padding, refreshed from the new interiors
     if (nsteps > 0 && halo_in_kernel(plan)) return launch_padding_refresh(plan, u, (hipStream_t)stream);
+    return DGR_OK;
+}
+
+int dgr_variable_range(dgr_plan *plan, const double *d_var, double *vmin, double *vmax, int64_t *nonfinite,
+                       void *stream) {
+    g_err.clear();
+    if (!plan || !d_var) return fail(DGR_E_INVALID, "null plan or variable");
+    HIP_OK(hipSetDevice(plan->device));
+    if (!plan->d_range) {
+        hipError_t e = hipMalloc(&plan->d_range, 3 * sizeof(unsigned long long));
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "range scratch");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_var_range_init, dim3(1), dim3(1), 0, st, plan->d_range);
+    for (const Chunk &c : plan->chunks)
+        hipLaunchKernelGGL(k_var_range, dim3(c.nwg), dim3(NTHREADS), 0, st, d_var, plan->d_blocks, plan->d_tiles,
+                           plan->d_wg_tile + c.wg_begin, c.tile_end, (int64_t)c.npoints, plan->d_range);
+    HIP_OK(hipGetLastError());
+    unsigned long long r[3];
+    HIP_OK(hipMemcpyAsync(r, plan->d_range, sizeof(r), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    auto value = [](unsigned long long k) {
+        const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+        double x;
+        std::memcpy(&x, &b, sizeof(x));
+        return x;
+    };
+    const bool any = r[0] <= r[1];  // no finite value leaves the initial keys
+    if (vmin) *vmin = any ? value(r[0]) : std::nan("");
+    if (vmax) *vmax = any ? value(r[1]) : std::nan("");
+    if (nonfinite) *nonfinite = (int64_t)r[2];
     return DGR_OK;
 }
 

@@ -25,7 +25,7 @@ SYMBOLS = [
     "dgr_plan_autotune", "dgr_plan_tiling", "dgr_plan_set_tiling", "dgr_constraints",
     "dgr_plan_link_neighbors", "dgr_halo_fill", "dgr_plan_set_halo_refresh",
     "dgr_constraint_l2_norm_noalloc", "dgr_padding_fill", "dgr_plan_fill_sources", "dgr_rk3_step",
-    "dgr_rk45_step",
+    "dgr_rk45_step", "dgr_variable_range",
 ]
 PRECISION = {"fp64": 0, "mixed": 1}
 TILINGS = ("wt", "w", "full")  # DGR_TILING_WT, _W, _FULL
@@ -80,6 +80,8 @@ def load():
         L.dgr_plan_set_precision.argtypes = [vp, C.c_int]
         L.dgr_rk4_steps.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), C.c_int, C.c_int, vp]
         L.dgr_rk3_step.argtypes = [vp, vp, C.c_double, C.POINTER(BssnParams), vp]
+        L.dgr_variable_range.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_int64), vp]
         L.dgr_rk45_step.argtypes = [vp, vp, C.POINTER(C.c_double), C.c_double, C.POINTER(BssnParams), C.c_int,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), vp]
         L.dgr_ham_constraint.argtypes = [vp, vp, vp, vp]
@@ -185,6 +187,16 @@ class Plan:
             _check(rc, "dgr_rk45_step")
         return {"h_used": hu.value, "h_next": h.value, "err_norm": en.value, "attempts": att.value,
                 "accepted": rc == 0}
+
+    def variable_range(self, var, stream=None):
+        """(min, max, non-finite count) of one variable (a szpdof float64 CUDA
+        tensor, e.g. state[0] for alpha) over the interior points: the
+        reference's after-step check (bssnCtx.cpp:1240-1262)."""
+        assert var.dim() == 1 and var.is_cuda and str(var.dtype) == "torch.float64" and var.is_contiguous()
+        lo, hi, bad = C.c_double(0.0), C.c_double(0.0), C.c_int64(0)
+        _check(load().dgr_variable_range(self._h, C.c_void_p(var.data_ptr()), C.byref(lo), C.byref(hi),
+                                         C.byref(bad), _stream_ptr(stream)), "dgr_variable_range")
+        return lo.value, hi.value, bad.value
 
     def enforce(self, state, chi_floor: float, stream=None):
         _check(load().dgr_enforce_constraints(self._h, var_pointers(state), chi_floor,

@@ -218,6 +218,16 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
  * dgr_rk4_step. */
 int dgr_rk3_step(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_params *params, void *stream);
 
+/* The smallest and largest finite value of one variable (a szpdof array on
+ * the plan's device) over every block's interior points, and the number of
+ * non-finite values among them: the failure check the reference makes after
+ * each step (BSSNCtx::terminal_output, BSSN_GR/src/bssnCtx.cpp:1240-1262, the
+ * range of alpha, stopping the run on a NaN), here returned to the caller
+ * instead of aborting. NaN for both when no value is finite. Synchronizes
+ * `stream`. */
+int dgr_variable_range(dgr_plan *plan, const double *d_var, double *vmin, double *vmax, int64_t *nonfinite,
+                       void *stream);
+
 /* One step of the embedded Runge-Kutta 4(5) with step-size control of the
  * reference's legacy solver (BSSN_RK_TYPE = 2, BSSN_GR/src/rkBSSN.cpp:861-1085,
  * tableau rkBSSN.h:57-71 as the reference has it, including its fourth-order

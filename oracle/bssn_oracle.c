@@ -1031,7 +1031,8 @@ int orc_enforce_constraints_point(double *const *u, uint64_t node, double chi_fl
     u[ORC_SYMGT0 + 4][node] = gtd[1][2];
     u[ORC_SYMGT0 + 5][node] = gtd[2][2];
     if (u[ORC_CHI][node] < chi_floor) u[ORC_CHI][node] = chi_floor;
-    u[ORC_ALPHA][node] = u[ORC_ALPHA][node] > chi_floor ? u[ORC_ALPHA][node] : chi_floor;
+    /* std::max(alpha, CHI_FLOOR), bssn_constraints.h:171: (alpha < floor) ? floor : alpha, so a NaN stays */
+    if (u[ORC_ALPHA][node] < chi_floor) u[ORC_ALPHA][node] = chi_floor;
     return 0;
 }
 

@@ -73,6 +73,7 @@ def test_entry_points_reject_null_plans_without_gpu():
         "dgr_bssn_rhs": lambda: L.dgr_bssn_rhs(None, None, None, 0.0, C.byref(p), None),
         "dgr_padding_fill": lambda: L.dgr_padding_fill(None, None, None),
         "dgr_halo_fill": lambda: L.dgr_halo_fill(None, None, None),
+        "dgr_variable_range": lambda: L.dgr_variable_range(None, None, None, None, None, None),
     }
     for name, call in calls.items():
         assert call() == -1, name

@@ -316,6 +316,23 @@ def test_enforce_constraints_projects_and_is_idempotent():
     np.testing.assert_allclose(st, before, rtol=1e-14, atol=1e-15)
 
 
+def test_enforce_constraints_keeps_nan_in_alpha_and_chi():
+    """The floors are the reference's comparisons (bssn_constraints.h:149-171:
+    chi < floor -> floor; alpha = std::max(alpha, floor), which returns its
+    first argument unless it is below the second): a NaN in alpha or chi
+    stays a NaN, so the after-step NaN check still sees it; values below the
+    floor are raised to it."""
+    st = np.zeros((24, 2))
+    for c, val in enumerate((1.0, 0.0, 0.0, 1.0, 0.0, 1.0)):
+        st[12 + c, :] = val
+    st[0, 0], st[1, 0] = np.nan, np.nan
+    st[0, 1], st[1, 1] = 0.05, 0.02
+    O.enforce_point(st, 0, 0.1)
+    O.enforce_point(st, 1, 0.1)
+    assert np.isnan(st[0, 0]) and np.isnan(st[1, 0])
+    assert st[0, 1] == 0.1 and st[1, 1] == 0.1
+
+
 def test_enforce_constraints_rejects_negative_determinant():
     st = np.zeros((24, 1))
     st[12, 0], st[15, 0], st[17, 0] = 1.0, 1.0, -1.0
