@@ -639,8 +639,10 @@ def generate_phased(path):
 // ctx.phase<V, NEXT, K>() (K: position of V in the order) makes variable V's
 // derivatives at this thread's point available (and may start fetching NEXT);
 // ctx.d1<V, D>() / ctx.d2<M, C>() may only be called in V's phase;
-// ctx.fld<V>() may be called anywhere. Every thread of a workgroup must execute
-// the whole function (the phases synchronize the workgroup). DGR_PIN(t) marks
+// ctx.fld<V>() may be called anywhere; eta is read by reference where B's
+// right-hand sides use it, after the last phase, so a phase may still set it.
+// Every thread of a workgroup must execute the whole function (the phases
+// synchronize the workgroup). DGR_PIN(t) marks
 // where a temporary must be computed: a device build defines it to keep the
 // compiler from sinking a statement into a later phase (which would keep the
 // statement's inputs alive across phases).
@@ -666,7 +668,7 @@ constexpr int kBssnFirstPhaseVar = {vidx[0]};
 constexpr int kBssnPhaseVar[{len(vidx)}] = {{{', '.join(str(v) for v in vidx)}}};
 
 template <typename T, typename Ctx, typename Out>
-DGR_HD DGR_INLINE void bssn_rhs_phased(Ctx &ctx, const T eta, const T *lam, const T *lamf, Out &out) {{
+DGR_HD DGR_INLINE void bssn_rhs_phased(Ctx &ctx, const T &eta, const T *lam, const T *lamf, Out &out) {{
 {chr(10).join(lines)}
 }}
 

@@ -110,6 +110,8 @@ struct RhsParams {
     double chi_floor;
     int cako;
     double eps_gauge, eps_other;
+    int eta_func;                    // eta from chi's gradient (BSSN_ETA_FUNCTION), else RIT eta(r)
+    double eta_r0, eta_p0, eta_p1;
 };
 
 // -------------------------------------------------------------------------
@@ -351,6 +353,21 @@ __device__ __forceinline__ double coord(double pmin, int i, double h) {
     return pmin + i * h;
 }
 
+// The eta of the reference's BSSN_ETA_FUNCTION build (CodeGen/bssn.py:71-75,
+// generated as bssneqs_eta_func_standard_gauge.cpp, which takes the place of
+// the RIT eta(r) below): r0 sqrt(gt^ij d_i chi d_j chi) (1 - chi^p0)^-p1, from
+// the point's gt (00 01 02 11 12 22), chi and chi's first derivatives.
+template <typename T>
+__device__ __forceinline__ T eta_function(const T *g, const T *dc, T chi, double r0, double p0, double p1) {
+    const T c00 = g[3] * g[5] - g[4] * g[4], c01 = g[2] * g[4] - g[1] * g[5], c02 = g[1] * g[4] - g[2] * g[3];
+    const T c11 = g[0] * g[5] - g[2] * g[2], c12 = g[1] * g[2] - g[0] * g[4], c22 = g[0] * g[3] - g[1] * g[1];
+    const T det = g[0] * c00 + g[1] * c01 + g[2] * c02;
+    const T q = (c00 * dc[0] * dc[0] + c11 * dc[1] * dc[1] + c22 * dc[2] * dc[2] +
+                 T(2) * (c01 * dc[0] * dc[1] + c02 * dc[0] * dc[2] + c12 * dc[1] * dc[2])) /
+                det;
+    return T(r0) * sqrt(q) * pow(T(1) - pow(chi, T(p0)), -T(p1));
+}
+
 __device__ __forceinline__ double rit_eta(double x, double y, double z, const RhsParams &prm) {
 #pragma clang fp contract(off)
     const double r_coord = sqrt(x * x + y * y + z * z);
@@ -417,9 +434,15 @@ __global__ __launch_bounds__(NTHREADS) void k_rhs(VarPtrs u, VarPtrs rhs, RKArgs
     const double x = coord(B.pmin[0], g.i, B.h[0]);
     const double y = coord(B.pmin[1], g.j, B.h[1]);
     const double z = coord(B.pmin[2], g.k, B.h[2]);
-    const double eta = rit_eta(x, y, z, prm);
-
     PointIn in{&u, g.idx, W, P, q};
+    double eta;
+    if (prm.eta_func) {
+        const double gt[6] = {in.fld<12>(), in.fld<13>(), in.fld<14>(), in.fld<15>(), in.fld<16>(), in.fld<17>()};
+        const double dc[3] = {in.d1<1, 0>(), in.d1<1, 1>(), in.d1<1, 2>()};
+        eta = eta_function<double>(gt, dc, in.fld<1>(), prm.eta_r0, prm.eta_p0, prm.eta_p1);
+    } else {
+        eta = rit_eta(x, y, z, prm);
+    }
     PointOut<MODE> out;
     out.rhs = &rhs;
     out.rk = &rk;
@@ -608,11 +631,19 @@ struct StageIO {
 struct RhsPhases {
     static constexpr int N = NV;
     static constexpr bool KO = true;
+    static constexpr bool ETAF = false;  // eta: the RIT eta(r)
     static constexpr int var(int k) { return dgr::kBssnPhaseVar[k]; }
+};
+// The right-hand side of the reference's BSSN_ETA_FUNCTION build: eta from
+// chi's gradient, formed in chi's phase (its own kernels, so the default ones
+// carry none of its registers)
+struct RhsPhasesEtaF : RhsPhases {
+    static constexpr bool ETAF = true;
 };
 struct ConsPhases {
     static constexpr int N = dgr::kConsPhases;
     static constexpr bool KO = false;
+    static constexpr bool ETAF = false;
     static constexpr int var(int k) { return dgr::kConsPhaseVar[k]; }
 };
 
@@ -648,6 +679,11 @@ struct FusedCtx {
     double pre[PD][PREF];  // fetched region of the variables of phases K+1 .. K+PD
     double poff[PD];       // the staging offset of each fetched variable (mixed precision)
     T f[NV];               // this point's field values
+    // the eta the equations read (by reference, after the last phase): the
+    // RIT eta(r), or with P::ETAF the eta of chi's gradient (BSSN_ETA_FUNCTION)
+    // set in chi's phase
+    T eta_v;
+    double eta_r0, eta_p0, eta_p1;
     // LDS source offsets of this thread's D_x / D_y evaluations
     int dxw[DXIT], dyw[DYIT];  // where this thread's D_x / D_y evaluations go (see dtable)
     T d1v[3], d2v[6];
@@ -857,6 +893,8 @@ struct FusedCtx {
 #pragma unroll
             for (int k = 0; k < 6; k++) DGR_PIN(d2v[k]);
         }
+        if constexpr (V == 1 && P::ETAF)  // chi: its first derivatives are at hand
+            eta_v = eta_function<T>(f + 12, d1v, f[1], eta_r0, eta_p0, eta_p1);
         // on a flagged face the boundary value needs the first derivatives
         if constexpr (P::KO && (m2 < 0 || FACE)) ko_term<V, S>(c, kx, ky, kz);
         if constexpr (!DEEP || m2 < 0) {
@@ -908,11 +946,11 @@ struct FusedCtx {
 // The equations' outputs: on a flagged face the boundary value (already
 // holding its KO term), elsewhere equations + KO term, kept in LDS until the
 // epilogue.
-template <bool FACE, class C, typename T, bool HALO>
+template <bool FACE, class C, typename T, bool HALO, class P = RhsPhases>
 struct FusedOut {
     static constexpr int FT = C::FT;
-    const FusedCtx<FACE, C, T, RhsPhases, HALO> &ctx;
-    __device__ explicit FusedOut(const FusedCtx<FACE, C, T, RhsPhases, HALO> &c) : ctx(c) {}
+    const FusedCtx<FACE, C, T, P, HALO> &ctx;
+    __device__ explicit FusedOut(const FusedCtx<FACE, C, T, P, HALO> &c) : ctx(c) {}
     template <int V>
     __device__ __forceinline__ void put(T val) {
 #pragma clang fp contract(off)
@@ -1082,11 +1120,11 @@ __device__ __forceinline__ void tile_setup(FusedCtx<FACE, C, T, P, HALO> &ctx, c
     active_out = active;
 }
 
-template <bool FACE, class C, typename T, bool HALO = false>
+template <bool FACE, class C, typename T, bool HALO = false, class P = RhsPhases>
 __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &prm, const Tile &tl, const DBlock &B,
                                            const dgr::AxisConsts *K, const uint16_t *__restrict__ tabs,
                                            int dtab, double *s_tile, double *s_dx, double *s_dy, T *s_ko) {
-    FusedCtx<FACE, C, T, RhsPhases, HALO> ctx(io);
+    FusedCtx<FACE, C, T, P, HALO> ctx(io);
     ctx.K = K;
     ctx.load_consts();
     ctx.s_tile = s_tile;
@@ -1110,8 +1148,15 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
             ctx.sig_gauge = ctx.sig_other = T(prm.ko_sigma);
         }
     }
-    const T eta = T(rit_eta(coord(B.pmin[0], ctx.ax, B.h[0]), coord(B.pmin[1], ctx.ay, B.h[1]),
-                            coord(B.pmin[2], ctx.az, B.h[2]), prm));
+    // eta: the RIT eta(r), or (P::ETAF) set in chi's phase from chi's gradient
+    if constexpr (P::ETAF)
+        ctx.eta_v = T(0);
+    else
+        ctx.eta_v = T(rit_eta(coord(B.pmin[0], ctx.ax, B.h[0]), coord(B.pmin[1], ctx.ay, B.h[1]),
+                              coord(B.pmin[2], ctx.az, B.h[2]), prm));
+    ctx.eta_r0 = prm.eta_r0;
+    ctx.eta_p0 = prm.eta_p0;
+    ctx.eta_p1 = prm.eta_p1;
     const T lam[4] = {T(prm.lam[0]), T(prm.lam[1]), T(prm.lam[2]), T(prm.lam[3])};
     const T lamf[2] = {T(prm.lamf[0]), T(prm.lamf[1])};
     ctx.template fetch<0>();
@@ -1119,8 +1164,8 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     if constexpr (C::PD > 2) ctx.template fetch<2>();
     static_assert(C::PD <= 3, "prefetch distance");
     ctx.begin();
-    FusedOut<FACE, C, T, HALO> out(ctx);
-    dgr::bssn_rhs_phased<T>(ctx, eta, lam, lamf, out);
+    FusedOut<FACE, C, T, HALO, P> out(ctx);
+    dgr::bssn_rhs_phased<T>(ctx, ctx.eta_v, lam, lamf, out);
     if (active) fused_epilogue<C, T>(io, s_ko, ctx.f, ctx.tid, idx, q, prm.chi_floor);
 }
 
@@ -1131,7 +1176,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
 #define DGR_FP32_WAVES 2  // waves per SIMD the FP32 kernels are built for
 #endif
 
-template <bool FACE, class C, typename T, bool HALO = false>
+template <bool FACE, class C, typename T, bool HALO = false, bool ETAF = false>
 __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES : 1) void k_fused(
     const StageIO io, const RhsParams prm, const DBlock *__restrict__ blocks,
     const dgr::AxisConsts *__restrict__ bconst, const Tile *__restrict__ tiles,
@@ -1146,8 +1191,8 @@ __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES 
 
     const Tile tl = tiles[blockIdx.x];
     const DBlock B = blocks[tl.block];
-    fused_tile<FACE, C, T, HALO>(io, prm, tl, B, bconst + 3 * (size_t)tl.block, tabs, dtab, s_tile, s_dx, s_dy,
-                                 s_ko);
+    fused_tile<FACE, C, T, HALO, std::conditional_t<ETAF, RhsPhasesEtaF, RhsPhases>>(
+        io, prm, tl, B, bconst + 3 * (size_t)tl.block, tabs, dtab, s_tile, s_dx, s_dy, s_ko);
 }
 
 struct ConsPtrs {
@@ -2262,6 +2307,10 @@ void dgr_default_params(dgr_bssn_params *p) {
     p->cako_enabled = 0;
     p->eps_cako_gauge = 0.99;
     p->eps_cako_other = 0.3;
+    p->eta_func = 0;  // the reference's default build (BSSN_ETA_FUNCTION off)
+    p->eta_r0 = 1.31;  // parameters.cpp:59-60
+    p->eta_power[0] = 2.0;
+    p->eta_power[1] = 2.0;
 }
 
 const char *dgr_last_error(void) { return g_err.c_str(); }
@@ -2563,6 +2612,10 @@ RhsParams to_params(const dgr_bssn_params *p) {
     r.cako = p->cako_enabled;
     r.eps_gauge = p->eps_cako_gauge;
     r.eps_other = p->eps_cako_other;
+    r.eta_func = p->eta_func != 0;
+    r.eta_r0 = p->eta_r0;
+    r.eta_p0 = p->eta_power[0];
+    r.eta_p1 = p->eta_power[1];
     return r;
 }
 
@@ -2625,11 +2678,12 @@ int run_chunks(dgr_plan *plan, const VarPtrs &state, const VarPtrs &rhs, const R
 
 // Address of a fused kernel: the FP64 ones live in this unit, the FP32 ones
 // in the unit compiled with DGR_F32_KERNELS (see there).
-template <bool FACE, class C, typename T, bool HALO>
+template <bool FACE, class C, typename T, bool HALO, bool ETAF = false>
 const void *fused_kernel() {
     if constexpr (sizeof(T) == sizeof(double)) {
-        return reinterpret_cast<const void *>(&k_fused<FACE, C, T, HALO>);
+        return reinterpret_cast<const void *>(&k_fused<FACE, C, T, HALO, ETAF>);
     } else {
+        static_assert(!ETAF, "the eta function is FP64 only");
         static const void *const *tab = [] {
             static const void *k[8];
             dgr_internal_fused_f32_kernels(k);
@@ -2643,7 +2697,7 @@ const void *fused_kernel() {
     }
 }
 
-template <bool FACE, class C, typename T, bool HALO = false>
+template <bool FACE, class C, typename T, bool HALO = false, bool ETAF = false>
 void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, int dtab, const StageIO &io,
                   const RhsParams &prm, hipStream_t st) {
     const int n = fg[group + 1] - fg[group];
@@ -2658,7 +2712,7 @@ void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, i
     int a_dtab = dtab;
     void *args[] = {&a_io, &a_prm, &a_blocks, &a_bconst, &a_tiles, &a_tabs, &a_dtab};
     timed(plan, 3 + group, st, [&] {
-        (void)hipLaunchKernel(fused_kernel<FACE, C, T, HALO>(), dim3(n), dim3(C::FT), args, 0, st);
+        (void)hipLaunchKernel(fused_kernel<FACE, C, T, HALO, ETAF>(), dim3(n), dim3(C::FT), args, 0, st);
     });
 }
 
@@ -2666,6 +2720,20 @@ void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, i
 template <typename T>
 void launch_fused_all(dgr_plan *plan, const Tile *tiles, const int *fg, const StageIO &io, const RhsParams &prm,
                       hipStream_t st) {
+    if constexpr (sizeof(T) == sizeof(double)) {
+        if (prm.eta_func) {  // the reference's BSSN_ETA_FUNCTION build (FP64 only)
+            if (io.nbo) {
+                launch_fused<false, CfgWide, T, true, true>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
+                launch_fused<false, CfgThin, T, true, true>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
+                launch_fused<true, CfgWide, T, true, true>(plan, tiles, fg, FG_FACE, plan->dtab_wide, io, prm, st);
+            } else {
+                launch_fused<false, CfgWide, T, false, true>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
+                launch_fused<false, CfgThin, T, false, true>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
+                launch_fused<true, CfgWide, T, false, true>(plan, tiles, fg, FG_FACE, plan->dtab_wide, io, prm, st);
+            }
+            return;
+        }
+    }
     if (io.nbo) {  // the padding refresh inside the stage kernel
         launch_fused<false, typename CfgOf<T>::Wide, T, true>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
         launch_fused<false, typename CfgOf<T>::Thin, T, true>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
@@ -2682,6 +2750,8 @@ void launch_fused_all(dgr_plan *plan, const Tile *tiles, const int *fg, const St
 }
 
 int run_fused(dgr_plan *plan, const StageIO &io, const RhsParams &prm, hipStream_t st) {
+    if (plan->precision == DGR_PRECISION_MIXED && prm.eta_func)
+        return fail(DGR_E_INVALID, "the eta function (eta_func, BSSN_ETA_FUNCTION) needs FP64 precision");
     if (plan->precision == DGR_PRECISION_MIXED)
         launch_fused_all<float>(plan, plan->d_ftiles[1], plan->fgroup[1], io, prm, st);
     else

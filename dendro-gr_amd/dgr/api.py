@@ -41,9 +41,11 @@ class BssnParams(C.Structure):
         ("ko_sigma", C.c_double),
         ("chi_floor", C.c_double),
         ("cako_enabled", C.c_int32),
-        ("reserved", C.c_int32),
+        ("eta_func", C.c_int32),
         ("eps_cako_gauge", C.c_double),
         ("eps_cako_other", C.c_double),
+        ("eta_r0", C.c_double),
+        ("eta_power", C.c_double * 2),
     ]
 
 
@@ -111,6 +113,8 @@ def default_params(**kw) -> BssnParams:
                 p.lambda_[i] = v[i]
         elif k == "lambda_f":
             p.lambda_f[0], p.lambda_f[1] = v
+        elif k == "eta_power":
+            p.eta_power[0], p.eta_power[1] = v
         else:
             setattr(p, k, v)
     return p

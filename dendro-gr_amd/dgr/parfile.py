@@ -23,6 +23,8 @@ Keys read, with the reference's behaviour:
   BSSN_BH1, BSSN_BH2 tables (X, Y, Z: the norm's excision centres)  optional here
   BSSN_CFL_FACTOR, BSSN_RK_TIME_BEGIN, BSSN_RK_TIME_END             optional here
   BSSN_RK_TYPE (0 RK3, 1 RK4, 2 RK45; grUtils.cpp:100)              optional here
+  BSSN_ETA_R0, BSSN_ETA_POWER (JSON: BSSN_ETA_POWER_1/_2; TOML: array;
+  the eta function's constants, grUtils.cpp:205-207)                 optional
   BSSN_RK45_TIME_STEP_SIZE, BSSN_RK45_DESIRED_TOL (parameters.cpp:
   220-223: the first step size and the tolerance of dgr_rk45_step)   optional here
 A required key that is missing raises KeyError, where the reference's JSON
@@ -93,6 +95,16 @@ def read_param_file(path: str | os.PathLike) -> ParamFile:
     if bool(d.get("BSSN_KO_SIGMA_SCALE_BY_CONFORMAL_POST_MERGER_ONLY", False)):
         scale = False
     p.cako_enabled = 1 if scale else 0
+    # the eta function's constants (grUtils.cpp:205-207, parameters.cpp:329-331);
+    # whether it is used is a build option there (BSSN_ETA_FUNCTION), here
+    # params.eta_func, left to the caller
+    if "BSSN_ETA_R0" in d:
+        p.eta_r0 = float(d["BSSN_ETA_R0"])
+    if "BSSN_ETA_POWER" in d:
+        pw = d["BSSN_ETA_POWER"]
+        if isinstance(pw, dict):
+            pw = (pw["BSSN_ETA_POWER_1"], pw["BSSN_ETA_POWER_2"])
+        p.eta_power[0], p.eta_power[1] = float(pw[0]), float(pw[1])
 
     bh = None
     if "BSSN_BH1" in d and "BSSN_BH2" in d:

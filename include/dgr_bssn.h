@@ -69,9 +69,14 @@ typedef struct dgr_bssn_params {
     double ko_sigma;        /* KO_DISS_SIGMA */
     double chi_floor;       /* CHI_FLOOR */
     int32_t cako_enabled;   /* BSSN_CAKO_ENABLED */
-    int32_t reserved;
+    int32_t eta_func;       /* 0: the RIT eta(r) above (rhs.cpp:209-215); 1: the eta of the
+                               reference's BSSN_ETA_FUNCTION build (USE_ETA_FUNC, equations
+                               bssneqs_eta_func_standard_gauge.cpp): eta_r0 |d chi| / (1 - chi^p0)^p1,
+                               |d chi|^2 = gt^ij d_i chi d_j chi (CodeGen/bssn.py:71-75) */
     double eps_cako_gauge;  /* BSSN_EPSILON_CAKO_GAUGE */
     double eps_cako_other;  /* BSSN_EPSILON_CAKO_OTHER */
+    double eta_r0;          /* BSSN_ETA_R0 */
+    double eta_power[2];    /* BSSN_ETA_POWER */
 } dgr_bssn_params;
 
 typedef struct dgr_plan dgr_plan;

@@ -201,6 +201,10 @@ void orc_default_params(orc_params *p) {
     p->cako_enabled = 0;
     p->eps_cako_gauge = 0.99;
     p->eps_cako_other = 0.3;
+    p->eta_func = 0;          /* the reference's default build (BSSN_ETA_FUNCTION off) */
+    p->eta_r0 = 1.31;         /* parameters.cpp:59-60 */
+    p->eta_power[0] = 2.0;
+    p->eta_power[1] = 2.0;
 }
 
 static const int SYM6[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
@@ -631,6 +635,21 @@ void orc_bssn_points(const double *in, int64_t npts, const orc_params *p, double
 /* ======================================================================== */
 
 /* The 11 variables with second derivatives (BSSN_GR/scripts/bssnrhs_derivs.h). */
+/* The damping of the reference's BSSN_ETA_FUNCTION build (CodeGen/bssn.py:
+ * 71-75, generated as bssneqs_eta_func_standard_gauge.cpp, which replaces the
+ * RIT eta(r) of rhs.cpp:209-215): R0 sqrt(gt^ij d_i chi d_j chi) / (1 - chi^p0)^p1
+ * at the point whose algebra input is `in` (fields, first derivatives). */
+double orc_eta_func(const double *in, const orc_params *p) {
+    const double *g = in + ORC_SYMGT0;  /* gt 00 01 02 11 12 22 */
+    const double c00 = g[3] * g[5] - g[4] * g[4], c01 = g[2] * g[4] - g[1] * g[5], c02 = g[1] * g[4] - g[2] * g[3];
+    const double c11 = g[0] * g[5] - g[2] * g[2], c12 = g[1] * g[2] - g[0] * g[4], c22 = g[0] * g[3] - g[1] * g[1];
+    const double det = g[0] * c00 + g[1] * c01 + g[2] * c02;
+    const double *dc = in + 24 + 3 * ORC_CHI;
+    const double q = (c00 * dc[0] * dc[0] + c11 * dc[1] * dc[1] + c22 * dc[2] * dc[2] +
+                      2.0 * (c01 * dc[0] * dc[1] + c02 * dc[0] * dc[2] + c12 * dc[1] * dc[2])) / det;
+    return p->eta_r0 * sqrt(q) * pow(1.0 - pow(in[ORC_CHI], p->eta_power[0]), -p->eta_power[1]);
+}
+
 static const int D2VARS[11] = {ORC_ALPHA, ORC_CHI, ORC_BETA0, ORC_BETA0 + 1, ORC_BETA0 + 2,
                                ORC_SYMGT0, ORC_SYMGT0 + 1, ORC_SYMGT0 + 2, ORC_SYMGT0 + 3,
                                ORC_SYMGT0 + 4, ORC_SYMGT0 + 5};
@@ -779,7 +798,7 @@ int orc_bssnrhs_block(double *const *rhs_all, const double *const *u_all, const 
                 }
                 for (int m = 0; m < 11; m++)
                     for (int c = 0; c < 6; c++) in[96 + 6 * m + c] = d2[m][c][pp];
-                in[162] = eta;
+                in[162] = p->eta_func ? orc_eta_func(in, p) : eta;
                 g_point_fn(in, p, out);
                 for (int v = 0; v < 24; v++) rhs[v][pp] = out[v];
             }

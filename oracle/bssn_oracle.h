@@ -80,9 +80,11 @@ typedef struct orc_params {
     double ko_sigma;       /* KO_DISS_SIGMA */
     double chi_floor;      /* CHI_FLOOR */
     int32_t cako_enabled;  /* BSSN_CAKO_ENABLED */
-    int32_t pad_;
+    int32_t eta_func;      /* 1: the eta of the reference's BSSN_ETA_FUNCTION build (CodeGen/bssn.py:71-75) */
     double eps_cako_gauge; /* BSSN_EPSILON_CAKO_GAUGE */
     double eps_cako_other; /* BSSN_EPSILON_CAKO_OTHER */
+    double eta_r0;         /* BSSN_ETA_R0 */
+    double eta_power[2];   /* BSSN_ETA_POWER */
 } orc_params;
 
 /* One padded block, the arguments bssnrhs() receives (BSSN_GR/src/rhs.cpp:49-70). */
@@ -101,6 +103,9 @@ typedef struct orc_block {
  *             (in that order), components 00 01 02 11 12 22: 96 + 6*m + c
  *   [162]     eta                                                          */
 enum { ORC_POINT_NIN = 163 };
+/* The eta of the reference's BSSN_ETA_FUNCTION build at one point's algebra input
+ * (CodeGen/bssn.py:71-75): eta_r0 sqrt(gt^ij d_i chi d_j chi) / (1 - chi^p0)^p1. */
+double orc_eta_func(const double *in, const orc_params *p);
 
 /* ---- stencils: BSSN_GR/src/derivs.cpp ---------------------------------- */
 void orc_deriv644_x(double *Du, const double *u, double dx, const uint32_t *sz, uint32_t bflag);

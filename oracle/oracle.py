@@ -34,9 +34,11 @@ class Params(C.Structure):
         ("ko_sigma", C.c_double),
         ("chi_floor", C.c_double),
         ("cako_enabled", C.c_int32),
-        ("pad_", C.c_int32),
+        ("eta_func", C.c_int32),
         ("eps_cako_gauge", C.c_double),
         ("eps_cako_other", C.c_double),
+        ("eta_r0", C.c_double),
+        ("eta_power", C.c_double * 2),
     ]
 
 
@@ -69,6 +71,8 @@ def _configure(L):
     dp = C.POINTER(C.c_double)
     L.orc_default_params.argtypes = [C.POINTER(Params)]
     L.orc_bssn_points.argtypes = [dp, C.c_int64, C.POINTER(Params), dp]
+    L.orc_eta_func.argtypes = [dp, C.POINTER(Params)]
+    L.orc_eta_func.restype = C.c_double
     L.orc_bssnrhs_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                     C.POINTER(Params), C.c_int]
     L.orc_bssnrhs_batch.restype = C.c_int
@@ -152,6 +156,7 @@ def ref_lib():
         R = C.CDLL(path)
         dp = C.POINTER(C.c_double)
         R.ref_bssn_points.argtypes = [dp, C.c_int64, dp, dp, dp]
+        R.ref_bssn_points_etafunc.argtypes = [dp, C.c_int64, dp, dp, C.c_double, C.c_double, C.c_double, dp]
         R.ref_ham_points.argtypes = [dp, C.c_int64, dp]
         R.ref_constraint_points.argtypes = [dp, C.c_int64, dp, dp]
         R.ref_constraints_xyz.argtypes = [dp, dp, C.c_int64, dp, dp, dp]
@@ -199,6 +204,8 @@ def default_params(**kw) -> Params:
                 p.lambda_[i] = v[i]
         elif k == "lambda_f":
             p.lambda_f[0], p.lambda_f[1] = v
+        elif k == "eta_power":
+            p.eta_power[0], p.eta_power[1] = v
         else:
             setattr(p, k, v)
     return p
@@ -222,6 +229,26 @@ def ref_bssn_points(inp: np.ndarray, lam, lamf) -> np.ndarray:
     lf = np.asarray(lamf, dtype=np.float64)
     R.ref_bssn_points(_dp(inp), n, _dp(la), _dp(lf), _dp(out))
     return out
+
+
+def ref_bssn_points_etafunc(inp: np.ndarray, lam, lamf, r0: float, p0: float, p1: float) -> np.ndarray:
+    """The reference's equations of its BSSN_ETA_FUNCTION build
+    (bssneqs_eta_func_standard_gauge.cpp) per point; in[162] is not read."""
+    R = ref_lib()
+    assert R is not None, "oracle/_ref not built (needs the reference checkout)"
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    n = inp.shape[0]
+    out = np.zeros((n, 24))
+    la = np.asarray(lam, dtype=np.float64)
+    lf = np.asarray(lamf, dtype=np.float64)
+    R.ref_bssn_points_etafunc(_dp(inp), n, _dp(la), _dp(lf), r0, p0, p1, _dp(out))
+    return out
+
+
+def eta_func_points(inp: np.ndarray, p: Params) -> np.ndarray:
+    """orc_eta_func at each point of an algebra-input array."""
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    return np.array([lib().orc_eta_func(_dp(inp[q]), C.byref(p)) for q in range(inp.shape[0])])
 
 
 def ham_points(inp: np.ndarray) -> np.ndarray:

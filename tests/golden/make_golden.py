@@ -25,7 +25,13 @@ Fixtures:
                       of the same equations at per-point coordinates xyz
                       (200 x 3, uniform in [-8, 8]^3, seed 11)
 
-`python make_golden.py ham` writes ham_points.npz only.
+  algebra_points_etafunc.npz  for the same 200 inputs, the 24 outputs of the
+                      reference's equations of its BSSN_ETA_FUNCTION build
+                      (bssneqs_eta_func_standard_gauge.cpp, compiled in place)
+                      for the parameter sets of ETAFUNC_SETS
+
+`python make_golden.py ham` writes ham_points.npz only, `... etafunc`
+algebra_points_etafunc.npz only.
 """
 import ctypes as C
 import os
@@ -96,12 +102,29 @@ def write_ham():
                         inputs_sum=np.array([inp.sum(), np.abs(inp).sum()]))
 
 
+# (lambda, lambda_f, BSSN_ETA_R0, BSSN_ETA_POWER): the reference defaults, and
+# other values (integer powers: 1 - chi^p0 changes sign over the inputs)
+ETAFUNC_SETS = {"A": ((1, 1, 1, 1), (1.0, 0.0), 1.31, (2.0, 2.0)),
+                "B": ((2, 3, 5, 7), (0.7, 0.3), 0.9, (4.0, 1.0))}
+
+
+def write_etafunc():
+    inp = algebra_inputs()
+    outs = {k: O.ref_bssn_points_etafunc(inp, lam, lf, r0, *pw) for k, (lam, lf, r0, pw) in ETAFUNC_SETS.items()}
+    np.savez_compressed(os.path.join(HERE, "algebra_points_etafunc.npz"), out_A=outs["A"], out_B=outs["B"],
+                        inputs_sum=np.array([inp.sum(), np.abs(inp).sum()]))
+
+
 def main():
     assert O.ref_lib() is not None, "build oracle/_ref first (make -C oracle)"
     if sys.argv[1:] == ["ham"]:
         write_ham()
         return
+    if sys.argv[1:] == ["etafunc"]:
+        write_etafunc()
+        return
     write_ham()
+    write_etafunc()
     inp = algebra_inputs()
     outs = {k: O.ref_bssn_points(inp, lam, lf) for k, (lam, lf) in PARAM_SETS.items()}
     np.savez_compressed(os.path.join(HERE, "algebra_points.npz"), inputs=inp,

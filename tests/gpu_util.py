@@ -48,7 +48,8 @@ def oracle_params(p):
                             eta_central=p.eta_central, eta_outer=p.eta_outer,
                             eta_width=p.eta_width, ko_sigma=p.ko_sigma, chi_floor=p.chi_floor,
                             cako_enabled=p.cako_enabled, eps_cako_gauge=p.eps_cako_gauge,
-                            eps_cako_other=p.eps_cako_other)
+                            eps_cako_other=p.eps_cako_other, eta_func=p.eta_func, eta_r0=p.eta_r0,
+                            eta_power=list(p.eta_power))
 
 
 def interior_values(arr, blocks):

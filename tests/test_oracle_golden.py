@@ -39,6 +39,41 @@ def test_algebra_matches_reference_vectors(pset):
     assert err.max() < 1e-13, err
 
 
+ETAFUNC_SETS = {"A": ((1, 1, 1, 1), (1.0, 0.0), 1.31, (2.0, 2.0)),
+                "B": ((2, 3, 5, 7), (0.7, 0.3), 0.9, (4.0, 1.0))}
+
+
+@pytest.mark.parametrize("pset", ["A", "B"])
+def test_eta_function_algebra_matches_reference_vectors(pset):
+    """The reference's BSSN_ETA_FUNCTION build (bssneqs_eta_func_standard_gauge.cpp,
+    compiled in place; tests/golden/algebra_points_etafunc.npz): the oracle's
+    eta of chi's gradient (orc_eta_func) fed to its point algebra gives the
+    reference's 24 outputs to 1e-13; only B's three depend on it, and the same
+    eta with the first power raised by one misses them."""
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    ge = np.load(os.path.join(GOLD, "algebra_points_etafunc.npz"))
+    lam, lf, r0, pw = ETAFUNC_SETS[pset]
+    p = O.default_params(lambda_=lam, lambda_f=lf, eta_func=1, eta_r0=r0, eta_power=pw)
+    inp = g["inputs"].copy()
+    inp[:, 162] = O.eta_func_points(inp, p)
+    got = O.bssn_points(inp, p)
+    err = normwise(got, ge[f"out_{pset}"], axis=0)
+    assert err.max() < 1e-13, err
+    pq = O.default_params(lambda_=lam, lambda_f=lf, eta_func=1, eta_r0=r0, eta_power=(pw[0] + 1, pw[1]))
+    inp[:, 162] = O.eta_func_points(inp, pq)
+    bad = normwise(O.bssn_points(inp, pq), ge[f"out_{pset}"], axis=0)
+    assert bad[9:12].min() > 1e-6 and np.delete(bad, [9, 10, 11]).max() < 1e-13
+
+
+def test_eta_function_fixture_still_matches_reference_build():
+    if O.ref_lib() is None:
+        pytest.skip("reference checkout not present")
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    ge = np.load(os.path.join(GOLD, "algebra_points_etafunc.npz"))
+    for k, (lam, lf, r0, pw) in ETAFUNC_SETS.items():
+        np.testing.assert_array_equal(O.ref_bssn_points_etafunc(g["inputs"], lam, lf, r0, *pw), ge[f"out_{k}"])
+
+
 def test_algebra_fixture_still_matches_reference_build():
     if O.ref_lib() is None:
         pytest.skip("reference checkout not present")

@@ -84,6 +84,16 @@ def test_rk_type_is_read():
         assert parfile.read_param_file(os.path.join(PARS, name)).rk_type == 1
 
 
+def test_eta_function_constants_are_read():
+    """BSSN_ETA_R0 and BSSN_ETA_POWER (JSON object of _1/_2, TOML array;
+    grUtils.cpp:205-207, parameters.cpp:329-331): 1.31 and (2, 2) in the
+    reference's files; the eta function itself stays off (a build option
+    there)."""
+    for name in ("q1.tinytest.par.json", "q1.tinytest.par.toml", "q2.par.toml"):
+        p = parfile.read_param_file(os.path.join(PARS, name)).params
+        assert (p.eta_r0, p.eta_power[0], p.eta_power[1], p.eta_func) == (1.31, 2.0, 2.0, 0)
+
+
 def test_rk45_keys_are_read():
     """BSSN_RK45_TIME_STEP_SIZE and BSSN_RK45_DESIRED_TOL (parameters.cpp:
     220-223) as the reference's files hold them: 0.01 and 1e-3."""
