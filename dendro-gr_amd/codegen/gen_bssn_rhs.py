@@ -15,8 +15,11 @@ The emitted function reads inputs through an accessor object
 and hands each right-hand side to `out.template put<V>(value)`, so a kernel
 decides where inputs come from and what happens to each output.
 
-Usage: python gen_bssn_rhs.py [--cse=local|global] [--phased | --ham | --cons-phased [--search]] [output.h]
+Usage: python gen_bssn_rhs.py [--cse=local|global] [--phased | --ham | --cons-phased [--search]] [--gauge]
+       [output.h]
 (--phased: the right-hand side in the phase order of the fused kernel;
+ --gauge: with or without --phased, the right-hand side with the gauge variants
+ of the reference's other builds, the Rochester gauge and the slow-start lapse;
  --ham: the constraints (Hamiltonian, momentum, psi4);
  --cons-phased: the constraints in the phase order of the fused constraint
  kernel, --search to look for that order again)
@@ -196,6 +199,29 @@ for i, j in PAIRS:
     OUTPUTS[VIDX["gt0"] + SYM6[i][j]] = gt_rhs[i, j]
     OUTPUTS[VIDX["At0"] + SYM6[i][j]] = At_rhs[i, j]
 
+# The gauge variants of the reference's other generated builds (rhs.cpp:219-265),
+# chosen at run time:
+# * the Rochester puncture gauge (bssn.py:415-455, bssn_rochester_puncture_gauge):
+#   b_rhs^i = xi_2 beta^j d_j beta^i + 3/4 xi_3 Gt^i - eta beta^i and B frozen
+#   (B_rhs = 0; the generated bssneqs_*_rochester_gauge*.cpp write no B_rhs at
+#   all); xi_2, xi_3 are BSSN_XI[1], BSSN_XI[2]. SEL(flag, x, y) prints as a
+#   select, so the standard gauge's values do not see the Rochester terms (an
+#   infinite eta times 0 would otherwise turn them into NaN);
+# * the slow-start lapse (bssn.py:90-99, sslGaugeCondition): a_rhs gains
+#   -sslt with sslt = 0.6 sqrt(chi) (alpha - sqrt(chi)) exp(-t^2/800), which
+#   the caller forms (0 without the term).
+xi1, xi2 = sp.symbols("xi1 xi2", real=True)
+rochester = sp.Symbol("rochester", real=True)
+sslt = sp.Symbol("sslt", real=True)
+SEL = sp.Function("SEL")
+OUTPUTS_GAUGE = list(OUTPUTS)
+OUTPUTS_GAUGE[VIDX["alpha"]] = a_rhs - sslt
+for i in E_I:
+    b_roch = (xi1 * sum(b[j] * d(j, b_n[i]) for j in E_I) + Rational(3, 4) * xi2 * fld[Gt_n[i]]
+              - eta * b[i])
+    OUTPUTS_GAUGE[VIDX[f"beta{i}"]] = SEL(rochester, b_roch, b_rhs[i])
+    OUTPUTS_GAUGE[VIDX[f"B{i}"]] = SEL(rochester, sp.Integer(0), B_rhs[i])
+
 
 # Hamiltonian constraint (CodeGen/constraints.py:250-254; dendro.sqr, dendro.py:309-318):
 # chi igt^jk R_jk - At_ij At^ij + 2/3 K^2, with the same R as the right-hand side.
@@ -284,7 +310,7 @@ def input_expr(sym: sp.Symbol) -> str:
 
 
 PARAMS = {"eta": "eta", "lam0": "lam[0]", "lam1": "lam[1]", "lam2": "lam[2]", "lam3": "lam[3]",
-          "lamf0": "lamf[0]", "lamf1": "lamf[1]"}
+          "lamf0": "lamf[0]", "lamf1": "lamf[1]", "xi1": "xi[1]", "xi2": "xi[2]", "sslt": "sslt"}
 
 
 class Printer(C99CodePrinter):
@@ -298,6 +324,16 @@ class Printer(C99CodePrinter):
         if s.name in PARAMS:
             return PARAMS[s.name]
         raise KeyError(s.name)
+
+    def _print_Function(self, e):
+        if e.func.__name__ != "SEL":
+            return super()._print_Function(e)
+        flag, x, y = e.args
+        assert flag == rochester
+        return f"(rochester ? {self._print_sel_arm(x)} : {self._print_sel_arm(y)})"
+
+    def _print_sel_arm(self, x):
+        return "T(0.0)" if x == 0 else f"({self._print(x)})"
 
     def _print_Rational(self, r):
         return f"T({r.p}.0/{r.q}.0)" if r.q != 1 else f"T({r.p}.0)"
@@ -462,12 +498,23 @@ def emit(order, defs, outs_r, input_fn, phase_line=None, inline_fields=False, pi
     return lines, ntmp, max_live
 
 
-def generate(path):
+GAUGE_DOC = """//
+// Gauge variants (rhs.cpp:219-265), chosen at run time: with rochester != 0
+// the Rochester puncture gauge (CodeGen/bssn.py:415-455; xi = BSSN_XI), and
+// sslt, the slow-start lapse term the caller forms (CodeGen/bssn.py:90-99:
+// 0.6 sqrt(chi) (alpha - sqrt(chi)) exp(-t^2/800), or 0), subtracted from
+// alpha's right-hand side.
+"""
+
+
+def generate(path, gauge=False):
     t0 = time.time()
-    defs, outs_r, ops_after = eliminate()
+    defs, outs_r, ops_after = eliminate(OUTPUTS_GAUGE if gauge else None)
     order = schedule(defs, outs_r)
     lines, ntmp, max_live = emit(order, defs, outs_r, input_expr)
-    hdr = HEADER_TOP + f"""// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values
+    fn, extra = ("bssn_rhs_point_gauge", "const T *xi, const int rochester, const T sslt, ") if gauge \
+        else ("bssn_rhs_point", "")
+    hdr = HEADER_TOP + (GAUGE_DOC if gauge else "") + f"""// Operation count: {ops_after} ({ntmp} temporaries); at most {max_live} values
 // live at once in the emitted order.
 //
 // In: accessor with fld<V>() (24 fields, VAR order of grDef.h), d1<V, D>()
@@ -489,7 +536,7 @@ def generate(path):
 namespace dgr {{
 
 template <typename T, typename In, typename Out>
-DGR_HD DGR_INLINE void bssn_rhs_point(const In &in, const T eta, const T *lam, const T *lamf, Out &out) {{
+DGR_HD DGR_INLINE void {fn}(const In &in, const T eta, const T *lam, const T *lamf, {extra}Out &out) {{
 {chr(10).join(lines)}
 }}
 
@@ -602,9 +649,9 @@ def schedule_phased(defs, outs, order_vars):
     return prog
 
 
-def generate_phased(path):
+def generate_phased(path, gauge=False):
     t0 = time.time()
-    defs, outs_r, ops_after = eliminate()
+    defs, outs_r, ops_after = eliminate(OUTPUTS_GAUGE if gauge else None)
     prog = schedule_phased(defs, outs_r, PHASE_ORDER)
     vidx = [VIDX[v] for v in PHASE_ORDER]
 
@@ -633,7 +680,15 @@ def generate_phased(path):
             live.add(x)
         max_live = max(max_live, len(live))
         live = {y for y in live if last_use.get(y, -1) > k}
-    hdr = HEADER_TOP + f"""//
+    fn, extra = ("bssn_rhs_phased_gauge", "const T *xi, const int rochester, const T &sslt, ") if gauge \
+        else ("bssn_rhs_phased", "")
+    consts = "" if gauge else f"""
+// first variable of the phase order (the kernel fetches it before calling)
+constexpr int kBssnFirstPhaseVar = {vidx[0]};
+// the whole phase order (variable of phase K)
+constexpr int kBssnPhaseVar[{len(vidx)}] = {{{', '.join(str(v) for v in vidx)}}};
+"""
+    hdr = HEADER_TOP + (GAUGE_DOC + "// sslt is read by reference, like eta.\n" if gauge else "") + f"""//
 // Phased form for a kernel that streams the variables through LDS one at a
 // time, in the order {', '.join(PHASE_ORDER)}.
 // ctx.phase<V, NEXT, K>() (K: position of V in the order) makes variable V's
@@ -661,14 +716,9 @@ def generate_phased(path):
 #endif
 
 namespace dgr {{
-
-// first variable of the phase order (the kernel fetches it before calling)
-constexpr int kBssnFirstPhaseVar = {vidx[0]};
-// the whole phase order (variable of phase K)
-constexpr int kBssnPhaseVar[{len(vidx)}] = {{{', '.join(str(v) for v in vidx)}}};
-
+{consts}
 template <typename T, typename Ctx, typename Out>
-DGR_HD DGR_INLINE void bssn_rhs_phased(Ctx &ctx, const T &eta, const T *lam, const T *lamf, Out &out) {{
+DGR_HD DGR_INLINE void {fn}(Ctx &ctx, const T &eta, const T *lam, const T *lamf, {extra}Out &out) {{
 {chr(10).join(lines)}
 }}
 
@@ -845,11 +895,12 @@ if __name__ == "__main__":
     for a in sys.argv[1:]:
         if a.startswith("--cse="):
             CSE_MODE = a.split("=", 1)[1]
+    gauge = "--gauge" in sys.argv
     if "--phased" in sys.argv:
-        generate_phased(args[0] if args else "bssn_rhs_phased.h")
+        generate_phased(args[0] if args else "bssn_rhs_phased.h", gauge)
     elif "--cons-phased" in sys.argv:
         generate_cons_phased(args[0] if args else "bssn_cons_phased.h", search="--search" in sys.argv)
     elif "--ham" in sys.argv:
         generate_ham(args[0] if args else "bssn_ham_generated.h")
     else:
-        generate(args[0] if args else "bssn_rhs_generated.h")
+        generate(args[0] if args else "bssn_rhs_generated.h", gauge)

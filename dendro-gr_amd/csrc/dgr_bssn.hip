@@ -45,6 +45,8 @@ __host__ __device__ inline T dgr_sqrt(T x) {
 #include "bssn_rhs_generated.h"
 #include "stencils.h"
 #include "bssn_rhs_phased.h"
+#include "bssn_rhs_gauge_generated.h"
+#include "bssn_rhs_phased_gauge.h"
 #include "bssn_cons_phased.h"
 #include "bssn_ham_generated.h"
 
@@ -112,7 +114,24 @@ struct RhsParams {
     double eps_gauge, eps_other;
     int eta_func;                    // eta from chi's gradient (BSSN_ETA_FUNCTION), else RIT eta(r)
     double eta_r0, eta_p0, eta_p1;
+    int rochester;                   // the Rochester puncture gauge (USE_ROCHESTER_GAUGE)
+    int ssl;                         // the slow-start lapse term (BSSN_ENABLE_SSL_IN_RHS)
+    double xi[3];                    // BSSN_XI (the Rochester gauge's shift coefficients)
+    double ssl_e;                    // exp(-t^2/800) at the time the call evaluates at
 };
+
+// The right-hand side of one of the reference's other builds (eta function,
+// Rochester gauge, slow-start lapse): its own kernels run it.
+inline bool gauge_variant(const RhsParams &prm) { return prm.eta_func || prm.rochester || prm.ssl; }
+
+// The slow-start lapse term 0.6 sqrt(chi) (alpha - sqrt(chi)) exp(-t^2/800)
+// (CodeGen/bssn.py:90-99), in the factor order of the generated
+// bssneqs_*_SSL.cpp; e = exp(-t^2/800).
+template <typename T>
+__device__ __forceinline__ T ssl_term(T alpha, T chi, double e) {
+    const T w = sqrt(chi);
+    return T(0.6) * w * (alpha - w) * T(e);
+}
 
 // -------------------------------------------------------------------------
 // k_derivs: one workgroup per (tile, variable)
@@ -451,7 +470,12 @@ __global__ __launch_bounds__(NTHREADS) void k_rhs(VarPtrs u, VarPtrs rhs, RKArgs
     out.P = P;
     out.q = q;
     ko_sigmas(prm, u, g.idx, out.sig_gauge, out.sig_other);
-    dgr::bssn_rhs_point<double>(in, eta, prm.lam, prm.lamf, out);
+    if (prm.rochester || prm.ssl) {
+        const double sslt = prm.ssl ? ssl_term<double>(in.fld<0>(), in.fld<1>(), prm.ssl_e) : 0.0;
+        dgr::bssn_rhs_point_gauge<double>(in, eta, prm.lam, prm.lamf, prm.xi, prm.rochester, sslt, out);
+    } else {
+        dgr::bssn_rhs_point<double>(in, eta, prm.lam, prm.lamf, out);
+    }
     out.finish(prm.chi_floor);
 }
 
@@ -631,19 +655,20 @@ struct StageIO {
 struct RhsPhases {
     static constexpr int N = NV;
     static constexpr bool KO = true;
-    static constexpr bool ETAF = false;  // eta: the RIT eta(r)
+    static constexpr bool GAUGE = false;  // the standard gauge with the RIT eta(r)
     static constexpr int var(int k) { return dgr::kBssnPhaseVar[k]; }
 };
-// The right-hand side of the reference's BSSN_ETA_FUNCTION build: eta from
-// chi's gradient, formed in chi's phase (its own kernels, so the default ones
-// carry none of its registers)
-struct RhsPhasesEtaF : RhsPhases {
-    static constexpr bool ETAF = true;
+// The right-hand side of the reference's other builds, chosen at run time:
+// BSSN_ETA_FUNCTION (eta from chi's gradient, formed in chi's phase), the
+// Rochester gauge, the slow-start lapse (its term formed in alpha's phase).
+// Its own kernels, so the default ones carry none of its registers.
+struct RhsPhasesGauge : RhsPhases {
+    static constexpr bool GAUGE = true;
 };
 struct ConsPhases {
     static constexpr int N = dgr::kConsPhases;
     static constexpr bool KO = false;
-    static constexpr bool ETAF = false;
+    static constexpr bool GAUGE = false;
     static constexpr int var(int k) { return dgr::kConsPhaseVar[k]; }
 };
 
@@ -680,10 +705,13 @@ struct FusedCtx {
     double poff[PD];       // the staging offset of each fetched variable (mixed precision)
     T f[NV];               // this point's field values
     // the eta the equations read (by reference, after the last phase): the
-    // RIT eta(r), or with P::ETAF the eta of chi's gradient (BSSN_ETA_FUNCTION)
-    // set in chi's phase
+    // RIT eta(r), or with P::GAUGE and eta_fn the eta of chi's gradient
+    // (BSSN_ETA_FUNCTION) set in chi's phase
     T eta_v;
     double eta_r0, eta_p0, eta_p1;
+    bool eta_fn, ssl_on;
+    T ssl_v;       // P::GAUGE: the slow-start lapse term (0 without it), set in alpha's phase
+    double ssl_e;  // exp(-t^2/800)
     // LDS source offsets of this thread's D_x / D_y evaluations
     int dxw[DXIT], dyw[DYIT];  // where this thread's D_x / D_y evaluations go (see dtable)
     T d1v[3], d2v[6];
@@ -893,8 +921,12 @@ struct FusedCtx {
 #pragma unroll
             for (int k = 0; k < 6; k++) DGR_PIN(d2v[k]);
         }
-        if constexpr (V == 1 && P::ETAF)  // chi: its first derivatives are at hand
-            eta_v = eta_function<T>(f + 12, d1v, f[1], eta_r0, eta_p0, eta_p1);
+        if constexpr (V == 1 && P::GAUGE) {  // chi: its first derivatives are at hand
+            if (eta_fn) eta_v = eta_function<T>(f + 12, d1v, f[1], eta_r0, eta_p0, eta_p1);
+        }
+        if constexpr (V == 0 && P::GAUGE) {
+            if (ssl_on) ssl_v = ssl_term<T>(f[0], f[1], ssl_e);
+        }
         // on a flagged face the boundary value needs the first derivatives
         if constexpr (P::KO && (m2 < 0 || FACE)) ko_term<V, S>(c, kx, ky, kz);
         if constexpr (!DEEP || m2 < 0) {
@@ -1148,8 +1180,9 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
             ctx.sig_gauge = ctx.sig_other = T(prm.ko_sigma);
         }
     }
-    // eta: the RIT eta(r), or (P::ETAF) set in chi's phase from chi's gradient
-    if constexpr (P::ETAF)
+    // eta: the RIT eta(r), or (P::GAUGE with eta_func) set in chi's phase from
+    // chi's gradient
+    if (P::GAUGE && prm.eta_func)
         ctx.eta_v = T(0);
     else
         ctx.eta_v = T(rit_eta(coord(B.pmin[0], ctx.ax, B.h[0]), coord(B.pmin[1], ctx.ay, B.h[1]),
@@ -1157,6 +1190,12 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     ctx.eta_r0 = prm.eta_r0;
     ctx.eta_p0 = prm.eta_p0;
     ctx.eta_p1 = prm.eta_p1;
+    if constexpr (P::GAUGE) {
+        ctx.eta_fn = prm.eta_func != 0;
+        ctx.ssl_on = prm.ssl != 0;
+        ctx.ssl_e = prm.ssl_e;
+        ctx.ssl_v = T(0);
+    }
     const T lam[4] = {T(prm.lam[0]), T(prm.lam[1]), T(prm.lam[2]), T(prm.lam[3])};
     const T lamf[2] = {T(prm.lamf[0]), T(prm.lamf[1])};
     ctx.template fetch<0>();
@@ -1165,7 +1204,12 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     static_assert(C::PD <= 3, "prefetch distance");
     ctx.begin();
     FusedOut<FACE, C, T, HALO, P> out(ctx);
-    dgr::bssn_rhs_phased<T>(ctx, ctx.eta_v, lam, lamf, out);
+    if constexpr (P::GAUGE) {
+        const T xi[3] = {T(prm.xi[0]), T(prm.xi[1]), T(prm.xi[2])};
+        dgr::bssn_rhs_phased_gauge<T>(ctx, ctx.eta_v, lam, lamf, xi, prm.rochester, ctx.ssl_v, out);
+    } else {
+        dgr::bssn_rhs_phased<T>(ctx, ctx.eta_v, lam, lamf, out);
+    }
     if (active) fused_epilogue<C, T>(io, s_ko, ctx.f, ctx.tid, idx, q, prm.chi_floor);
 }
 
@@ -1176,7 +1220,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
 #define DGR_FP32_WAVES 2  // waves per SIMD the FP32 kernels are built for
 #endif
 
-template <bool FACE, class C, typename T, bool HALO = false, bool ETAF = false>
+template <bool FACE, class C, typename T, bool HALO = false, bool GAUGE = false>
 __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES : 1) void k_fused(
     const StageIO io, const RhsParams prm, const DBlock *__restrict__ blocks,
     const dgr::AxisConsts *__restrict__ bconst, const Tile *__restrict__ tiles,
@@ -1191,7 +1235,7 @@ __global__ __launch_bounds__(C::FT, sizeof(T) < sizeof(double) ? DGR_FP32_WAVES 
 
     const Tile tl = tiles[blockIdx.x];
     const DBlock B = blocks[tl.block];
-    fused_tile<FACE, C, T, HALO, std::conditional_t<ETAF, RhsPhasesEtaF, RhsPhases>>(
+    fused_tile<FACE, C, T, HALO, std::conditional_t<GAUGE, RhsPhasesGauge, RhsPhases>>(
         io, prm, tl, B, bconst + 3 * (size_t)tl.block, tabs, dtab, s_tile, s_dx, s_dy, s_ko);
 }
 
@@ -2311,6 +2355,10 @@ void dgr_default_params(dgr_bssn_params *p) {
     p->eta_r0 = 1.31;  // parameters.cpp:59-60
     p->eta_power[0] = 2.0;
     p->eta_power[1] = 2.0;
+    p->rochester = 0;  // the reference's default build: standard gauge, no slow-start lapse
+    p->ssl = 0;
+    p->xi[0] = p->xi[1] = p->xi[2] = 0.0;  // parameters.cpp:157
+    p->time = 0.0;
 }
 
 const char *dgr_last_error(void) { return g_err.c_str(); }
@@ -2599,7 +2647,8 @@ int dgr_plan_kernel_times(dgr_plan *plan, double *ms, int64_t *launches) {
 
 namespace {
 
-RhsParams to_params(const dgr_bssn_params *p) {
+// t: the time the slow-start lapse term reads
+RhsParams to_params(const dgr_bssn_params *p, double t) {
     RhsParams r;
     for (int i = 0; i < 4; i++) r.lam[i] = p->lambda[i];
     r.lamf[0] = p->lambda_f[0];
@@ -2616,8 +2665,15 @@ RhsParams to_params(const dgr_bssn_params *p) {
     r.eta_r0 = p->eta_r0;
     r.eta_p0 = p->eta_power[0];
     r.eta_p1 = p->eta_power[1];
+    r.rochester = p->rochester != 0;
+    r.ssl = p->ssl != 0;
+    for (int i = 0; i < 3; i++) r.xi[i] = p->xi[i];
+    r.ssl_e = r.ssl ? std::exp(-1.0 / 800.0 * (t * t)) : 0.0;
     return r;
 }
+
+// dgr_rk*_step(s): every stage reads the step's start time, params->time
+RhsParams to_params(const dgr_bssn_params *p) { return to_params(p, p->time); }
 
 int load_ptrs(VarPtrs &vp, const double *const *src, const char *what) {
     if (!src) return fail(DGR_E_INVALID, std::string(what) + " is null");
@@ -2678,12 +2734,12 @@ int run_chunks(dgr_plan *plan, const VarPtrs &state, const VarPtrs &rhs, const R
 
 // Address of a fused kernel: the FP64 ones live in this unit, the FP32 ones
 // in the unit compiled with DGR_F32_KERNELS (see there).
-template <bool FACE, class C, typename T, bool HALO, bool ETAF = false>
+template <bool FACE, class C, typename T, bool HALO, bool GAUGE = false>
 const void *fused_kernel() {
     if constexpr (sizeof(T) == sizeof(double)) {
-        return reinterpret_cast<const void *>(&k_fused<FACE, C, T, HALO, ETAF>);
+        return reinterpret_cast<const void *>(&k_fused<FACE, C, T, HALO, GAUGE>);
     } else {
-        static_assert(!ETAF, "the eta function is FP64 only");
+        static_assert(!GAUGE, "the gauge variants are FP64 only");
         static const void *const *tab = [] {
             static const void *k[8];
             dgr_internal_fused_f32_kernels(k);
@@ -2697,7 +2753,7 @@ const void *fused_kernel() {
     }
 }
 
-template <bool FACE, class C, typename T, bool HALO = false, bool ETAF = false>
+template <bool FACE, class C, typename T, bool HALO = false, bool GAUGE = false>
 void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, int dtab, const StageIO &io,
                   const RhsParams &prm, hipStream_t st) {
     const int n = fg[group + 1] - fg[group];
@@ -2712,7 +2768,7 @@ void launch_fused(dgr_plan *plan, const Tile *tiles, const int *fg, int group, i
     int a_dtab = dtab;
     void *args[] = {&a_io, &a_prm, &a_blocks, &a_bconst, &a_tiles, &a_tabs, &a_dtab};
     timed(plan, 3 + group, st, [&] {
-        (void)hipLaunchKernel(fused_kernel<FACE, C, T, HALO, ETAF>(), dim3(n), dim3(C::FT), args, 0, st);
+        (void)hipLaunchKernel(fused_kernel<FACE, C, T, HALO, GAUGE>(), dim3(n), dim3(C::FT), args, 0, st);
     });
 }
 
@@ -2721,7 +2777,7 @@ template <typename T>
 void launch_fused_all(dgr_plan *plan, const Tile *tiles, const int *fg, const StageIO &io, const RhsParams &prm,
                       hipStream_t st) {
     if constexpr (sizeof(T) == sizeof(double)) {
-        if (prm.eta_func) {  // the reference's BSSN_ETA_FUNCTION build (FP64 only)
+        if (gauge_variant(prm)) {  // the reference's other builds (FP64 only)
             if (io.nbo) {
                 launch_fused<false, CfgWide, T, true, true>(plan, tiles, fg, FG_WIDE, plan->dtab_wide, io, prm, st);
                 launch_fused<false, CfgThin, T, true, true>(plan, tiles, fg, FG_THIN, plan->dtab_thin, io, prm, st);
@@ -2750,8 +2806,9 @@ void launch_fused_all(dgr_plan *plan, const Tile *tiles, const int *fg, const St
 }
 
 int run_fused(dgr_plan *plan, const StageIO &io, const RhsParams &prm, hipStream_t st) {
-    if (plan->precision == DGR_PRECISION_MIXED && prm.eta_func)
-        return fail(DGR_E_INVALID, "the eta function (eta_func, BSSN_ETA_FUNCTION) needs FP64 precision");
+    if (plan->precision == DGR_PRECISION_MIXED && gauge_variant(prm))
+        return fail(DGR_E_INVALID, "the eta function, the Rochester gauge and the slow-start lapse "
+                                   "(eta_func, rochester, ssl) need FP64 precision");
     if (plan->precision == DGR_PRECISION_MIXED)
         launch_fused_all<float>(plan, plan->d_ftiles[1], plan->fgroup[1], io, prm, st);
     else
@@ -2766,7 +2823,6 @@ extern "C" {
 
 int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u, double t,
                  const dgr_bssn_params *params, void *stream) {
-    (void)t;
     g_err.clear();
     if (!plan || !params) return fail(DGR_E_INVALID, "null plan or params");
     VarPtrs u, r;
@@ -2779,10 +2835,10 @@ int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
         io.rhs = r;
         io.stage = -1;
         io.err = plan->d_err;
-        return run_fused(plan, io, to_params(params), (hipStream_t)stream);
+        return run_fused(plan, io, to_params(params, t), (hipStream_t)stream);
     }
     RKArgs rk{};
-    return run_chunks<MODE_RHS>(plan, u, r, rk, to_params(params), (hipStream_t)stream);
+    return run_chunks<MODE_RHS>(plan, u, r, rk, to_params(params, t), (hipStream_t)stream);
 }
 
 int dgr_plan_autotune(dgr_plan *plan, double *const *d_rhs, const double *const *d_u,
@@ -3427,19 +3483,24 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
     VarPtrs u;
     int rc;
     if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
-    // per-launch timing and the split path run the steps eagerly
-    if (!plan->fused || plan->timing) {
-        for (int i = 0; i < nsteps; i++)
-            if ((rc = dgr_rk4_step(plan, d_u, dt, params, enforce_per_stage, stream))) return rc;
+    // per-launch timing and the split path run the steps eagerly, and so does
+    // the slow-start lapse, whose time moves on by dt per step (rkBSSN.cpp:1087)
+    if (!plan->fused || plan->timing || params->ssl) {
+        dgr_bssn_params p = *params;
+        for (int i = 0; i < nsteps; i++, p.time += dt)
+            if ((rc = dgr_rk4_step(plan, d_u, dt, &p, enforce_per_stage, stream))) return rc;
         return DGR_OK;
     }
+    // without the slow-start lapse the time is not read: a new time keeps the graph
+    dgr_bssn_params kp = *params;
+    kp.time = 0.0;
     HIP_OK(hipSetDevice(plan->device));
     if ((rc = ensure_istage(plan))) return rc;
     auto &key = plan->rk4_key;
     const int enf = enforce_per_stage ? 1 : 0;
     bool same = plan->rk4_exec && key.dt == dt && key.enforce == enf && key.halo == plan->halo &&
                 key.fill_general == plan->fill_general &&
-                std::memcmp(&key.prm, params, sizeof(*params)) == 0;
+                std::memcmp(&key.prm, &kp, sizeof(kp)) == 0;
     for (int v = 0; same && v < NV; v++) same = key.u[v] == u.p[v];
     if (!same) {
         if (plan->rk4_exec) {
@@ -3464,7 +3525,7 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         }
         for (int v = 0; v < NV; v++) key.u[v] = u.p[v];
         key.dt = dt;
-        key.prm = *params;
+        key.prm = kp;
         key.enforce = enf;
         key.halo = plan->halo;
         key.fill_general = plan->fill_general;

@@ -46,6 +46,10 @@ class BssnParams(C.Structure):
         ("eps_cako_other", C.c_double),
         ("eta_r0", C.c_double),
         ("eta_power", C.c_double * 2),
+        ("rochester", C.c_int32),
+        ("ssl", C.c_int32),
+        ("xi", C.c_double * 3),
+        ("time", C.c_double),
     ]
 
 
@@ -115,6 +119,8 @@ def default_params(**kw) -> BssnParams:
             p.lambda_f[0], p.lambda_f[1] = v
         elif k == "eta_power":
             p.eta_power[0], p.eta_power[1] = v
+        elif k == "xi":
+            p.xi[0], p.xi[1], p.xi[2] = v
         else:
             setattr(p, k, v)
     return p

@@ -77,6 +77,19 @@ typedef struct dgr_bssn_params {
     double eps_cako_other;  /* BSSN_EPSILON_CAKO_OTHER */
     double eta_r0;          /* BSSN_ETA_R0 */
     double eta_power[2];    /* BSSN_ETA_POWER */
+    int32_t rochester;      /* 1: the Rochester puncture gauge of the reference's USE_ROCHESTER_GAUGE
+                               build (bssneqs_*_rochester_gauge*.cpp, CodeGen/bssn.py:415-455):
+                               beta_rhs^i = xi[1] beta^j d_j beta^i + 3/4 xi[2] Gt^i - eta beta^i,
+                               B frozen (B's right-hand side is 0 before Kreiss-Oliger; the
+                               reference's generated code leaves it unwritten) */
+    int32_t ssl;            /* 1: the slow-start lapse of the BSSN_ENABLE_SSL_IN_RHS build
+                               (bssneqs_*_SSL.cpp, CodeGen/bssn.py:90-99): alpha's right-hand side
+                               gains -0.6 sqrt(chi) (alpha - sqrt(chi)) exp(-t^2/800) */
+    double xi[3];           /* BSSN_XI (unsigned ints in the reference, parameters.cpp:157) */
+    double time;            /* the t the slow-start lapse reads in dgr_rk3_step, dgr_rk4_step(s)
+                               and dgr_rk45_step: the step's start time at every stage, as
+                               rkBSSN.cpp:694-836 passes current_t; dgr_rk4_steps adds dt per
+                               step. dgr_bssn_rhs reads its t argument instead. */
 } dgr_bssn_params;
 
 typedef struct dgr_plan dgr_plan;
@@ -180,8 +193,8 @@ int dgr_plan_kernel_times(dgr_plan *plan, double *ms, int64_t *launches);
  * (BSSN_GR/src/rhs.cpp:80-504): 6th-order derivatives, the BSSN equations,
  * the radiative boundary fix-up on flagged faces, Kreiss-Oliger dissipation.
  * d_rhs[v] receives interior points only; its padding is not written.
- * `t` is accepted for signature parity (the standard-gauge equations do not
- * read it). Asynchronous on `stream`. Replaces bssnRHS() (rhs.h:28-30) and
+ * `t` is the time of the slow-start lapse term (params->ssl; the other
+ * equations do not read it). Asynchronous on `stream`. Replaces bssnRHS() (rhs.h:28-30) and
  * device::bssnrhs3 (cuda/include/bssn_kernels.cuh:1526-1538). */
 int dgr_bssn_rhs(dgr_plan *plan, double *const *d_rhs, const double *const *d_u, double t,
                  const dgr_bssn_params *params, void *stream);

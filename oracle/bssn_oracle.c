@@ -205,6 +205,10 @@ void orc_default_params(orc_params *p) {
     p->eta_r0 = 1.31;         /* parameters.cpp:59-60 */
     p->eta_power[0] = 2.0;
     p->eta_power[1] = 2.0;
+    p->rochester = 0;         /* standard gauge, no slow-start lapse: the default build */
+    p->ssl = 0;
+    p->xi[0] = p->xi[1] = p->xi[2] = 0.0; /* parameters.cpp:157 */
+    p->time = 0.0;
 }
 
 static const int SYM6[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
@@ -504,12 +508,19 @@ static void point_algebra(const double *in, const orc_params *p, double *rhs, do
         adv_K += b[i] * dK[i];
     }
     rhs[ORC_ALPHA] = l1 * adv_a - 2.0 * alpha * K;
+    if (p->ssl) { /* slow-start lapse (bssn.py:90-99): - W h exp(-t^2/(2 sig^2)) (alpha - W), W = chi^(1/2) */
+        const double W = sqrt(chi), hs = 0.6, sig = 20.0;
+        rhs[ORC_ALPHA] -= W * (hs * exp(-(p->time * p->time) / (2.0 * sig * sig))) * (alpha - W);
+    }
 
-    /* b_rhs (bssn.py:102-105) */
+    /* b_rhs (bssn.py:102-105); the Rochester gauge's (bssn.py:443-450) */
     for (int i = 0; i < 3; i++) {
         double adv = 0.0;
         for (int j = 0; j < 3; j++) adv += b[j] * db[j][i];
-        rhs[ORC_BETA0 + i] = 0.75 * (lf0 + lf1 * alpha) * B[i] + l2 * adv;
+        if (p->rochester)
+            rhs[ORC_BETA0 + i] = p->xi[1] * adv + 0.75 * p->xi[2] * f[ORC_GT0 + i] - eta * b[i];
+        else
+            rhs[ORC_BETA0 + i] = 0.75 * (lf0 + lf1 * alpha) * B[i] + l2 * adv;
     }
 
     /* chi_rhs (bssn.py:109) */
@@ -559,7 +570,8 @@ static void point_algebra(const double *in, const orc_params *p, double *rhs, do
             advB += b[j] * dB[j][i];
             advGt += b[j] * dGt[j][i];
         }
-        rhs[ORC_B0 + i] = s - eta * B[i] + l3 * advB - l4 * advGt;
+        /* the Rochester gauge freezes B (bssn.py:466, B_rhs = 0) */
+        rhs[ORC_B0 + i] = p->rochester ? 0.0 : s - eta * B[i] + l3 * advB - l4 * advGt;
     }
 }
 

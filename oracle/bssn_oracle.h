@@ -85,6 +85,10 @@ typedef struct orc_params {
     double eps_cako_other; /* BSSN_EPSILON_CAKO_OTHER */
     double eta_r0;         /* BSSN_ETA_R0 */
     double eta_power[2];   /* BSSN_ETA_POWER */
+    int32_t rochester;     /* 1: the Rochester puncture gauge (USE_ROCHESTER_GAUGE, CodeGen/bssn.py:415-455) */
+    int32_t ssl;           /* 1: the slow-start lapse (BSSN_ENABLE_SSL_IN_RHS, CodeGen/bssn.py:90-99) */
+    double xi[3];          /* BSSN_XI (parameters.cpp:157) */
+    double time;           /* t of bssnrhs (rhs.cpp:82), read by the slow-start lapse only */
 } orc_params;
 
 /* One padded block, the arguments bssnrhs() receives (BSSN_GR/src/rhs.cpp:49-70). */

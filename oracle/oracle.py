@@ -39,6 +39,10 @@ class Params(C.Structure):
         ("eps_cako_other", C.c_double),
         ("eta_r0", C.c_double),
         ("eta_power", C.c_double * 2),
+        ("rochester", C.c_int32),
+        ("ssl", C.c_int32),
+        ("xi", C.c_double * 3),
+        ("time", C.c_double),
     ]
 
 
@@ -157,6 +161,8 @@ def ref_lib():
         dp = C.POINTER(C.c_double)
         R.ref_bssn_points.argtypes = [dp, C.c_int64, dp, dp, dp]
         R.ref_bssn_points_etafunc.argtypes = [dp, C.c_int64, dp, dp, C.c_double, C.c_double, C.c_double, dp]
+        R.ref_bssn_points_variant.argtypes = [dp, C.c_int64, dp, dp, C.c_double, C.c_double, C.c_double, dp,
+                                              C.c_double, C.c_int, dp]
         R.ref_ham_points.argtypes = [dp, C.c_int64, dp]
         R.ref_constraint_points.argtypes = [dp, C.c_int64, dp, dp]
         R.ref_constraints_xyz.argtypes = [dp, dp, C.c_int64, dp, dp, dp]
@@ -206,6 +212,8 @@ def default_params(**kw) -> Params:
             p.lambda_f[0], p.lambda_f[1] = v
         elif k == "eta_power":
             p.eta_power[0], p.eta_power[1] = v
+        elif k == "xi":
+            p.xi[0], p.xi[1], p.xi[2] = v
         else:
             setattr(p, k, v)
     return p
@@ -242,6 +250,27 @@ def ref_bssn_points_etafunc(inp: np.ndarray, lam, lamf, r0: float, p0: float, p1
     la = np.asarray(lam, dtype=np.float64)
     lf = np.asarray(lamf, dtype=np.float64)
     R.ref_bssn_points_etafunc(_dp(inp), n, _dp(la), _dp(lf), r0, p0, p1, _dp(out))
+    return out
+
+
+def ref_bssn_points_variant(inp: np.ndarray, p: Params) -> np.ndarray:
+    """The reference's equations of the build p selects (rhs.cpp:219-265:
+    eta function, Rochester gauge, slow-start lapse at p.time) per point; with
+    the eta function in[162] is not read. The Rochester builds write no B_rhs
+    (0 here)."""
+    R = ref_lib()
+    if R is None:
+        raise RuntimeError("reference library oracle/_ref/libdgr_ref.so not built")
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    n = inp.shape[0]
+    out = np.zeros((n, 24))
+    la = np.array(list(p.lambda_), dtype=np.float64)
+    lf = np.array(list(p.lambda_f), dtype=np.float64)
+    xi = np.array(list(p.xi), dtype=np.float64)
+    variant = (1 if p.eta_func else 0) | (2 if p.rochester else 0) | (4 if p.ssl else 0)
+    rc = R.ref_bssn_points_variant(_dp(inp), n, _dp(la), _dp(lf), p.eta_r0, p.eta_power[0], p.eta_power[1],
+                                   _dp(xi), p.time, variant, _dp(out))
+    assert rc == 0
     return out
 
 

@@ -119,6 +119,107 @@ extern "C" int ref_bssn_points_etafunc(const double *in, int64_t npts, const dou
     return 0;
 }
 
+// All eight builds of the reference's equations (rhs.cpp:219-265), chosen by
+// variant = eta_func (1) | Rochester gauge (2) | slow-start lapse (4), at the
+// time t; the Rochester builds write no B_rhs (left at 0 here).
+extern "C" int ref_bssn_points_variant(const double *in, int64_t npts, const double *lambda_d,
+                                       const double *lambda_f_in, double eta_r0, double eta_p0, double eta_p1,
+                                       const double *xi, double t, int variant, double *out) {
+    // the names the variants read (parameters.h:288-289, 340)
+    const double BSSN_ETA_R0 = eta_r0;
+    const double BSSN_ETA_POWER[2] = {eta_p0, eta_p1};
+    const unsigned int BSSN_XI[3] = {(unsigned)xi[0], (unsigned)xi[1], (unsigned)xi[2]};
+    if (variant < 0 || variant > 7) return -1;
+    std::vector<double> soa((size_t)NIN * (size_t)npts);
+    std::vector<double> res((size_t)24 * (size_t)npts, 0.0);
+    for (int64_t q = 0; q < npts; q++)
+        for (int c = 0; c < NIN; c++) soa[(size_t)c * npts + q] = in[(size_t)q * NIN + c];
+
+    const unsigned int lambda[4] = {(unsigned)lambda_d[0], (unsigned)lambda_d[1],
+                                    (unsigned)lambda_d[2], (unsigned)lambda_d[3]};
+    const double lambda_f[2] = {lambda_f_in[0], lambda_f_in[1]};
+
+    FIELD(alpha, 0) FIELD(chi, 1) FIELD(K, 2) FIELD(Gt0, 3) FIELD(Gt1, 4) FIELD(Gt2, 5)
+    FIELD(beta0, 6) FIELD(beta1, 7) FIELD(beta2, 8) FIELD(B0, 9) FIELD(B1, 10) FIELD(B2, 11)
+    FIELD(gt0, 12) FIELD(gt1, 13) FIELD(gt2, 14) FIELD(gt3, 15) FIELD(gt4, 16) FIELD(gt5, 17)
+    FIELD(At0, 18) FIELD(At1, 19) FIELD(At2, 20) FIELD(At3, 21) FIELD(At4, 22) FIELD(At5, 23)
+    GRAD(alpha, 0) GRAD(chi, 1) GRAD(K, 2) GRAD(Gt0, 3) GRAD(Gt1, 4) GRAD(Gt2, 5)
+    GRAD(beta0, 6) GRAD(beta1, 7) GRAD(beta2, 8) GRAD(B0, 9) GRAD(B1, 10) GRAD(B2, 11)
+    GRAD(gt0, 12) GRAD(gt1, 13) GRAD(gt2, 14) GRAD(gt3, 15) GRAD(gt4, 16) GRAD(gt5, 17)
+    GRAD(At0, 18) GRAD(At1, 19) GRAD(At2, 20) GRAD(At3, 21) GRAD(At4, 22) GRAD(At5, 23)
+    GRAD2(alpha, 0) GRAD2(chi, 1) GRAD2(beta0, 2) GRAD2(beta1, 3) GRAD2(beta2, 4)
+    GRAD2(gt0, 5) GRAD2(gt1, 6) GRAD2(gt2, 7) GRAD2(gt3, 8) GRAD2(gt4, 9) GRAD2(gt5, 10)
+    const double *const eta_arr = &soa[(size_t)162 * npts];
+    OUT(a_rhs, 0) OUT(chi_rhs, 1) OUT(K_rhs, 2) OUT(Gt_rhs0, 3) OUT(Gt_rhs1, 4) OUT(Gt_rhs2, 5)
+    OUT(b_rhs0, 6) OUT(b_rhs1, 7) OUT(b_rhs2, 8) OUT(B_rhs0, 9) OUT(B_rhs1, 10) OUT(B_rhs2, 11)
+    OUT(gt_rhs00, 12) OUT(gt_rhs01, 13) OUT(gt_rhs02, 14) OUT(gt_rhs11, 15) OUT(gt_rhs12, 16)
+    OUT(gt_rhs22, 17) OUT(At_rhs00, 18) OUT(At_rhs01, 19) OUT(At_rhs02, 20) OUT(At_rhs11, 21)
+    OUT(At_rhs12, 22) OUT(At_rhs22, 23)
+
+    switch (variant) {
+    case 0:
+        for (int64_t pp = 0; pp < npts; pp++) {
+            const double eta = eta_arr[pp];
+            (void)eta;
+#include "bssneqs_eta_const_standard_gauge.cpp"
+        }
+        break;
+    case 1:
+        for (int64_t pp = 0; pp < npts; pp++) {
+            const double eta = eta_arr[pp];
+            (void)eta;
+#include "bssneqs_eta_func_standard_gauge.cpp"
+        }
+        break;
+    case 2:
+        for (int64_t pp = 0; pp < npts; pp++) {
+            const double eta = eta_arr[pp];
+            (void)eta;
+#include "bssneqs_eta_const_rochester_gauge.cpp"
+        }
+        break;
+    case 3:
+        for (int64_t pp = 0; pp < npts; pp++) {
+            const double eta = eta_arr[pp];
+            (void)eta;
+#include "bssneqs_eta_func_rochester_gauge.cpp"
+        }
+        break;
+    case 4:
+        for (int64_t pp = 0; pp < npts; pp++) {
+            const double eta = eta_arr[pp];
+            (void)eta;
+#include "bssneqs_eta_const_standard_gauge_SSL.cpp"
+        }
+        break;
+    case 5:
+        for (int64_t pp = 0; pp < npts; pp++) {
+            const double eta = eta_arr[pp];
+            (void)eta;
+#include "bssneqs_eta_func_standard_gauge_SSL.cpp"
+        }
+        break;
+    case 6:
+        for (int64_t pp = 0; pp < npts; pp++) {
+            const double eta = eta_arr[pp];
+            (void)eta;
+#include "bssneqs_eta_const_rochester_gauge_SSL.cpp"
+        }
+        break;
+    case 7:
+        for (int64_t pp = 0; pp < npts; pp++) {
+            const double eta = eta_arr[pp];
+            (void)eta;
+#include "bssneqs_eta_func_rochester_gauge_SSL.cpp"
+        }
+        break;
+    }
+
+    for (int64_t q = 0; q < npts; q++)
+        for (int v = 0; v < 24; v++) out[(size_t)q * 24 + v] = res[(size_t)v * npts + q];
+    return 0;
+}
+
 // Single point with the oracle's parameter struct, whose first six doubles are
 // lambda[4] and lambda_f[2]; usable as the oracle's orc_point_fn hook.
 extern "C" void ref_bssn_point_orc(const double *in, const void *params, double *out) {

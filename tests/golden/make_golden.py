@@ -29,9 +29,13 @@ Fixtures:
                       reference's equations of its BSSN_ETA_FUNCTION build
                       (bssneqs_eta_func_standard_gauge.cpp, compiled in place)
                       for the parameter sets of ETAFUNC_SETS
+  algebra_points_gauge.npz  for the same 200 inputs, the 24 outputs of the
+                      reference's Rochester-gauge and slow-start-lapse builds
+                      (bssneqs_eta_{const,func}_{rochester,standard}_gauge[_SSL].cpp,
+                      compiled in place) for the six cases of GAUGE_SETS
 
 `python make_golden.py ham` writes ham_points.npz only, `... etafunc`
-algebra_points_etafunc.npz only.
+algebra_points_etafunc.npz only, `... gauge` algebra_points_gauge.npz only.
 """
 import ctypes as C
 import os
@@ -115,6 +119,27 @@ def write_etafunc():
                         inputs_sum=np.array([inp.sum(), np.abs(inp).sum()]))
 
 
+# The reference's other builds (rhs.cpp:219-265): name -> (eta_func, rochester,
+# ssl, time), with lambda (2, 3, 5, 7), lambda_f (0.7, 0.3), BSSN_ETA_R0 0.9,
+# BSSN_ETA_POWER (4, 1) and BSSN_XI (2, 3, 5) (distinct, so a swapped xi shows)
+GAUGE_SETS = {"rochester": (0, 1, 0, 0.0), "rochester_etafunc": (1, 1, 0, 0.0),
+              "ssl": (0, 0, 1, 7.5), "ssl_etafunc": (1, 0, 1, 7.5),
+              "rochester_ssl": (0, 1, 1, 31.0), "rochester_ssl_etafunc": (1, 1, 1, 31.0)}
+
+
+def gauge_params(name):
+    ef, ro, ssl, t = GAUGE_SETS[name]
+    return O.default_params(lambda_=(2, 3, 5, 7), lambda_f=(0.7, 0.3), eta_func=ef, eta_r0=0.9,
+                            eta_power=(4.0, 1.0), rochester=ro, ssl=ssl, xi=(2, 3, 5), time=t)
+
+
+def write_gauge():
+    inp = algebra_inputs()
+    outs = {f"out_{k}": O.ref_bssn_points_variant(inp, gauge_params(k)) for k in GAUGE_SETS}
+    np.savez_compressed(os.path.join(HERE, "algebra_points_gauge.npz"), **outs,
+                        inputs_sum=np.array([inp.sum(), np.abs(inp).sum()]))
+
+
 def main():
     assert O.ref_lib() is not None, "build oracle/_ref first (make -C oracle)"
     if sys.argv[1:] == ["ham"]:
@@ -123,8 +148,12 @@ def main():
     if sys.argv[1:] == ["etafunc"]:
         write_etafunc()
         return
+    if sys.argv[1:] == ["gauge"]:
+        write_gauge()
+        return
     write_ham()
     write_etafunc()
+    write_gauge()
     inp = algebra_inputs()
     outs = {k: O.ref_bssn_points(inp, lam, lf) for k, (lam, lf) in PARAM_SETS.items()}
     np.savez_compressed(os.path.join(HERE, "algebra_points.npz"), inputs=inp,

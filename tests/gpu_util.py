@@ -49,7 +49,8 @@ def oracle_params(p):
                             eta_width=p.eta_width, ko_sigma=p.ko_sigma, chi_floor=p.chi_floor,
                             cako_enabled=p.cako_enabled, eps_cako_gauge=p.eps_cako_gauge,
                             eps_cako_other=p.eps_cako_other, eta_func=p.eta_func, eta_r0=p.eta_r0,
-                            eta_power=list(p.eta_power))
+                            eta_power=list(p.eta_power), rochester=p.rochester, ssl=p.ssl,
+                            xi=list(p.xi), time=p.time)
 
 
 def interior_values(arr, blocks):

@@ -74,6 +74,78 @@ def test_eta_function_fixture_still_matches_reference_build():
         np.testing.assert_array_equal(O.ref_bssn_points_etafunc(g["inputs"], lam, lf, r0, *pw), ge[f"out_{k}"])
 
 
+# The reference's other builds (rhs.cpp:219-265) in tests/golden/algebra_points_gauge.npz:
+# name -> (eta_func, rochester, ssl, time) with lambda (2, 3, 5, 7), lambda_f (0.7, 0.3),
+# BSSN_ETA_R0 0.9, BSSN_ETA_POWER (4, 1), BSSN_XI (2, 3, 5)
+GAUGE_SETS = {"rochester": (0, 1, 0, 0.0), "rochester_etafunc": (1, 1, 0, 0.0),
+              "ssl": (0, 0, 1, 7.5), "ssl_etafunc": (1, 0, 1, 7.5),
+              "rochester_ssl": (0, 1, 1, 31.0), "rochester_ssl_etafunc": (1, 1, 1, 31.0)}
+
+
+def gauge_params(name, **over):
+    ef, ro, ssl, t = GAUGE_SETS[name]
+    kw = dict(lambda_=(2, 3, 5, 7), lambda_f=(0.7, 0.3), eta_func=ef, eta_r0=0.9, eta_power=(4.0, 1.0),
+              rochester=ro, ssl=ssl, xi=(2, 3, 5), time=t)
+    kw.update(over)
+    return O.default_params(**kw)
+
+
+def gauge_outputs(p, inp):
+    inp = inp.copy()
+    if p.eta_func:
+        inp[:, 162] = O.eta_func_points(inp, p)
+    return O.bssn_points(inp, p)
+
+
+@pytest.mark.parametrize("name", list(GAUGE_SETS))
+def test_gauge_variant_algebra_matches_reference_vectors(name):
+    """The Rochester gauge (bssneqs_*_rochester_gauge*.cpp) and the slow-start
+    lapse (bssneqs_*_SSL.cpp) of the reference, compiled in place: the oracle's
+    24 outputs agree to 1e-13 (B's exactly 0 under the Rochester gauge). Each
+    planted change misses exactly the outputs it touches: the lapse term left
+    out or read at another time (alpha), xi[1] and xi[2] swapped or the
+    standard gauge's shift (beta), B not frozen (B)."""
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    gg = np.load(os.path.join(GOLD, "algebra_points_gauge.npz"))
+    ref = gg[f"out_{name}"]
+    p = gauge_params(name)
+    err = normwise(gauge_outputs(p, g["inputs"]), ref, axis=0)
+    live = np.abs(ref).max(axis=0) > 0
+    assert err[live].max() < 1e-13, err
+    if p.rochester:
+        assert not live[9:12].any()
+        np.testing.assert_array_equal(gauge_outputs(p, g["inputs"])[:, 9:12], 0.0)
+
+    def misses(q, idx):
+        got = gauge_outputs(q, g["inputs"])
+        bad = np.abs(got - ref).max(axis=0) / np.maximum(np.abs(ref).max(axis=0), 1e-300)
+        others = np.delete(bad, idx)
+        return bad[idx].min() > 1e-6 and others.max() < 1e-13
+
+    ef, ro, ssl, t = GAUGE_SETS[name]
+    if ssl:
+        assert misses(gauge_params(name, ssl=0), [0])
+        assert misses(gauge_params(name, time=t + 1.0), [0])
+    if ro:
+        assert misses(gauge_params(name, xi=(2, 5, 3)), [6, 7, 8])
+        q = gauge_params(name, rochester=0)
+        got = gauge_outputs(q, g["inputs"])
+        assert np.abs(got[:, 9:12]).max() > 1e-6 and np.abs(got[:, 6:9] - ref[:, 6:9]).max() > 1e-6
+
+
+def test_gauge_fixture_still_matches_reference_build():
+    if O.ref_lib() is None:
+        pytest.skip("reference checkout not present")
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    gg = np.load(os.path.join(GOLD, "algebra_points_gauge.npz"))
+    for k in GAUGE_SETS:
+        np.testing.assert_array_equal(O.ref_bssn_points_variant(g["inputs"], gauge_params(k)), gg[f"out_{k}"])
+    # the default build through the same entry point is the plain one
+    p0 = O.default_params(lambda_=(2, 3, 5, 7), lambda_f=(0.7, 0.3))
+    np.testing.assert_array_equal(O.ref_bssn_points_variant(g["inputs"], p0),
+                                  O.ref_bssn_points(g["inputs"], (2, 3, 5, 7), (0.7, 0.3)))
+
+
 def test_algebra_fixture_still_matches_reference_build():
     if O.ref_lib() is None:
         pytest.skip("reference checkout not present")
