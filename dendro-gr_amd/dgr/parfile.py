@@ -27,6 +27,9 @@ Keys read, with the reference's behaviour:
   the eta function's constants, grUtils.cpp:205-207)                 optional
   BSSN_RK45_TIME_STEP_SIZE, BSSN_RK45_DESIRED_TOL (parameters.cpp:
   220-223: the first step size and the tolerance of dgr_rk45_step)   optional here
+  BSSN_XI (JSON: BSSN_XI_0.._2, TOML: an array; cast to unsigned like
+  grUtils.cpp:161-163, parameters.cpp:315-317: the Rochester gauge's
+  coefficients)                                                     optional
 A required key that is missing raises KeyError, where the reference's JSON
 reader throws on the null it would convert.
 """
@@ -105,6 +108,15 @@ def read_param_file(path: str | os.PathLike) -> ParamFile:
         if isinstance(pw, dict):
             pw = (pw["BSSN_ETA_POWER_1"], pw["BSSN_ETA_POWER_2"])
         p.eta_power[0], p.eta_power[1] = float(pw[0]), float(pw[1])
+    # the Rochester gauge's coefficients; whether that gauge (USE_ROCHESTER_GAUGE)
+    # or the slow-start lapse (BSSN_ENABLE_SSL_IN_RHS) is used is a build option
+    # there, here params.rochester / params.ssl, left to the caller
+    if "BSSN_XI" in d:
+        xi = d["BSSN_XI"]
+        if isinstance(xi, dict):
+            xi = [xi[f"BSSN_XI_{i}"] for i in range(3)]
+        for i in range(3):
+            p.xi[i] = float(int(xi[i]) & 0xFFFFFFFF)
 
     bh = None
     if "BSSN_BH1" in d and "BSSN_BH2" in d:

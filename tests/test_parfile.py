@@ -99,3 +99,19 @@ def test_rk45_keys_are_read():
     220-223) as the reference's files hold them: 0.01 and 1e-3."""
     for name in ("q1.tinytest.par.json", "q1.tinytest.par.toml", "q2.par.toml"):
         assert parfile.read_param_file(os.path.join(PARS, name)).rk45 == (0.01, 1e-3)
+
+
+def test_xi_is_read_and_cast_to_unsigned(tmp_path):
+    """BSSN_XI (grUtils.cpp:161-163 as an object of _0.._2, parameters.cpp:
+    315-317 as an array; both cast to unsigned int): (0, 0, 0) in the
+    reference's files; other values pass through as the integers the casts
+    give; the Rochester gauge itself stays off (a build option there)."""
+    for name in ("q1.tinytest.par.json", "q1.tinytest.par.toml", "q2.par.toml"):
+        p = parfile.read_param_file(os.path.join(PARS, name)).params
+        assert (list(p.xi), p.rochester, p.ssl) == ([0.0, 0.0, 0.0], 0, 0)
+    with open(os.path.join(PARS, "q1.tinytest.par.json")) as fh:
+        d = json.load(fh)
+    d["BSSN_XI"] = {"BSSN_XI_0": 1, "BSSN_XI_1": 2, "BSSN_XI_2": -1}
+    f = tmp_path / "xi.par.json"
+    f.write_text(json.dumps(d))
+    assert list(parfile.read_param_file(str(f)).params.xi) == [1.0, 2.0, 4294967295.0]
