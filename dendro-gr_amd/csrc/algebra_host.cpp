@@ -8,6 +8,7 @@
 #include <cstdint>
 
 #include "bssn_rhs_generated.h"
+#include "bssn_rhs_gauge_generated.h"
 #include "bssn_ham_generated.h"
 
 namespace {
@@ -85,4 +86,23 @@ extern "C" void dgr_host_bssn_points(const double *in, int64_t n, const double *
 extern "C" void dgr_host_bssn_points_f32(const double *in, int64_t n, const double *lam, const double *lamf,
                                          double *out) {
     run<float>(in, n, lam, lamf, out);
+}
+
+// The generated equations with the gauge variants (Rochester gauge when
+// rochester != 0, the slow-start lapse at time t when ssl != 0, formed here as
+// the device kernels form it: 0.6 sqrt(chi) (alpha - sqrt(chi)) exp(-t^2/800)).
+extern "C" void dgr_host_bssn_points_gauge(const double *in, int64_t n, const double *lam, const double *lamf,
+                                           const double *xi, int rochester, int ssl, double t, double *out) {
+    const double e = std::exp(-1.0 / 800.0 * (t * t));
+    for (int64_t q = 0; q < n; q++) {
+        const double *p = in + 163 * q;
+        In163<double> a{p};
+        Out24<double> b{out + 24 * q};
+        double sslt = 0.0;
+        if (ssl) {
+            const double w = std::sqrt(p[1]);
+            sslt = 0.6 * w * (p[0] - w) * e;
+        }
+        dgr::bssn_rhs_point_gauge<double>(a, p[162], lam, lamf, xi, rochester, sslt, b);
+    }
 }

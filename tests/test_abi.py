@@ -99,6 +99,36 @@ def test_generated_equations_match_reference_vectors(pset):
     assert err.max() < 1e-13, err
 
 
+@pytest.mark.parametrize("name", ["rochester", "rochester_etafunc", "ssl", "ssl_etafunc", "rochester_ssl",
+                                  "rochester_ssl_etafunc"])
+def test_generated_gauge_equations_match_reference_vectors(name):
+    """The generated equations with the gauge variants (bssn_rhs_gauge_generated.h,
+    the program the split path runs and, in phase order, the fused kernel's
+    gauge instantiations) built on the host against the reference's own
+    Rochester-gauge and slow-start-lapse builds (tests/golden/
+    algebra_points_gauge.npz); with the eta function, in[162] holds the
+    oracle's eta of chi's gradient (pinned to the reference's eta-function build)."""
+    import oracle as O
+    from test_oracle_golden import gauge_params
+    g = np.load(os.path.join(GOLD, "algebra_points.npz"))
+    ref = np.load(os.path.join(GOLD, "algebra_points_gauge.npz"))[f"out_{name}"]
+    p = gauge_params(name)
+    inp = np.ascontiguousarray(g["inputs"].copy())
+    if p.eta_func:
+        inp[:, 162] = O.eta_func_points(inp, p)
+    out = np.zeros((len(inp), 24))
+    la, lff, xi = (np.array(list(v), float) for v in (p.lambda_, p.lambda_f, p.xi))
+    dp = C.POINTER(C.c_double)
+    L = _host_lib()
+    L.dgr_host_bssn_points_gauge.argtypes = [dp, C.c_int64, dp, dp, dp, C.c_int, C.c_int, C.c_double, dp]
+    L.dgr_host_bssn_points_gauge(inp.ctypes.data_as(dp), len(inp), la.ctypes.data_as(dp), lff.ctypes.data_as(dp),
+                                 xi.ctypes.data_as(dp), p.rochester, p.ssl, p.time, out.ctypes.data_as(dp))
+    live = np.abs(ref).max(0) > 0
+    err = np.abs(out - ref).max(0)[live] / np.abs(ref).max(0)[live]
+    assert err.max() < 1e-13, err
+    np.testing.assert_array_equal(out[:, ~live], 0.0)
+
+
 def test_generated_ham_matches_reference_constraint_vectors():
     g = np.load(os.path.join(GOLD, "algebra_points.npz"))
     ref = np.load(os.path.join(GOLD, "ham_points.npz"))["ham"]
