@@ -674,6 +674,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tiling instead of timing the tilings per block size")
+    ap.add_argument("--gauge", default="",
+                    help="comma list of etafunc, rochester, ssl: time the right-hand side of the "
+                         "reference's other builds instead of the default one (FP64; use --no-mixed)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (3 times), to scale "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
@@ -700,6 +703,13 @@ def main():
     stream = torch.cuda.current_stream()
     composite = args.workload == "octree"
     wl = Workload("puncture" if composite else args.workload, args.nblocks, args.scale)
+    if args.gauge:
+        # the reference's other builds of the equations (their own kernels); the
+        # traffic profiles are of the default kernels, so none is looked up
+        names = args.gauge.split(",")
+        wl.params.update(eta_func=int("etafunc" in names), rochester=int("rochester" in names),
+                         ssl=int("ssl" in names), xi=(1.0, 1.0, 1.0))
+        wl.key += "_" + "_".join(names)
     if wl.name in ("lattice13", "refined"):  # the comparisons are for the frozen-padding step
         args.no_mixed = args.no_cpu_baseline = True
     want_mixed = args.precision == "fp64" and not args.no_mixed
