@@ -141,7 +141,9 @@ class Workload:
 # used only when it was measured on a build of these exact files
 KERNEL_SOURCES = ("dendro-gr_amd/csrc/dgr_bssn.hip", "dendro-gr_amd/csrc/stencils.h",
                   "dendro-gr_amd/csrc/bssn_rhs_phased.h", "dendro-gr_amd/csrc/bssn_rhs_generated.h",
-                  "dendro-gr_amd/csrc/bssn_ham_generated.h", "dendro-gr_amd/build.py")
+                  "dendro-gr_amd/csrc/bssn_ham_generated.h", "dendro-gr_amd/csrc/bssn_cons_phased.h",
+                  "dendro-gr_amd/csrc/bssn_rhs_phased_gauge.h", "dendro-gr_amd/csrc/bssn_rhs_gauge_generated.h",
+                  "dendro-gr_amd/build.py")
 
 
 def source_sha():

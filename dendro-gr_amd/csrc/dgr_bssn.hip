@@ -118,6 +118,7 @@ struct RhsParams {
     int ssl;                         // the slow-start lapse term (BSSN_ENABLE_SSL_IN_RHS)
     double xi[3];                    // BSSN_XI (the Rochester gauge's shift coefficients)
     double ssl_e;                    // exp(-t^2/800) at the time the call evaluates at
+    const double *ssl_dev;           // if set, {t, exp(-t^2/800)} in device memory: read [1] instead
 };
 
 // The right-hand side of one of the reference's other builds (eta function,
@@ -1193,7 +1194,7 @@ __device__ __forceinline__ void fused_tile(const StageIO &io, const RhsParams &p
     if constexpr (P::GAUGE) {
         ctx.eta_fn = prm.eta_func != 0;
         ctx.ssl_on = prm.ssl != 0;
-        ctx.ssl_e = prm.ssl_e;
+        ctx.ssl_e = prm.ssl_dev ? prm.ssl_dev[1] : prm.ssl_e;
         ctx.ssl_v = T(0);
     }
     const T lam[4] = {T(prm.lam[0]), T(prm.lam[1]), T(prm.lam[2]), T(prm.lam[3])};
@@ -1918,6 +1919,12 @@ __global__ __launch_bounds__(NTHREADS) void k_rk45_restore(VarPtrs u, VarPtrs u0
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+// The slow-start lapse's time in a captured RK4 step: at the step's start the
+// factor exp(-t^2/800) of the time d[0] goes to d[1]; at its end the time moves
+// on by dt (rkBSSN.cpp:1087).
+__global__ void k_ssl_begin(double *d) { d[1] = exp(-1.0 / 800.0 * (d[0] * d[0])); }
+__global__ void k_ssl_advance(double *d, double dt) { d[0] += dt; }
+
 // The range of one variable over every interior point and the number of
 // non-finite values there (the reference's check after each step,
 // BSSNCtx::terminal_output, bssnCtx.cpp:1240-1262, takes the min and max of
@@ -2091,6 +2098,7 @@ struct dgr_plan {
     double *d_istage = nullptr; // k_fused RK4: 2 * 24 * interior (u0 copy, accumulator)
     double *d_rk45 = nullptr;   // dgr_rk45_step: the six stages' right-hand sides, 6 * 24 * interior
     unsigned long long *d_rk45_emax = nullptr;  // dgr_rk45_step: the error estimate's bits
+    double *d_ssl = nullptr;    // dgr_rk4_steps with the slow-start lapse: {time, exp(-time^2/800)}
     unsigned long long *d_range = nullptr;      // dgr_variable_range: min key, max key, non-finite count
     double *d_wstate = nullptr; // k_fused RK4: 24 * szpdof, the padded work state W
     int *d_err = nullptr;
@@ -2555,6 +2563,7 @@ int dgr_plan_destroy(dgr_plan *plan) {
     (void)hipFree(plan->d_istage);
     (void)hipFree(plan->d_rk45);
     (void)hipFree(plan->d_rk45_emax);
+    (void)hipFree(plan->d_ssl);
     (void)hipFree(plan->d_range);
     (void)hipFree(plan->d_wstate);
     (void)hipFree(plan->d_nbr);
@@ -2669,6 +2678,7 @@ RhsParams to_params(const dgr_bssn_params *p, double t) {
     r.ssl = p->ssl != 0;
     for (int i = 0; i < 3; i++) r.xi[i] = p->xi[i];
     r.ssl_e = r.ssl ? std::exp(-1.0 / 800.0 * (t * t)) : 0.0;
+    r.ssl_dev = nullptr;
     return r;
 }
 
@@ -3483,17 +3493,23 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
     VarPtrs u;
     int rc;
     if ((rc = load_ptrs(u, d_u, "d_u"))) return rc;
-    // per-launch timing and the split path run the steps eagerly, and so does
-    // the slow-start lapse, whose time moves on by dt per step (rkBSSN.cpp:1087)
-    if (!plan->fused || plan->timing || params->ssl) {
+    // per-launch timing and the split path run the steps eagerly (the
+    // slow-start lapse's time moving on by dt per step, rkBSSN.cpp:1087)
+    if (!plan->fused || plan->timing) {
         dgr_bssn_params p = *params;
         for (int i = 0; i < nsteps; i++, p.time += dt)
             if ((rc = dgr_rk4_step(plan, d_u, dt, &p, enforce_per_stage, stream))) return rc;
         return DGR_OK;
     }
-    // without the slow-start lapse the time is not read: a new time keeps the graph
+    // the captured step reads the slow-start lapse's time from device memory,
+    // where the graph moves it on: a new time keeps the graph
     dgr_bssn_params kp = *params;
     kp.time = 0.0;
+    const bool ssl = params->ssl != 0;
+    if (ssl && !plan->d_ssl) {
+        hipError_t e = hipMalloc(&plan->d_ssl, 2 * sizeof(double));
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? DGR_E_NOMEM : DGR_E_HIP, "lapse time");
+    }
     HIP_OK(hipSetDevice(plan->device));
     if ((rc = ensure_istage(plan))) return rc;
     auto &key = plan->rk4_key;
@@ -3509,7 +3525,13 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         }
         if (!plan->cap_stream) HIP_OK(hipStreamCreateWithFlags(&plan->cap_stream, hipStreamNonBlocking));
         HIP_OK(hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal));
-        rc = rk4_fused_launches(plan, plan_rk_buffers(plan), u, dt, to_params(params), enf, plan->cap_stream);
+        RhsParams prm = to_params(params);
+        if (ssl) {
+            prm.ssl_dev = plan->d_ssl;
+            hipLaunchKernelGGL(k_ssl_begin, dim3(1), dim3(1), 0, plan->cap_stream, plan->d_ssl);
+        }
+        rc = rk4_fused_launches(plan, plan_rk_buffers(plan), u, dt, prm, enf, plan->cap_stream);
+        if (ssl) hipLaunchKernelGGL(k_ssl_advance, dim3(1), dim3(1), 0, plan->cap_stream, plan->d_ssl, dt);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(plan->cap_stream, &g);
         if (rc) {
@@ -3529,6 +3551,12 @@ int dgr_rk4_steps(dgr_plan *plan, double *const *d_u, double dt, const dgr_bssn_
         key.enforce = enf;
         key.halo = plan->halo;
         key.fill_general = plan->fill_general;
+    }
+    if (nsteps > 0 && ssl) {
+        unsigned long long bits;
+        std::memcpy(&bits, &params->time, sizeof(bits));
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, (hipStream_t)stream,
+                           reinterpret_cast<unsigned long long *>(plan->d_ssl), bits);
     }
     if (nsteps > 0 && ((plan->halo && !halo_in_kernel(plan) && (rc = launch_halo_fill(plan, u, (hipStream_t)stream))) ||
                        (rc = launch_pad_copy(plan, plan_rk_buffers(plan), u, (hipStream_t)stream))))

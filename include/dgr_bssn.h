@@ -89,7 +89,8 @@ typedef struct dgr_bssn_params {
     double time;            /* the t the slow-start lapse reads in dgr_rk3_step, dgr_rk4_step(s)
                                and dgr_rk45_step: the step's start time at every stage, as
                                rkBSSN.cpp:694-836 passes current_t; dgr_rk4_steps adds dt per
-                               step. dgr_bssn_rhs reads its t argument instead. */
+                               step (on the device, inside its captured graph). dgr_bssn_rhs
+                               reads its t argument instead. */
 } dgr_bssn_params;
 
 typedef struct dgr_plan dgr_plan;
