@@ -154,3 +154,32 @@ def test_time_does_not_matter_without_the_lapse_term():
         res.append(u.cpu().numpy())
     plan.close()
     np.testing.assert_array_equal(res[0], res[1])
+
+
+def test_lapse_term_time_is_set_per_call_of_the_graph():
+    """dgr_rk4_steps with the slow-start lapse replays one captured graph whose
+    time lives in device memory: two calls of one step each, at time and
+    time + dt, give the same bits as one call of two steps."""
+    import torch
+    t = uneven_sample()
+    st = _state(t)
+    dt = 0.005
+    plan = dgr.Plan(t, st.shape[1])
+    u2 = to_dev(st)
+    plan.rk4_steps(u2, dt, _params("ssl", time=5.0), nsteps=2)
+    u1 = to_dev(st)
+    plan.rk4_steps(u1, dt, _params("ssl", time=5.0), nsteps=1)
+    plan.rk4_steps(u1, dt, _params("ssl", time=5.0 + dt), nsteps=1)
+    torch.cuda.synchronize()
+    assert plan.status() == 0
+    a, b = u1.cpu().numpy(), u2.cpu().numpy()
+    plan.close()
+    np.testing.assert_array_equal(interior_values(a, t), interior_values(b, t))
+    # and a call at another time gives other bits
+    plan = dgr.Plan(t, st.shape[1])
+    u3 = to_dev(st)
+    plan.rk4_steps(u3, dt, _params("ssl", time=9.0), nsteps=2)
+    torch.cuda.synchronize()
+    c = u3.cpu().numpy()
+    plan.close()
+    assert np.abs(interior_values(c, t) - interior_values(b, t)).max() > 0
